@@ -1,0 +1,257 @@
+"""ctypes binding of libdvc_amd.so (include/dvc_amd.h).
+
+The library is the product path: every call below runs HIP kernels on a gfx950
+device.  There is no CPU fallback -- a missing library or device raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libdvc_amd.so"
+
+POINT_GOOD, RANGE_FAIL, CONVG_FAIL, NOT_SEARCH = 0, -1, -2, -3
+OBJ = {"sad": 0, "ssd": 1, "zssd": 2, "nssd": 3, "znssd": 4}
+INTERP = {"nearest": 0, "trilinear": 1, "tricubic": 2}
+GEOM = {"cube": 0, "sphere": 1}
+
+ERRORS = {0: "ok", -1: "invalid argument", -2: "HIP failure", -3: "bad call order",
+          -4: "dependency wait timed out", -5: "no gfx950 device", -6: "outside the interpolation box"}
+ERR_RANGE = -6
+
+
+class RunParams(C.Structure):
+    """dvc_run_params: the RunControl fields of Core/Utility.h:90-134 the path reads."""
+    _fields_ = [
+        ("ndof", C.c_int32), ("obj", C.c_int32), ("interp", C.c_int32), ("geom", C.c_int32),
+        ("subvol_size", C.c_int32), ("subvol_npts", C.c_int32), ("disp_max", C.c_int32),
+        ("sphere_seed", C.c_uint32), ("basin_radius", C.c_double),
+        ("rigid_trans", C.c_double * 3), ("aspect", C.c_double * 3),
+    ]
+
+
+class PointResult(C.Structure):
+    """dvc_point_result: ResultRecord (Core/DataCloud.h:45-56) plus iterations."""
+    _fields_ = [("status", C.c_int32), ("iters", C.c_int32), ("obj", C.c_double),
+                ("par", C.c_double * 12)]
+
+
+RESULT_DTYPE = np.dtype([("status", np.int32), ("iters", np.int32), ("obj", np.float64),
+                         ("par", np.float64, (12,))])
+assert RESULT_DTYPE.itemsize == C.sizeof(PointResult)
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load libdvc_amd.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"{LIB_PATH} is missing: run `python -m digitalvolumecorrelation_amd.build`")
+    lib = C.CDLL(str(LIB_PATH))
+    P, I, D, VP = C.POINTER, C.c_int32, C.c_double, C.c_void_p
+    sig = {
+        "dvc_create": (I, [P(VP), C.c_int]),
+        "dvc_destroy": (None, [VP]),
+        "dvc_last_error": (C.c_char_p, [VP]),
+        "dvc_set_stream": (I, [VP, VP]),
+        "dvc_last_kernel_ms": (I, [VP, P(C.c_float)]),
+        "dvc_volume_upload": (I, [VP, C.c_int, VP, C.c_int64, C.c_int, C.c_int, C.c_int64, C.c_int64, C.c_int64]),
+        "dvc_volume_attach": (I, [VP, C.c_int, VP, C.c_int, C.c_int64, C.c_int64, C.c_int64]),
+        "dvc_interpolate": (I, [VP, C.c_int, P(RunParams), P(D), P(D), I, P(D), C.c_int, P(D), P(D)]),
+        "dvc_objective": (I, [VP, C.c_int, P(D), P(D), I, P(D), P(D)]),
+        "dvc_search_points": (I, [VP, P(RunParams), P(D), I, P(I), I, P(I), I, VP]),
+        "dvc_search_prepare": (I, [VP, P(RunParams), P(D), I, P(I), I, P(I), I]),
+        "dvc_search_execute": (I, [VP]),
+        "dvc_search_fetch": (I, [VP, VP]),
+        "dvc_set_block_threads": (I, [VP, C.c_int]),
+        "dvc_set_wait_limit": (I, [VP, C.c_double]),
+        "dvc_strain_fit": (I, [VP, P(D), P(D), P(I), P(D), P(I), I, I, I, D, I, I,
+                               P(D), P(I), P(D), P(D), P(D), P(D)]),
+        "dvc_sort_order_neighbors": (I, [P(D), I, P(D), P(I), P(I), I]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def exported_symbols() -> list[str]:
+    """Entry points declared in include/dvc_amd.h."""
+    hdr = Path(__file__).resolve().parent.parent / "include" / "dvc_amd.h"
+    import re
+    return sorted(set(re.findall(r"^\s*(?:int|void|const char)\s*\*?\s*(dvc_\w+)\s*\(", hdr.read_text(), re.M)))
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def make_params(*, ndof=3, obj="znssd", interp="tricubic", geom="cube", subvol_size=31,
+                subvol_npts=2000, disp_max=5, sphere_seed=1, basin_radius=0.0,
+                rigid_trans=(0.0, 0.0, 0.0), aspect=(1.0, 1.0, 1.0)) -> RunParams:
+    p = RunParams()
+    p.ndof = ndof
+    p.obj = OBJ[obj] if isinstance(obj, str) else int(obj)
+    p.interp = INTERP[interp] if isinstance(interp, str) else int(interp)
+    p.geom = GEOM[geom] if isinstance(geom, str) else int(geom)
+    p.subvol_size = subvol_size
+    p.subvol_npts = subvol_npts
+    p.disp_max = disp_max
+    p.sphere_seed = sphere_seed
+    p.basin_radius = basin_radius
+    for q in range(3):
+        p.rigid_trans[q] = rigid_trans[q]
+        p.aspect[q] = aspect[q]
+    return p
+
+
+class DvcError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Context:
+    """One device context (dvc_create / dvc_destroy)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load()
+        h = C.c_void_p()
+        rc = self.lib.dvc_create(C.byref(h), device)
+        if rc != 0:
+            raise DvcError(rc, "dvc_create failed (a gfx950 GPU is required)")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.dvc_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int, allow=()):
+        if rc != 0 and rc not in allow:
+            raise DvcError(rc, self.lib.dvc_last_error(self.h).decode())
+        return rc
+
+    # ---- volumes
+    def upload(self, slot: int, vol: np.ndarray, big_endian: bool = False, header: bytes = b""):
+        vol = np.ascontiguousarray(vol)
+        if vol.dtype not in (np.uint8, np.uint16):
+            raise ValueError("volumes are uint8 or uint16")
+        t, h, w = vol.shape
+        raw = np.frombuffer(header + vol.tobytes(), dtype=np.uint8) if header else vol.view(np.uint8).reshape(-1)
+        self._check(self.lib.dvc_volume_upload(self.h, slot, raw.ctypes.data_as(C.c_void_p), len(header),
+                                               vol.dtype.itemsize, int(big_endian), w, h, t))
+
+    def attach(self, slot: int, dev_ptr: int, bytes_per: int, shape):
+        t, h, w = shape
+        self._check(self.lib.dvc_volume_attach(self.h, slot, C.c_void_p(dev_ptr), bytes_per, w, h, t))
+
+    # ---- interpolation / objective
+    def interpolate(self, slot, params, center, xyz, bbox, method=2, grad=False):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        n = xyz.shape[0]
+        out = np.zeros(n)
+        g = np.zeros((n, 3)) if grad else None
+        c = np.ascontiguousarray(center, dtype=np.float64)
+        b = np.ascontiguousarray(bbox, dtype=np.float64)
+        rc = self.lib.dvc_interpolate(self.h, slot, C.byref(params), _dp(c), _dp(xyz), n, _dp(b), method,
+                                      _dp(out), _dp(g) if grad else None)
+        self._check(rc, allow=(ERR_RANGE,))
+        if rc == ERR_RANGE:
+            return None if not grad else (None, None)
+        return (out, g) if grad else out
+
+    def objective(self, kind, ref, tar, residual=False):
+        r = np.ascontiguousarray(ref, dtype=np.float64)
+        t = np.ascontiguousarray(tar, dtype=np.float64)
+        o = C.c_double()
+        res = np.zeros(len(r)) if residual else None
+        self._check(self.lib.dvc_objective(self.h, OBJ[kind] if isinstance(kind, str) else kind, _dp(r), _dp(t),
+                                           len(r), C.byref(o), _dp(res) if residual else None))
+        return (o.value, res) if residual else o.value
+
+    # ---- search
+    def set_block_threads(self, n: int):
+        self._check(self.lib.dvc_set_block_threads(self.h, n))
+
+    def set_wait_limit(self, seconds: float):
+        self._check(self.lib.dvc_set_wait_limit(self.h, seconds))
+
+    def prepare(self, params, points, order, neigh, nproc=0):
+        self._pts = np.ascontiguousarray(points, dtype=np.float64)
+        self._ord = np.ascontiguousarray(order, dtype=np.int32)
+        self._nb = np.ascontiguousarray(neigh, dtype=np.int32)
+        self._npts = self._pts.shape[0]
+        self._check(self.lib.dvc_search_prepare(self.h, C.byref(params), _dp(self._pts), self._npts, _ip(self._ord),
+                                                nproc, _ip(self._nb), self._nb.shape[1]))
+
+    def execute(self):
+        self._check(self.lib.dvc_search_execute(self.h))
+
+    def fetch(self) -> np.ndarray:
+        res = np.zeros(self._npts, dtype=RESULT_DTYPE)
+        self._check(self.lib.dvc_search_fetch(self.h, res.ctypes.data_as(C.c_void_p)))
+        return res
+
+    def search(self, params, points, order, neigh, nproc=0) -> np.ndarray:
+        self.prepare(params, points, order, neigh, nproc)
+        self.execute()
+        return self.fetch()
+
+    def kernel_ms(self) -> float:
+        ms = C.c_float()
+        self._check(self.lib.dvc_last_kernel_ms(self.h, C.byref(ms)))
+        return ms.value
+
+    # ---- strain
+    def strain(self, pts, disp, status, objmin, neigh, ndp=25, objt=1.0, refill=False, planar=0):
+        pts = np.ascontiguousarray(pts, dtype=np.float64)
+        disp = np.ascontiguousarray(disp, dtype=np.float64)
+        st = np.ascontiguousarray(status, dtype=np.int32)
+        om = np.ascontiguousarray(objmin, dtype=np.float64)
+        nb = np.ascontiguousarray(neigh, dtype=np.int32)
+        n, k = nb.shape
+        o = {"disp": np.zeros((n, 3)), "npts": np.zeros(n, np.int32), "swr": np.zeros(n),
+             "E": np.zeros((n, 9)), "L": np.zeros((n, 9)), "DG": np.zeros((n, 9))}
+        self._check(self.lib.dvc_strain_fit(self.h, _dp(pts), _dp(disp), _ip(st), _dp(om), _ip(nb), k, n, ndp,
+                                            float(objt), int(refill), int(planar), _dp(o["disp"]), _ip(o["npts"]),
+                                            _dp(o["swr"]), _dp(o["E"]), _dp(o["L"]), _dp(o["DG"])))
+        return o
+
+
+def sort_order_neighbors(points: np.ndarray, start=None, nthreads: int = 0):
+    """DataCloud::sort_order_neighbors on the host (no device needed)."""
+    lib = load()
+    p = np.ascontiguousarray(points, dtype=np.float64)
+    n = p.shape[0]
+    s = np.ascontiguousarray(p[0] if start is None else start, dtype=np.float64)
+    k = min(75, n)
+    order = np.zeros(n, np.int32)
+    neigh = np.zeros((n, k), np.int32)
+    rc = lib.dvc_sort_order_neighbors(_dp(p), n, _dp(s), _ip(order), _ip(neigh), nthreads)
+    if rc != k:
+        raise DvcError(rc, "sort_order_neighbors failed")
+    return order, neigh

@@ -1,0 +1,592 @@
+// dvc_api.cpp -- C ABI of libdvc_amd.so (declared in include/dvc_amd.h).
+//
+// Host responsibilities: the device context, HBM-resident volumes, the FP64 geometry
+// that Search::Search and Interpolate::Interpolate fix once per run
+// (Core/Search.cpp:24-82, Core/Interpolate.cpp:34-58), the sphere sample template
+// (Core/FloatingCloud.cpp:40-82), launching the kernels of dvc_kernels.hip, and the
+// host-side DataCloud ordering (Core/DataCloud.cpp:69-144).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <random>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "dvc_internal.h"
+
+using namespace dvc;
+
+namespace {
+
+// Point / BoundBox arithmetic of Core/Point.cpp:57-66 and Core/BoundBox.cpp.
+inline int trunc_i(double v) { return (int)(v / 1); }
+
+struct HBox {
+    double mn[3], mx[3], cn[3];
+    HBox(const double a[3], const double b[3]) {
+        for (int q = 0; q < 3; q++) { mn[q] = a[q]; mx[q] = b[q]; cn[q] = (a[q] + b[q]) / 2.0; }
+    }
+    void grow_by(double f) {
+        for (int q = 0; q < 3; q++) { mn[q] = mn[q] + -f; mx[q] = mx[q] + f; }
+        for (int q = 0; q < 3; q++) cn[q] = (mn[q] + mx[q]) / 2.0;
+    }
+    void center_on(const double c[3]) {
+        for (int q = 0; q < 3; q++) {
+            double d = c[q] - cn[q];
+            mn[q] = mn[q] + d; mx[q] = mx[q] + d; cn[q] = cn[q] + d;
+        }
+    }
+};
+
+struct Volume {
+    uint8_t *d = nullptr;
+    bool owned = false;
+    int bpv = 0;
+    int64_t W = 0, H = 0, T = 0;
+};
+
+// Run geometry derived from dvc_run_params (Search.cpp:28-80).
+struct Geometry {
+    double subv_rad;
+    int subv_num, g;
+    double est_mn[3], est_mx[3], est_cn[3];
+    double act_mn[3], act_mx[3], act_cn[3];
+    int box[3];
+};
+
+Geometry make_geometry(const dvc_run_params &rc) {
+    Geometry G;
+    G.subv_rad = rc.subvol_size / 2.0;
+    if (rc.geom == DVC_GEOM_CUBE) G.subv_num = (int)pow(ceil(pow((double)rc.subvol_npts, 1.0 / 3.0)), 3.0);
+    else G.subv_num = rc.subvol_npts;
+    G.g = (int)ceil(pow((double)G.subv_num, 1.0 / 3.0));
+    double z[3] = {0.0, 0.0, 0.0};
+    double m[3] = {2 * G.subv_rad * rc.aspect[0], 2 * G.subv_rad * rc.aspect[1], 2 * G.subv_rad * rc.aspect[2]};
+    HBox est(z, m);
+    est.grow_by(rc.disp_max);
+    est.grow_by(2.0);
+    HBox e2(est.mn, est.mx);                     // Interpolate.cpp:51
+    HBox act(est.mn, est.mx);                    // Interpolate.cpp:52-55
+    act.grow_by(-1.0);
+    for (int q = 0; q < 3; q++) {
+        G.est_mn[q] = e2.mn[q]; G.est_mx[q] = e2.mx[q]; G.est_cn[q] = e2.cn[q];
+        G.act_mn[q] = act.mn[q]; G.act_mx[q] = act.mx[q]; G.act_cn[q] = act.cn[q];
+        G.box[q] = trunc_i(e2.mx[q]) - trunc_i(e2.mn[q]);
+    }
+    return G;
+}
+
+}  // namespace
+
+struct dvc_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipStream_t own_stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err;
+    Volume vol[2];
+    int cus = 0;
+    size_t lds_max = 0;
+    int wall_khz = 0;
+    int block_threads = 0;
+    double wait_seconds = 30.0;
+    float last_ms = 0.0f;
+    // search plan
+    bool plan_ready = false;
+    dvc_run_params rc{};
+    Geometry geo{};
+    int npts = 0, nproc = 0, k = 0;
+    double *d_pts = nullptr, *d_sph = nullptr;
+    int *d_order = nullptr, *d_pos = nullptr, *d_neigh = nullptr, *d_done = nullptr, *d_ctrl = nullptr;
+    dvc_point_result *d_res = nullptr;
+    double off_min[3]{}, off_max[3]{};
+    int chosen_threads = 256;
+    std::vector<char> searched;
+};
+
+namespace {
+
+int fail(dvc_ctx *c, int code, const std::string &msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+int hip_fail(dvc_ctx *c, hipError_t e, const char *what) {
+    return fail(c, DVC_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIPCHK(call)                                        \
+    do {                                                    \
+        hipError_t e_ = (call);                             \
+        if (e_ != hipSuccess) return hip_fail(ctx, e_, #call); \
+    } while (0)
+
+template <typename T>
+int dev_alloc(dvc_ctx *ctx, T **p, size_t count) {
+    if (*p) { hipFree(*p); *p = nullptr; }
+    if (count == 0) count = 1;
+    HIPCHK(hipMalloc((void **)p, count * sizeof(T)));
+    return 0;
+}
+
+void free_plan(dvc_ctx *c) {
+    hipFree(c->d_pts); hipFree(c->d_sph); hipFree(c->d_order); hipFree(c->d_pos);
+    hipFree(c->d_neigh); hipFree(c->d_done); hipFree(c->d_ctrl); hipFree(c->d_res);
+    c->d_pts = c->d_sph = nullptr;
+    c->d_order = c->d_pos = c->d_neigh = c->d_done = c->d_ctrl = nullptr;
+    c->d_res = nullptr;
+    c->plan_ready = false;
+}
+
+int check_params(dvc_ctx *ctx, const dvc_run_params *rc) {
+    if (!rc) return fail(ctx, DVC_ERR_ARG, "null run parameters");
+    if (rc->ndof != 3 && rc->ndof != 6 && rc->ndof != 12) return fail(ctx, DVC_ERR_ARG, "ndof must be 3, 6 or 12");
+    if (rc->obj < 0 || rc->obj > 4) return fail(ctx, DVC_ERR_ARG, "unknown objective function");
+    if (rc->interp < 0 || rc->interp > 2) return fail(ctx, DVC_ERR_ARG, "unknown interpolation type");
+    if (rc->geom != DVC_GEOM_CUBE && rc->geom != DVC_GEOM_SPHERE) return fail(ctx, DVC_ERR_ARG, "unknown subvolume geometry");
+    if (rc->subvol_size < 1) return fail(ctx, DVC_ERR_ARG, "subvol_size must be >= 1");
+    if (rc->subvol_npts < 1 || rc->subvol_npts > 50000) return fail(ctx, DVC_ERR_ARG, "subvol_npts must be in 1..50000");
+    if (rc->disp_max < 0) return fail(ctx, DVC_ERR_ARG, "disp_max must be >= 0");
+    if (!(rc->basin_radius >= 0.0) || rc->basin_radius > rc->disp_max) return fail(ctx, DVC_ERR_ARG, "basin_radius must be in 0..disp_max");
+    for (int q = 0; q < 3; q++)
+        if (!(rc->aspect[q] >= 0.1 && rc->aspect[q] <= 10.0)) return fail(ctx, DVC_ERR_ARG, "subvol_aspect must be in 0.1..10");
+    if (rc->geom == DVC_GEOM_CUBE) {
+        int g = (int)ceil(pow((double)rc->subvol_npts, 1.0 / 3.0));
+        if (g < 2) return fail(ctx, DVC_ERR_ARG, "a cube needs at least 2 samples per side");
+    }
+    return 0;
+}
+
+int check_volumes(dvc_ctx *ctx) {
+    if (!ctx->vol[0].d || !ctx->vol[1].d) return fail(ctx, DVC_ERR_STATE, "both volumes must be uploaded first");
+    const Volume &a = ctx->vol[0], &b = ctx->vol[1];
+    if (a.bpv != b.bpv || a.W != b.W || a.H != b.H || a.T != b.T)
+        return fail(ctx, DVC_ERR_ARG, "reference and correlated volumes differ in size or depth");
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dvc_create(dvc_ctx **out, int device) {
+    if (!out) return DVC_ERR_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= device || device < 0) return DVC_ERR_NODEV;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return DVC_ERR_NODEV;
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return DVC_ERR_NODEV;
+    dvc_ctx *ctx = new dvc_ctx();
+    ctx->device = device;
+    if (hipSetDevice(device) != hipSuccess) { delete ctx; return DVC_ERR_NODEV; }
+    ctx->cus = prop.multiProcessorCount;
+    ctx->lds_max = prop.sharedMemPerBlock;
+    int khz = 0;
+    hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device);
+    ctx->wall_khz = khz > 0 ? khz : 100000;
+    if (hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
+        delete ctx;
+        return DVC_ERR_HIP;
+    }
+    ctx->stream = ctx->own_stream;
+    *out = ctx;
+    return DVC_OK;
+}
+
+void dvc_destroy(dvc_ctx *ctx) {
+    if (!ctx) return;
+    hipSetDevice(ctx->device);
+    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    free_plan(ctx);
+    for (int s = 0; s < 2; s++)
+        if (ctx->vol[s].owned) hipFree(ctx->vol[s].d);
+    if (ctx->ev0) hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) hipEventDestroy(ctx->ev1);
+    if (ctx->own_stream) hipStreamDestroy(ctx->own_stream);
+    delete ctx;
+}
+
+const char *dvc_last_error(const dvc_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int dvc_set_stream(dvc_ctx *ctx, void *s) {
+    if (!ctx) return DVC_ERR_ARG;
+    ctx->stream = s ? (hipStream_t)s : ctx->own_stream;
+    return DVC_OK;
+}
+
+int dvc_last_kernel_ms(const dvc_ctx *ctx, float *ms) {
+    if (!ctx || !ms) return DVC_ERR_ARG;
+    *ms = ctx->last_ms;
+    return DVC_OK;
+}
+
+int dvc_set_block_threads(dvc_ctx *ctx, int threads) {
+    if (!ctx) return DVC_ERR_ARG;
+    if (threads != 0 && threads != 64 && threads != 128 && threads != 256 && threads != 512 && threads != 1024)
+        return fail(ctx, DVC_ERR_ARG, "threads per point must be 0, 64, 128, 256, 512 or 1024");
+    ctx->block_threads = threads;
+    return DVC_OK;
+}
+
+int dvc_set_wait_limit(dvc_ctx *ctx, double seconds) {
+    if (!ctx || !(seconds > 0.0)) return DVC_ERR_ARG;
+    ctx->wait_seconds = seconds;
+    return DVC_OK;
+}
+
+int dvc_volume_upload(dvc_ctx *ctx, int slot, const void *host, int64_t header, int bpv, int big_endian,
+                      int64_t w, int64_t h, int64_t t) {
+    if (!ctx) return DVC_ERR_ARG;
+    if (slot != 0 && slot != 1) return fail(ctx, DVC_ERR_ARG, "slot must be DVC_VOL_REF or DVC_VOL_COR");
+    if (!host || (bpv != 1 && bpv != 2) || w < 4 || h < 4 || t < 4 || header < 0)
+        return fail(ctx, DVC_ERR_ARG, "invalid volume description");
+    if (w > 2147483647LL / 2 || h > 2147483647LL || t > 2147483647LL) return fail(ctx, DVC_ERR_ARG, "volume too large");
+    HIPCHK(hipSetDevice(ctx->device));
+    Volume &v = ctx->vol[slot];
+    if (v.owned) hipFree(v.d);
+    v = Volume();
+    size_t bytes = (size_t)w * h * t * bpv;
+    HIPCHK(hipMalloc((void **)&v.d, bytes + 64));
+    v.owned = true;
+    v.bpv = bpv; v.W = w; v.H = h; v.T = t;
+    HIPCHK(hipMemsetAsync(v.d + bytes, 0, 64, ctx->stream));
+    HIPCHK(hipMemcpyAsync(v.d, (const uint8_t *)host + header, bytes, hipMemcpyHostToDevice, ctx->stream));
+    if (bpv == 2 && big_endian)
+        if (launch_swap16((uint16_t *)v.d, (long long)w * h * t, ctx->stream) != 0) return fail(ctx, DVC_ERR_HIP, "byte swap launch failed");
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return DVC_OK;
+}
+
+int dvc_volume_attach(dvc_ctx *ctx, int slot, const void *dptr, int bpv, int64_t w, int64_t h, int64_t t) {
+    if (!ctx) return DVC_ERR_ARG;
+    if (slot != 0 && slot != 1) return fail(ctx, DVC_ERR_ARG, "slot must be DVC_VOL_REF or DVC_VOL_COR");
+    if (!dptr || (bpv != 1 && bpv != 2) || w < 4 || h < 4 || t < 4) return fail(ctx, DVC_ERR_ARG, "invalid volume description");
+    if (((uintptr_t)dptr & 3) != 0) return fail(ctx, DVC_ERR_ARG, "device volume must be 4-byte aligned");
+    Volume &v = ctx->vol[slot];
+    if (v.owned) hipFree(v.d);
+    v = Volume();
+    v.d = (uint8_t *)dptr; v.owned = false; v.bpv = bpv; v.W = w; v.H = h; v.T = t;
+    return DVC_OK;
+}
+
+int dvc_interpolate(dvc_ctx *ctx, int slot, const dvc_run_params *rc, const double center[3], const double *xyz,
+                    int32_t n, const double bbox[6], int method, double *out, double *grad) {
+    if (!ctx) return DVC_ERR_ARG;
+    if (int e = check_params(ctx, rc)) return e;
+    if (slot != 0 && slot != 1) return fail(ctx, DVC_ERR_ARG, "bad slot");
+    const Volume &v = ctx->vol[slot];
+    if (!v.d) return fail(ctx, DVC_ERR_STATE, "volume not uploaded");
+    if (n < 0 || (n > 0 && (!xyz || !out || !bbox || !center))) return fail(ctx, DVC_ERR_ARG, "bad arrays");
+    if (method < 0 || method > 2) return fail(ctx, DVC_ERR_ARG, "bad interpolation method");
+    if (grad && method != DVC_INTERP_TRICUBIC) return fail(ctx, DVC_ERR_ARG, "gradients are tricubic only");
+    Geometry G = make_geometry(*rc);
+    HBox est(G.est_mn, G.est_mx);
+    HBox act(G.act_mn, G.act_mx);
+    for (int q = 0; q < 3; q++) { act.cn[q] = G.act_cn[q]; est.cn[q] = G.est_cn[q]; }
+    est.center_on(center);
+    act.center_on(center);
+    int b0[3];
+    bool ok = true;
+    int64_t dims[3] = {v.W, v.H, v.T};
+    for (int q = 0; q < 3; q++) {
+        b0[q] = trunc_i(est.mn[q]);
+        if (!(est.mn[q] == est.mn[q]) || b0[q] < 0 || (int64_t)b0[q] + G.box[q] > dims[q]) ok = false;
+        if (!(bbox[q] >= act.mn[q]) || !(bbox[3 + q] <= act.mx[q])) ok = false;   // BoundBox::contains
+    }
+    int lim = method == DVC_INTERP_NEAREST ? 1 : 2;
+    if (ok)
+        for (int i = 0; i < n && ok; i++)
+            for (int q = 0; q < 3; q++) {
+                double c = xyz[3 * i + q];
+                if (!(c == c) || std::fabs(c) > 1e15) { ok = false; break; }
+                int cell = trunc_i(c) - b0[q];
+                if (cell < 0 || cell > G.box[q] - lim) { ok = false; break; }
+            }
+    if (!ok) return fail(ctx, DVC_ERR_RANGE, "points outside the active interpolation box");
+    if (n == 0) return DVC_OK;
+    HIPCHK(hipSetDevice(ctx->device));
+    double *d_xyz = nullptr, *d_out = nullptr, *d_grad = nullptr;
+    HIPCHK(hipMalloc((void **)&d_xyz, sizeof(double) * 3 * n));
+    HIPCHK(hipMalloc((void **)&d_out, sizeof(double) * n));
+    if (grad) HIPCHK(hipMalloc((void **)&d_grad, sizeof(double) * 3 * n));
+    HIPCHK(hipMemcpyAsync(d_xyz, xyz, sizeof(double) * 3 * n, hipMemcpyHostToDevice, ctx->stream));
+    int r = launch_interp(v.d, v.bpv, (int)v.W, (int)v.H, (int)v.T, b0, G.box, d_xyz, n, method, d_out, d_grad, ctx->stream);
+    if (r == 0) {
+        hipMemcpyAsync(out, d_out, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream);
+        if (grad) hipMemcpyAsync(grad, d_grad, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, ctx->stream);
+    }
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    hipFree(d_xyz); hipFree(d_out); hipFree(d_grad);
+    if (r != 0) return fail(ctx, DVC_ERR_HIP, "interpolation launch failed");
+    if (e != hipSuccess) return hip_fail(ctx, e, "interpolation");
+    return DVC_OK;
+}
+
+int dvc_objective(dvc_ctx *ctx, int kind, const double *ref, const double *tar, int32_t n, double *obj, double *res) {
+    if (!ctx) return DVC_ERR_ARG;
+    if (kind < 0 || kind > 4) return fail(ctx, DVC_ERR_ARG, "unknown objective");
+    if (n < 1 || !ref || !tar || !obj) return fail(ctx, DVC_ERR_ARG, "bad arrays");
+    HIPCHK(hipSetDevice(ctx->device));
+    double *d = nullptr;
+    HIPCHK(hipMalloc((void **)&d, sizeof(double) * (3 * (size_t)n + 1)));
+    double *dr = d, *dt = d + n, *dres = d + 2 * (size_t)n, *dobj = d + 3 * (size_t)n;
+    hipMemcpyAsync(dr, ref, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream);
+    hipMemcpyAsync(dt, tar, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream);
+    int r = launch_objective(kind, dr, dt, n, dobj, res ? dres : nullptr, ctx->stream);
+    if (r == 0) {
+        hipMemcpyAsync(obj, dobj, sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
+        if (res) hipMemcpyAsync(res, dres, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream);
+    }
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    hipFree(d);
+    if (r != 0) return fail(ctx, DVC_ERR_HIP, "objective launch failed");
+    if (e != hipSuccess) return hip_fail(ctx, e, "objective");
+    return DVC_OK;
+}
+
+int dvc_search_prepare(dvc_ctx *ctx, const dvc_run_params *rc, const double *points, int32_t npts,
+                       const int32_t *order, int32_t nproc, const int32_t *neigh, int32_t k) {
+    if (!ctx) return DVC_ERR_ARG;
+    ctx->plan_ready = false;
+    if (int e = check_params(ctx, rc)) return e;
+    if (int e = check_volumes(ctx)) return e;
+    if (npts < 1 || !points || !order || !neigh) return fail(ctx, DVC_ERR_ARG, "bad point arrays");
+    if (k < 1 || k > kMaxNeigh || k > npts) return fail(ctx, DVC_ERR_ARG, "k must be in 1..min(npts, 128)");
+    // dvc.cpp:229-232
+    int N = npts;
+    if (nproc > 1 && nproc < npts) N = nproc;
+    std::vector<int> pos(npts, npts);
+    for (int i = 0; i < npts; i++) {
+        int o = order[i];
+        if (o < 0 || o >= npts || pos[o] != npts) return fail(ctx, DVC_ERR_ARG, "order must be a permutation of 0..npts-1");
+        pos[o] = i;
+    }
+    for (int i = 0; i < npts; i++)
+        if (pos[order[i]] >= N) pos[order[i]] = npts;   // never searched
+    for (int64_t i = 0; i < (int64_t)npts * k; i++)
+        if (neigh[i] < 0 || neigh[i] >= npts) return fail(ctx, DVC_ERR_ARG, "neighbour index out of range");
+    for (int i = 0; i < npts * 3; i++)
+        if (!std::isfinite(points[i])) return fail(ctx, DVC_ERR_ARG, "non-finite point coordinate");
+    Geometry G = make_geometry(*rc);
+    int bt = ctx->block_threads ? ctx->block_threads : 256;
+    int nsamp_pad = (G.subv_num + 3) & ~3;
+    size_t lds = search_lds_bytes(nsamp_pad, bt);
+    if (lds + 8192 > ctx->lds_max) return fail(ctx, DVC_ERR_ARG, "too many samples per point for one workgroup's LDS");
+    HIPCHK(hipSetDevice(ctx->device));
+    // sphere template: identical offsets for every point (one seed, Search.cpp:102-105)
+    std::vector<double> sph;
+    double omn[3] = {0, 0, 0}, omx[3] = {0, 0, 0};
+    if (rc->geom == DVC_GEOM_SPHERE) {
+        sph.resize((size_t)3 * G.subv_num);
+        std::mt19937 mt(rc->sphere_seed);
+        std::uniform_real_distribution<double> dist(-G.subv_rad, G.subv_rad);
+        int count = 0;
+        while (count < G.subv_num) {
+            double rx = dist(mt), ry = dist(mt), rz = dist(mt);
+            double radius = std::sqrt(rx * rx + ry * ry + rz * rz);
+            if (radius <= G.subv_rad) {
+                sph[3 * count + 0] = rc->aspect[0] * (rx);
+                sph[3 * count + 1] = rc->aspect[1] * (ry);
+                sph[3 * count + 2] = rc->aspect[2] * (rz);
+                count += 1;
+            }
+        }
+        for (int q = 0; q < 3; q++) { omn[q] = 1e300; omx[q] = -1e300; }
+        for (int i = 0; i < G.subv_num; i++)
+            for (int q = 0; q < 3; q++) { omn[q] = std::min(omn[q], sph[3 * i + q]); omx[q] = std::max(omx[q], sph[3 * i + q]); }
+    }
+    if (int e = dev_alloc(ctx, &ctx->d_pts, (size_t)3 * npts)) return e;
+    if (int e = dev_alloc(ctx, &ctx->d_sph, sph.size())) return e;
+    if (int e = dev_alloc(ctx, &ctx->d_order, (size_t)npts)) return e;
+    if (int e = dev_alloc(ctx, &ctx->d_pos, (size_t)npts)) return e;
+    if (int e = dev_alloc(ctx, &ctx->d_neigh, (size_t)npts * k)) return e;
+    if (int e = dev_alloc(ctx, &ctx->d_done, (size_t)npts)) return e;
+    if (int e = dev_alloc(ctx, &ctx->d_ctrl, (size_t)4)) return e;
+    if (int e = dev_alloc(ctx, &ctx->d_res, (size_t)npts)) return e;
+    HIPCHK(hipMemcpyAsync(ctx->d_pts, points, sizeof(double) * 3 * npts, hipMemcpyHostToDevice, ctx->stream));
+    if (!sph.empty()) HIPCHK(hipMemcpyAsync(ctx->d_sph, sph.data(), sizeof(double) * sph.size(), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->d_order, order, sizeof(int) * npts, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->d_pos, pos.data(), sizeof(int) * npts, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->d_neigh, neigh, sizeof(int) * (size_t)npts * k, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->rc = *rc;
+    ctx->geo = G;
+    ctx->npts = npts; ctx->nproc = N; ctx->k = k;
+    for (int q = 0; q < 3; q++) { ctx->off_min[q] = omn[q]; ctx->off_max[q] = omx[q]; }
+    ctx->chosen_threads = bt;
+    ctx->searched.assign(npts, 0);
+    for (int i = 0; i < npts; i++) ctx->searched[i] = pos[i] < N ? 1 : 0;
+    ctx->plan_ready = true;
+    return DVC_OK;
+}
+
+int dvc_search_execute(dvc_ctx *ctx) {
+    if (!ctx) return DVC_ERR_ARG;
+    if (!ctx->plan_ready) return fail(ctx, DVC_ERR_STATE, "dvc_search_prepare first");
+    if (int e = check_volumes(ctx)) return e;
+    HIPCHK(hipSetDevice(ctx->device));
+    const Geometry &G = ctx->geo;
+    const dvc_run_params &rc = ctx->rc;
+    SearchArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.vol[0] = ctx->vol[0].d; a.vol[1] = ctx->vol[1].d;
+    a.W = (int)ctx->vol[0].W; a.H = (int)ctx->vol[0].H; a.T = (int)ctx->vol[0].T; a.bpv = ctx->vol[0].bpv;
+    a.ndof = rc.ndof; a.obj = rc.obj; a.interp = rc.interp; a.geom = rc.geom;
+    a.nsamp = G.subv_num; a.g = G.g;
+    a.npts = ctx->npts; a.nproc = ctx->nproc; a.k = ctx->k;
+    a.nsamp_pad = (G.subv_num + 3) & ~3;
+    for (int q = 0; q < 3; q++) {
+        a.box[q] = G.box[q];
+        a.est_mn[q] = G.est_mn[q]; a.est_mx[q] = G.est_mx[q]; a.est_cn[q] = G.est_cn[q];
+        a.act_mn[q] = G.act_mn[q]; a.act_mx[q] = G.act_mx[q]; a.act_cn[q] = G.act_cn[q];
+        a.aspect[q] = rc.aspect[q]; a.rigid[q] = rc.rigid_trans[q];
+        a.off_min[q] = ctx->off_min[q]; a.off_max[q] = ctx->off_max[q];
+    }
+    a.subv_rad = G.subv_rad; a.basin = rc.basin_radius; a.disp_max = (double)rc.disp_max;
+    a.sph = ctx->d_sph; a.pts = ctx->d_pts; a.order = ctx->d_order; a.pos = ctx->d_pos; a.neigh = ctx->d_neigh;
+    a.res = ctx->d_res; a.done = ctx->d_done; a.counter = ctx->d_ctrl; a.error = ctx->d_ctrl + 1;
+    a.wait_ticks = (long long)(ctx->wait_seconds * (double)ctx->wall_khz * 1000.0);
+    int bt = ctx->chosen_threads;
+    size_t lds = search_lds_bytes(a.nsamp_pad, bt);
+    int bpc = 0;
+    if (search_max_blocks(a.bpv, bt, lds, &bpc) != 0 || bpc < 1) return fail(ctx, DVC_ERR_HIP, "search kernel does not fit on a CU");
+    long long grid = (long long)bpc * ctx->cus;
+    if (grid > ctx->nproc) grid = ctx->nproc;
+    if (grid < 1) grid = 1;
+    // results start as not_search (DataCloud.cpp:163-172): zero then mark via the kernel
+    if (launch_zero_int(ctx->d_done, ctx->npts, ctx->stream) != 0) return fail(ctx, DVC_ERR_HIP, "zero launch");
+    if (launch_zero_int(ctx->d_ctrl, 4, ctx->stream) != 0) return fail(ctx, DVC_ERR_HIP, "zero launch");
+    if (launch_zero_int((int *)ctx->d_res, (long long)ctx->npts * (long long)(sizeof(dvc_point_result) / 4), ctx->stream) != 0)
+        return fail(ctx, DVC_ERR_HIP, "zero launch");
+    HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+    if (launch_search(a, bt, (int)grid, lds, ctx->stream) != 0) return fail(ctx, DVC_ERR_HIP, "search kernel launch failed");
+    HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+    return DVC_OK;
+}
+
+int dvc_search_fetch(dvc_ctx *ctx, dvc_point_result *results) {
+    if (!ctx || !results) return DVC_ERR_ARG;
+    if (!ctx->plan_ready) return fail(ctx, DVC_ERR_STATE, "dvc_search_prepare first");
+    HIPCHK(hipSetDevice(ctx->device));
+    int ctrl[4] = {0, 0, 0, 0};
+    HIPCHK(hipMemcpyAsync(results, ctx->d_res, sizeof(dvc_point_result) * ctx->npts, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctrl, ctx->d_ctrl, sizeof(ctrl), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) == hipSuccess) ctx->last_ms = ms;
+    // points never searched keep the not_search record of DataCloud.cpp:168-170
+    for (int i = 0; i < ctx->npts; i++)
+        if (!ctx->searched[i]) {
+            results[i].status = DVC_NOT_SEARCH;
+            results[i].iters = 0;
+            results[i].obj = 0.0;
+            for (int q = 0; q < 12; q++) results[i].par[q] = 0.0;
+        }
+    if (ctrl[1] & 1) return fail(ctx, DVC_ERR_TIMEOUT, "a dependency wait exceeded its time limit");
+    return DVC_OK;
+}
+
+int dvc_search_points(dvc_ctx *ctx, const dvc_run_params *rc, const double *points, int32_t npts,
+                      const int32_t *order, int32_t nproc, const int32_t *neigh, int32_t k,
+                      dvc_point_result *results) {
+    if (int e = dvc_search_prepare(ctx, rc, points, npts, order, nproc, neigh, k)) return e;
+    if (int e = dvc_search_execute(ctx)) return e;
+    return dvc_search_fetch(ctx, results);
+}
+
+int dvc_strain_fit(dvc_ctx *ctx, const double *pts, const double *disp, const int32_t *status, const double *objmin,
+                   const int32_t *neigh, int32_t k, int32_t n, int32_t ndp, double objt, int32_t refill, int32_t planar,
+                   double *out_disp, int32_t *out_npts, double *out_swr, double *out_E, double *out_L, double *out_DG) {
+    if (!ctx) return DVC_ERR_ARG;
+    if (n < 1 || !pts || !disp || !status || !objmin || !neigh) return fail(ctx, DVC_ERR_ARG, "bad input arrays");
+    if (!out_disp || !out_npts || !out_swr || !out_E || !out_L || !out_DG) return fail(ctx, DVC_ERR_ARG, "bad output arrays");
+    if (k < 1 || k > kMaxNeigh) return fail(ctx, DVC_ERR_ARG, "k must be in 1..128");
+    if (ndp < 1 || ndp > 75) return fail(ctx, DVC_ERR_ARG, "ndp must be in 1..75");
+    for (int64_t i = 0; i < (int64_t)n * k; i++)
+        if (neigh[i] < 0 || neigh[i] >= n) return fail(ctx, DVC_ERR_ARG, "neighbour index out of range");
+    // plane detection (strain.cpp:340-365)
+    double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
+    for (int i = 0; i < n; i++)
+        for (int q = 0; q < 3; q++) { mn[q] = std::min(mn[q], pts[3 * i + q]); mx[q] = std::max(mx[q], pts[3 * i + q]); }
+    int yz = (planar & 1) || std::fabs(mx[0] - mn[0]) < 0.001;
+    int zx = (planar & 2) || std::fabs(mx[1] - mn[1]) < 0.001;
+    int xy = (planar & 4) || std::fabs(mx[2] - mn[2]) < 0.001;
+    HIPCHK(hipSetDevice(ctx->device));
+    size_t nd = (size_t)n;
+    size_t bytes = sizeof(double) * (3 * nd + 3 * nd + nd + 3 * nd + nd + 9 * nd * 3) + sizeof(int) * (nd + nd * k + nd);
+    uint8_t *buf = nullptr;
+    HIPCHK(hipMalloc((void **)&buf, bytes + 256));
+    double *d_pts = (double *)buf, *d_disp = d_pts + 3 * nd, *d_obj = d_disp + 3 * nd;
+    double *d_odisp = d_obj + nd, *d_swr = d_odisp + 3 * nd, *d_E = d_swr + nd, *d_L = d_E + 9 * nd, *d_DG = d_L + 9 * nd;
+    int *d_st = (int *)(d_DG + 9 * nd), *d_nb = d_st + nd, *d_np = d_nb + nd * k;
+    hipMemcpyAsync(d_pts, pts, sizeof(double) * 3 * nd, hipMemcpyHostToDevice, ctx->stream);
+    hipMemcpyAsync(d_disp, disp, sizeof(double) * 3 * nd, hipMemcpyHostToDevice, ctx->stream);
+    hipMemcpyAsync(d_obj, objmin, sizeof(double) * nd, hipMemcpyHostToDevice, ctx->stream);
+    hipMemcpyAsync(d_st, status, sizeof(int) * nd, hipMemcpyHostToDevice, ctx->stream);
+    hipMemcpyAsync(d_nb, neigh, sizeof(int) * nd * k, hipMemcpyHostToDevice, ctx->stream);
+    StrainArgs a;
+    a.pts = d_pts; a.disp = d_disp; a.objmin = d_obj; a.status = d_st; a.neigh = d_nb;
+    a.k = k; a.n = n; a.ndp = ndp; a.refill = refill ? 1 : 0;
+    a.yz = yz; a.zx = zx; a.xy = xy; a.objt = objt;
+    a.out_disp = d_odisp; a.out_swr = d_swr; a.out_E = d_E; a.out_L = d_L; a.out_DG = d_DG; a.out_npts = d_np;
+    hipEventRecord(ctx->ev0, ctx->stream);
+    int r = launch_strain(a, ctx->stream);
+    hipEventRecord(ctx->ev1, ctx->stream);
+    if (r == 0) {
+        hipMemcpyAsync(out_disp, d_odisp, sizeof(double) * 3 * nd, hipMemcpyDeviceToHost, ctx->stream);
+        hipMemcpyAsync(out_swr, d_swr, sizeof(double) * nd, hipMemcpyDeviceToHost, ctx->stream);
+        hipMemcpyAsync(out_E, d_E, sizeof(double) * 9 * nd, hipMemcpyDeviceToHost, ctx->stream);
+        hipMemcpyAsync(out_L, d_L, sizeof(double) * 9 * nd, hipMemcpyDeviceToHost, ctx->stream);
+        hipMemcpyAsync(out_DG, d_DG, sizeof(double) * 9 * nd, hipMemcpyDeviceToHost, ctx->stream);
+        hipMemcpyAsync(out_npts, d_np, sizeof(int) * nd, hipMemcpyDeviceToHost, ctx->stream);
+    }
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    float ms = 0.0f;
+    if (e == hipSuccess && hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) == hipSuccess) ctx->last_ms = ms;
+    hipFree(buf);
+    if (r != 0) return fail(ctx, DVC_ERR_HIP, "strain launch failed");
+    if (e != hipSuccess) return hip_fail(ctx, e, "strain");
+    return DVC_OK;
+}
+
+// DataCloud::sort_order_neighbors (DataCloud.cpp:69-144) with the same element type,
+// comparator and std::sort, so ties resolve exactly as in the reference.
+struct DualSort { int index; double value; };
+static bool sortByValue(const DualSort &lhs, const DualSort &rhs) { return lhs.value < rhs.value; }
+static inline double pdist(const double *a, const double *b) {
+    return std::sqrt(std::pow(a[0] - b[0], 2) + std::pow(a[1] - b[1], 2) + std::pow(a[2] - b[2], 2));
+}
+
+int dvc_sort_order_neighbors(const double *pts, int32_t n, const double start[3], int32_t *order, int32_t *neigh,
+                             int32_t nthreads) {
+    if (n < 1 || !pts || !start || !order || !neigh) return DVC_ERR_ARG;
+    int k = 75 < n ? 75 : n;
+    std::vector<DualSort> idx(n);
+    for (int i = 0; i < n; i++) { idx[i].index = i; idx[i].value = pdist(start, &pts[3 * i]); }
+    std::sort(idx.begin(), idx.end(), sortByValue);
+    for (int i = 0; i < n; i++) order[i] = idx[i].index;
+    int nt = nthreads > 0 ? nthreads : (int)std::thread::hardware_concurrency();
+    if (nt < 1) nt = 1;
+    if (nt > n) nt = n;
+    auto work = [&](int t) {
+        std::vector<DualSort> cp(n);
+        for (int i = t; i < n; i += nt) {
+            for (int j = 0; j < n; j++) { cp[j].index = j; cp[j].value = pdist(&pts[3 * i], &pts[3 * j]); }
+            std::sort(cp.begin(), cp.end(), sortByValue);
+            for (int j = 0; j < k; j++) neigh[(size_t)i * k + j] = cp[j].index;
+        }
+    };
+    std::vector<std::thread> th;
+    for (int t = 1; t < nt; t++) th.emplace_back(work, t);
+    work(0);
+    for (auto &x : th) x.join();
+    return k;
+}
+
+}  // extern "C"

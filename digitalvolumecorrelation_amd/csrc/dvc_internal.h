@@ -1,0 +1,59 @@
+// dvc_internal.h -- structures shared by the host code (dvc_api.cpp) and the device
+// code (dvc_kernels.hip).  Not part of the public ABI.
+#pragma once
+
+#include <stdint.h>
+#include "../../include/dvc_amd.h"
+
+namespace dvc {
+
+constexpr int kMaxNeigh = 128;       // neighbour slots read per point (reference: 75)
+
+// Arguments of the persistent search kernel.  Geometry mirrors Search::Search
+// (Search.cpp:24-82) and Interpolate::Interpolate (Interpolate.cpp:34-56).
+struct SearchArgs {
+    const uint8_t *vol[2];           // reference and correlated volume bytes
+    int W, H, T, bpv;
+    int ndof, obj, interp, geom;
+    int nsamp, g;                    // samples per point; cube grid side
+    int npts, nproc, k, nsamp_pad;
+    int box[3];                      // interpolation box size in voxels (kern_4d extent)
+    double est_mn[3], est_mx[3], est_cn[3];   // est_box_nom after the two grow_by calls
+    double act_mn[3], act_mx[3], act_cn[3];   // act_box = est_box shrunk by one voxel
+    double subv_rad, aspect[3], rigid[3], basin, disp_max;
+    double off_min[3], off_max[3];   // sphere offsets' bounding box
+    const double *sph;               // sphere offsets aspect*rel [nsamp*3]
+    const double *pts;               // [npts*3]
+    const int *order;                // [npts]
+    const int *pos;                  // position of each point in order[] (npts if unsearched)
+    const int *neigh;                // [npts*k]
+    dvc_point_result *res;           // [npts]
+    int *done;                       // [npts] completion flags
+    int *counter;                    // work counter
+    int *error;                      // bit0: dependency wait timed out
+    long long wait_ticks;            // wall_clock64 ticks a dependency wait may take
+};
+
+struct StrainArgs {
+    const double *pts, *disp, *objmin;
+    const int *status, *neigh;
+    int k, n, ndp, refill;
+    int yz, zx, xy;
+    double objt;
+    double *out_disp, *out_swr, *out_E, *out_L, *out_DG;
+    int *out_npts;
+};
+
+// launchers implemented in dvc_kernels.hip
+int launch_search(const SearchArgs &a, int block_threads, int grid_blocks, size_t lds_bytes, void *stream);
+int search_max_blocks(int bpv, int block_threads, size_t lds_bytes, int *blocks_per_cu);
+size_t search_lds_bytes(int nsamp_pad, int block_threads);
+int launch_interp(const uint8_t *vol, int bpv, int W, int H, int T, const int b0[3], const int box[3],
+                  const double *xyz, int n, int method, double *out, double *grad, void *stream);
+int launch_objective(int kind, const double *ref, const double *tar, int n, double *obj_out,
+                     double *residual, void *stream);
+int launch_strain(const StrainArgs &a, void *stream);
+int launch_zero_int(int *p, long long n, void *stream);
+int launch_swap16(uint16_t *p, long long n, void *stream);
+
+}  // namespace dvc

@@ -1,0 +1,1399 @@
+// dvc_kernels.hip -- gfx950 kernels of the DVC per-point correlation path.
+//
+// Search kernel (one workgroup per ROI point, persistent, dependency ordered):
+//   Search::process_point            Core/Search.cpp:95-192
+//   Search::search_pt_setup          Core/Search.cpp:248-275
+//   Search::starting_param           Core/Search.cpp:277-377
+//   Search::trgrid_global            Core/Search.cpp:934-1043
+//   Search::min_Lev_Mar / LM_prep_at Core/Search.cpp:194-246, 490-525
+//   FloatingCloud ctors, affine_to   Core/FloatingCloud.cpp:40-128, 148-286
+//   Interpolate::tri_cub_Lek etc.    Core/Interpolate.cpp:144-567, Core/Matrix_4d.h:108-133
+//   obj_* with residual              Core/ObjectiveFunctions.cpp:24-263
+//
+// Design (gfx950, wave64):
+//   * Volumes stay resident in HBM as uint8/uint16, x fastest.  A sample reads its
+//     4x4x4 support as 16 row loads (one dwordx2 for uint8, dwordx3 for uint16).
+//   * The tricubic Lekien-Marsden interpolant of the reference (64x64 BI times the 64
+//     finite-difference data of a cell) is the tensor-product cubic Hermite
+//     interpolant with central-difference slopes.  Inside the interpolation box it is
+//     therefore separable Catmull-Rom: 3 x 4-tap filters, value and gradient in ~190
+//     FP32 FMAs.  Cells touching the box faces, where the reference copies derivatives
+//     from the inner neighbour (Interpolate.cpp:293-335), use the general Hermite form
+//     with those clamped derivatives.
+//   * Gauss-Newton uses the analytic Jacobian of the residual.  The normal-equation
+//     moments of all objectives are entries of the 16x16 matrix V = sum_j v_j v_j^T with
+//     v_j = [dt_j/dp (12), t_j - rbar, r_j - rbar, 1, 0]; each wave forms it on the
+//     matrix cores with v_mfma_f32_16x16x4_f32 (A = B = the same staged register),
+//     flushing the FP32 accumulator into FP64 every 64 samples.
+//   * Objective values are FP64 sums over the FP32 sample values kept in LDS; the
+//     12x12 solve (column-pivoted Householder QR, as Eigen's colPivHouseholderQr) and
+//     all point geometry (cloud, warp, bounding boxes, range checks) are FP64 with
+//     the reference's operation order and no fused multiply-add, so statuses follow
+//     the reference's comparisons.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include "dvc_internal.h"
+
+namespace dvc {
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+// ------------------------------------------------------------------ helpers
+
+__device__ __forceinline__ double wave_sum(double x) {
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    return x;
+}
+__device__ __forceinline__ double wave_min(double x) {
+    for (int o = 32; o > 0; o >>= 1) x = fmin(x, __shfl_xor(x, o, 64));
+    return x;
+}
+__device__ __forceinline__ double wave_max(double x) {
+    for (int o = 32; o > 0; o >>= 1) x = fmax(x, __shfl_xor(x, o, 64));
+    return x;
+}
+
+// Deterministic block sum of K doubles per thread; results land in out[0..K).
+template <int B, int K>
+__device__ __forceinline__ void block_sum(double (&v)[K], double *red, double *out) {
+    constexpr int NW = B / 64;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < K; q++) v[q] = wave_sum(v[q]);
+    if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < K; q++) red[w * K + q] = v[q];
+    }
+    __syncthreads();
+    if (threadIdx.x < K) {
+        double s = 0.0;
+        for (int ww = 0; ww < NW; ww++) s += red[ww * K + threadIdx.x];
+        out[threadIdx.x] = s;
+    }
+    __syncthreads();
+}
+
+// Catmull-Rom weights (Keys a = -1/2) and their derivatives for t in [0,1].
+__device__ __forceinline__ void cr_weights(float t, float w[4], float d[4]) {
+    float t2 = t * t, t3 = t2 * t;
+    w[0] = 0.5f * (-t3 + 2.0f * t2 - t);
+    w[1] = 0.5f * (3.0f * t3 - 5.0f * t2 + 2.0f);
+    w[2] = 0.5f * (-3.0f * t3 + 4.0f * t2 + t);
+    w[3] = 0.5f * (t3 - t2);
+    d[0] = 0.5f * (-3.0f * t2 + 4.0f * t - 1.0f);
+    d[1] = 0.5f * (9.0f * t2 - 10.0f * t);
+    d[2] = 0.5f * (-9.0f * t2 + 8.0f * t + 1.0f);
+    d[3] = 0.5f * (3.0f * t2 - 2.0f * t);
+}
+
+struct VolView {
+    const uint8_t *p;
+    int W;
+    long long WH;
+};
+
+template <int BPV>
+__device__ __forceinline__ float voxel(const VolView &v, int x, int y, int z) {
+    long long i = (long long)z * v.WH + (long long)y * v.W + x;
+    if (BPV == 1) return (float)v.p[i];
+    return (float)((const uint16_t *)v.p)[i];
+}
+
+// Four consecutive voxels x0..x0+3 of row (y, z).  The volume buffer is dword aligned
+// and padded by 16 bytes, so the aligned window read stays inside the allocation.
+template <int BPV>
+__device__ __forceinline__ void load_row(const VolView &v, int x0, int y, int z, float r[4]) {
+    long long i = (long long)z * v.WH + (long long)y * v.W + x0;
+    if (BPV == 1) {
+        uintptr_t ad = (uintptr_t)(v.p + i);
+        const uint32_t *a4 = (const uint32_t *)(ad & ~(uintptr_t)3);
+        uint32_t s = (uint32_t)(ad & 3);
+        uint32_t lo = a4[0], hi = a4[1];
+        uint32_t w = __builtin_amdgcn_alignbyte(hi, lo, s);
+        r[0] = (float)(w & 0xffu);
+        r[1] = (float)((w >> 8) & 0xffu);
+        r[2] = (float)((w >> 16) & 0xffu);
+        r[3] = (float)(w >> 24);
+    } else {
+        uintptr_t ad = (uintptr_t)(v.p + 2 * i);
+        const uint32_t *a4 = (const uint32_t *)(ad & ~(uintptr_t)3);
+        uint32_t s = (uint32_t)(ad & 3);
+        uint32_t d0 = a4[0], d1 = a4[1], d2 = a4[2];
+        uint32_t w0 = __builtin_amdgcn_alignbyte(d1, d0, s);
+        uint32_t w1 = __builtin_amdgcn_alignbyte(d2, d1, s);
+        r[0] = (float)(w0 & 0xffffu);
+        r[1] = (float)(w0 >> 16);
+        r[2] = (float)(w1 & 0xffffu);
+        r[3] = (float)(w1 >> 16);
+    }
+}
+
+// Interpolation box: base voxel (truncated est_box.min) and size.
+struct BoxView {
+    int b[3];
+    int n[3];
+};
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// Cubic Hermite basis on [0,1]: h[a][q] is the weight of the value (q=0) or slope (q=1)
+// at node a; dh its derivative.
+__device__ __forceinline__ void hermite(float t, float h[2][2], float dh[2][2]) {
+    float t2 = t * t, t3 = t2 * t;
+    h[0][0] = 2.0f * t3 - 3.0f * t2 + 1.0f;
+    h[1][0] = -2.0f * t3 + 3.0f * t2;
+    h[0][1] = t3 - 2.0f * t2 + t;
+    h[1][1] = t3 - t2;
+    dh[0][0] = 6.0f * t2 - 6.0f * t;
+    dh[1][0] = -6.0f * t2 + 6.0f * t;
+    dh[0][1] = 3.0f * t2 - 4.0f * t + 1.0f;
+    dh[1][1] = 3.0f * t2 - 2.0f * t;
+}
+
+// Tricubic value (and gradient) of a cell with a corner on the box faces: the
+// 8 Matrix_4d quantities (f, fx, fy, fz, fxy, fxz, fyz, fxyz; Matrix_4d.h:105-133) at
+// each corner, derivatives taken at the corner clamped into the box interior as the
+// face copies of Interpolate.cpp:293-335 leave them, combined with the Hermite basis.
+template <int BPV>
+__device__ float tricubic_faces(const VolView &v, const BoxView &bx, int ix, int iy, int iz,
+                                float tx, float ty, float tz, bool want_grad,
+                                float &gx, float &gy, float &gz) {
+    float hx[2][2], hy[2][2], hz[2][2], dhx[2][2], dhy[2][2], dhz[2][2];
+    hermite(tx, hx, dhx);
+    hermite(ty, hy, dhy);
+    hermite(tz, hz, dhz);
+    float val = 0.0f;
+    gx = gy = gz = 0.0f;
+    for (int c = 0; c < 8; c++) {
+        int a0 = c & 1, a1 = (c >> 1) & 1, a2 = (c >> 2) & 1;
+        int vx = ix + a0, vy = iy + a1, vz = iz + a2;
+        int qx = clampi(vx, bx.b[0] + 1, bx.b[0] + bx.n[0] - 2);
+        int qy = clampi(vy, bx.b[1] + 1, bx.b[1] + bx.n[1] - 2);
+        int qz = clampi(vz, bx.b[2] + 1, bx.b[2] + bx.n[2] - 2);
+#define F(dx, dy, dz) voxel<BPV>(v, qx + (dx), qy + (dy), qz + (dz))
+        float D[8];
+        D[0] = voxel<BPV>(v, vx, vy, vz);
+        D[1] = (F(1, 0, 0) - F(-1, 0, 0)) / 2.0f;
+        D[2] = (F(0, 1, 0) - F(0, -1, 0)) / 2.0f;
+        D[3] = (F(0, 0, 1) - F(0, 0, -1)) / 2.0f;
+        D[4] = (F(1, 1, 0) - F(-1, 1, 0) + F(-1, -1, 0) - F(1, -1, 0)) / 4.0f;
+        D[5] = (F(1, 0, 1) - F(-1, 0, 1) + F(-1, 0, -1) - F(1, 0, -1)) / 4.0f;
+        D[6] = (F(0, 1, 1) - F(0, -1, 1) + F(0, -1, -1) - F(0, 1, -1)) / 4.0f;
+        float t1 = (F(1, 1, -1) - F(-1, 1, -1) + F(-1, -1, -1) - F(1, -1, -1)) / 4.0f;
+        float t2 = (F(1, 1, 1) - F(-1, 1, 1) + F(-1, -1, 1) - F(1, -1, 1)) / 4.0f;
+        D[7] = (t2 - t1) / 2.0f;
+#undef F
+        const int qd[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 1, 0}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1}};
+        for (int q = 0; q < 8; q++) {
+            float wx = hx[a0][qd[q][0]], wy = hy[a1][qd[q][1]], wz = hz[a2][qd[q][2]];
+            val += wx * wy * wz * D[q];
+            if (want_grad) {
+                gx += dhx[a0][qd[q][0]] * wy * wz * D[q];
+                gy += wx * dhy[a1][qd[q][1]] * wz * D[q];
+                gz += wx * wy * dhz[a2][qd[q][2]] * D[q];
+            }
+        }
+    }
+    return val;
+}
+
+// Interpolated value at (x, y, z) with the reference's cell split (Point.cpp:57-66:
+// truncated integer part, remainder) and the box semantics above.  method: 0 nearest,
+// 1 trilinear, 2 tricubic.  Callers guarantee every sample lies in a valid cell; the
+// clamps below only keep stray (NaN) coordinates inside the box.
+template <int BPV>
+__device__ __forceinline__ float interp_at(const VolView &v, const BoxView &bx, double x, double y,
+                                           double z, int method, bool want_grad,
+                                           float &gx, float &gy, float &gz) {
+    int ix = (int)x, iy = (int)y, iz = (int)z;
+    float tx = (float)(x - (double)ix), ty = (float)(y - (double)iy), tz = (float)(z - (double)iz);
+    int cx = ix - bx.b[0], cy = iy - bx.b[1], cz = iz - bx.b[2];
+    gx = gy = gz = 0.0f;
+    if (method == 0) {
+        ix = bx.b[0] + clampi(cx, 0, bx.n[0] - 1);
+        iy = bx.b[1] + clampi(cy, 0, bx.n[1] - 1);
+        iz = bx.b[2] + clampi(cz, 0, bx.n[2] - 1);
+        return voxel<BPV>(v, ix, iy, iz);
+    }
+    if (cx < 0 || cx > bx.n[0] - 2 || cy < 0 || cy > bx.n[1] - 2 || cz < 0 || cz > bx.n[2] - 2 ||
+        !(tx >= 0.0f && tx <= 1.0f && ty >= 0.0f && ty <= 1.0f && tz >= 0.0f && tz <= 1.0f)) {
+        cx = clampi(cx, 0, bx.n[0] - 2);
+        cy = clampi(cy, 0, bx.n[1] - 2);
+        cz = clampi(cz, 0, bx.n[2] - 2);
+        tx = fminf(fmaxf(tx, 0.0f), 1.0f);
+        ty = fminf(fmaxf(ty, 0.0f), 1.0f);
+        tz = fminf(fmaxf(tz, 0.0f), 1.0f);
+        if (!(tx == tx)) tx = 0.0f;
+        if (!(ty == ty)) ty = 0.0f;
+        if (!(tz == tz)) tz = 0.0f;
+        ix = bx.b[0] + cx; iy = bx.b[1] + cy; iz = bx.b[2] + cz;
+    }
+    if (method == 1) {
+        // tri_lin (Interpolate.cpp:429-470)
+        float c000 = voxel<BPV>(v, ix, iy, iz), c100 = voxel<BPV>(v, ix + 1, iy, iz);
+        float c010 = voxel<BPV>(v, ix, iy + 1, iz), c110 = voxel<BPV>(v, ix + 1, iy + 1, iz);
+        float c001 = voxel<BPV>(v, ix, iy, iz + 1), c101 = voxel<BPV>(v, ix + 1, iy, iz + 1);
+        float c011 = voxel<BPV>(v, ix, iy + 1, iz + 1), c111 = voxel<BPV>(v, ix + 1, iy + 1, iz + 1);
+        float x0 = 1.0f - tx, y0 = 1.0f - ty, z0 = 1.0f - tz;
+        float a00 = c000 * x0 + c100 * tx, a10 = c010 * x0 + c110 * tx;
+        float a01 = c001 * x0 + c101 * tx, a11 = c011 * x0 + c111 * tx;
+        float b0 = a00 * y0 + a10 * ty, b1 = a01 * y0 + a11 * ty;
+        if (want_grad) {
+            float e00 = c100 - c000, e10 = c110 - c010, e01 = c101 - c001, e11 = c111 - c011;
+            gx = (e00 * y0 + e10 * ty) * z0 + (e01 * y0 + e11 * ty) * tz;
+            gy = (a10 - a00) * z0 + (a11 - a01) * tz;
+            gz = b1 - b0;
+        }
+        return b0 * z0 + b1 * tz;
+    }
+    cx = ix - bx.b[0]; cy = iy - bx.b[1]; cz = iz - bx.b[2];
+    if (cx < 1 || cx > bx.n[0] - 3 || cy < 1 || cy > bx.n[1] - 3 || cz < 1 || cz > bx.n[2] - 3)
+        return tricubic_faces<BPV>(v, bx, ix, iy, iz, tx, ty, tz, want_grad, gx, gy, gz);
+    float wx[4], dx[4], wy[4], dy[4], wz[4], dz[4];
+    cr_weights(tx, wx, dx);
+    cr_weights(ty, wy, dy);
+    cr_weights(tz, wz, dz);
+    float val = 0.0f;
+#pragma unroll
+    for (int kz = 0; kz < 4; kz++) {
+        float pv = 0.0f, px = 0.0f, py = 0.0f;
+#pragma unroll
+        for (int ky = 0; ky < 4; ky++) {
+            float r[4];
+            load_row<BPV>(v, ix - 1, iy - 1 + ky, iz - 1 + kz, r);
+            float rv = r[0] * wx[0] + r[1] * wx[1] + r[2] * wx[2] + r[3] * wx[3];
+            pv += wy[ky] * rv;
+            if (want_grad) {
+                float rd = r[0] * dx[0] + r[1] * dx[1] + r[2] * dx[2] + r[3] * dx[3];
+                px += wy[ky] * rd;
+                py += dy[ky] * rv;
+            }
+        }
+        val += wz[kz] * pv;
+        if (want_grad) {
+            gx += wz[kz] * px;
+            gy += wz[kz] * py;
+            gz += dz[kz] * pv;
+        }
+    }
+    return val;
+}
+
+// ------------------------------------------------------------- point state
+
+struct PtShared {
+    int status, iters, fail, stop;
+    int b0[3], box_ok;
+    int nb_status[kMaxNeigh];
+    double nb_par[kMaxNeigh][3];
+    double sp[3], sref[3], smin[3], inc[3], sbb_mn[3], sbb_mx[3];
+    double act_mn[3], act_mx[3];
+    double par[12], upd[12], evalpar[12];
+    double u[3], mref[3], A[9], Sm[9];
+    float M[3][9], Af[9];
+    double bb[6];
+    double rmean, R2, Rn2, Src;
+    double tmean, N2, Str, T2, Strr, Szssd;
+    double obj, csum, omin;
+    int gmin[3];
+    double sums[8];
+    double JTJ[144], rhs[12];
+};
+
+// Stable cloud sample j (FloatingCloud.cpp:84-128 cube grid, z outer / x inner;
+// FloatingCloud.cpp:71-76 sphere = centre + aspect*rel).
+__device__ __forceinline__ void stable_pt(const SearchArgs &a, const PtShared &S, int j,
+                                          double &x, double &y, double &z) {
+#pragma clang fp contract(off)
+    if (a.geom == DVC_GEOM_CUBE) {
+        int g = a.g;
+        int kx = j % g, ky = (j / g) % g, kz = j / (g * g);
+        x = S.smin[0] + (double)kx * S.inc[0];
+        y = S.smin[1] + (double)ky * S.inc[1];
+        z = S.smin[2] + (double)kz * S.inc[2];
+    } else {
+        x = S.sp[0] + a.sph[3 * j + 0];
+        y = S.sp[1] + a.sph[3 * j + 1];
+        z = S.sp[2] + a.sph[3 * j + 2];
+    }
+}
+
+// FloatingCloud::affine_to for one point (FloatingCloud.cpp:236-286 with rotate_ptvect
+// :174-204 and strain_ptvect :148-172), same operation order, no contraction.
+__device__ __forceinline__ void warp_pt(const PtShared &S, int ndof, double xs, double ys, double zs,
+                                        double &xo, double &yo, double &zo) {
+#pragma clang fp contract(off)
+    double x = xs + S.u[0], y = ys + S.u[1], z = zs + S.u[2];
+    if (ndof > 3) {
+        double rx = x - S.mref[0], ry = y - S.mref[1], rz = z - S.mref[2];
+        double ox = S.A[0] * rx + S.A[1] * ry + S.A[2] * rz;
+        double oy = S.A[3] * rx + S.A[4] * ry + S.A[5] * rz;
+        double oz = S.A[6] * rx + S.A[7] * ry + S.A[8] * rz;
+        double dx = ox - rx, dy = oy - ry, dz = oz - rz;
+        x = x + dx; y = y + dy; z = z + dz;
+        if (ndof > 6) {
+            rx = x - S.mref[0]; ry = y - S.mref[1]; rz = z - S.mref[2];
+            double sx = S.Sm[0] * rx + S.Sm[1] * ry + S.Sm[2] * rz;
+            double sy = S.Sm[3] * rx + S.Sm[4] * ry + S.Sm[5] * rz;
+            double sz = S.Sm[6] * rx + S.Sm[7] * ry + S.Sm[8] * rz;
+            x = x + sx; y = y + sy; z = z + sz;
+        }
+    }
+    xo = x; yo = y; zo = z;
+}
+
+// Interpolate::center_on (Interpolate.cpp:569-574, BoundBox.cpp:75-85) applied to the
+// nominal boxes, then the volume extent check.  Thread 0 only.
+__device__ void place_box(const SearchArgs &a, PtShared &S, double cx, double cy, double cz) {
+#pragma clang fp contract(off)
+    double c[3] = {cx, cy, cz};
+    S.box_ok = 1;
+    int dims[3] = {a.W, a.H, a.T};
+    for (int q = 0; q < 3; q++) {
+        double d = c[q] - a.est_cn[q];
+        double mn = a.est_mn[q] + d;
+        double da = c[q] - a.act_cn[q];
+        S.act_mn[q] = a.act_mn[q] + da;
+        S.act_mx[q] = a.act_mx[q] + da;
+        int b = (int)(mn / 1);
+        S.b0[q] = b;
+        if (!(mn == mn) || b < 0 || (long long)b + a.box[q] > (long long)dims[q]) S.box_ok = 0;
+    }
+}
+
+// Range rule of tri_cub_Lek / tri_lin / nearest: act_box.contains(bbox)
+// (Interpolate.cpp:474-481, BoundBox.cpp:87-99), box inside the volume, and the cells
+// of the extreme samples inside the box (the reference indexes past kern_4d there).
+__device__ bool range_ok(const SearchArgs &a, const PtShared &S, const double bb[6]) {
+    for (int q = 0; q < 6; q++)
+        if (!(bb[q] == bb[q]) || fabs(bb[q]) > 1.0e15) return false;
+    if (bb[0] < S.act_mn[0] || bb[1] < S.act_mn[1] || bb[2] < S.act_mn[2] ||
+        bb[3] > S.act_mx[0] || bb[4] > S.act_mx[1] || bb[5] > S.act_mx[2])
+        return false;
+    if (!S.box_ok) return false;
+    int lim = (a.interp == DVC_INTERP_NEAREST) ? 1 : 2;
+    for (int q = 0; q < 3; q++) {
+        int lo = (int)(bb[q] / 1) - S.b0[q];
+        int hi = (int)(bb[3 + q] / 1) - S.b0[q];
+        if (lo < 0 || hi > a.box[q] - lim) return false;
+    }
+    return true;
+}
+
+// Rotation matrix of SearchParams::set_param_vect (SearchParams.cpp:77-87) and its
+// derivatives with respect to (phi, the, psi).
+__device__ void rotation(double phi, double the, double psi, double A[9], double dA[3][9]) {
+    double cf = cos(phi), sf = sin(phi), ct = cos(the), st = sin(the), cp = cos(psi), sp = sin(psi);
+    A[0] = ct * cf; A[1] = ct * sf; A[2] = -st;
+    A[3] = sp * st * cf - cp * sf; A[4] = sp * st * sf + cp * cf; A[5] = ct * sp;
+    A[6] = cp * st * cf + sp * sf; A[7] = cp * st * sf - sp * cf; A[8] = ct * cp;
+    double *P = dA[0];
+    P[0] = -ct * sf; P[1] = ct * cf; P[2] = 0;
+    P[3] = -sp * st * sf - cp * cf; P[4] = sp * st * cf - cp * sf; P[5] = 0;
+    P[6] = -cp * st * sf + sp * cf; P[7] = cp * st * cf + sp * sf; P[8] = 0;
+    double *T = dA[1];
+    T[0] = -st * cf; T[1] = -st * sf; T[2] = -ct;
+    T[3] = sp * ct * cf; T[4] = sp * ct * sf; T[5] = -st * sp;
+    T[6] = cp * ct * cf; T[7] = cp * ct * sf; T[8] = -st * cp;
+    double *Q = dA[2];
+    Q[0] = 0; Q[1] = 0; Q[2] = 0;
+    Q[3] = cp * st * cf + sp * sf; Q[4] = cp * st * sf - sp * cf; Q[5] = ct * cp;
+    Q[6] = -sp * st * cf + cp * sf; Q[7] = -sp * st * sf - cp * cf; Q[8] = -ct * sp;
+}
+
+// Warp state for parameter vector evalpar (thread 0).  SearchParams.cpp:53-107.
+__device__ void set_warp(PtShared &S, int ndof) {
+#pragma clang fp contract(off)
+    for (int q = 0; q < 3; q++) {
+        S.u[q] = S.evalpar[q];
+        S.mref[q] = S.sref[q] + S.u[q];
+    }
+    double dA[3][9];
+    if (ndof > 3) {
+        rotation(S.evalpar[3], S.evalpar[4], S.evalpar[5], S.A, dA);
+    } else {
+        for (int q = 0; q < 9; q++) S.A[q] = (q % 4 == 0) ? 1.0 : 0.0;
+        for (int r = 0; r < 3; r++) for (int q = 0; q < 9; q++) dA[r][q] = 0.0;
+    }
+    for (int q = 0; q < 9; q++) S.Sm[q] = 0.0;
+    if (ndof > 6) {
+        const double *v = S.evalpar;
+        S.Sm[0] = v[6];  S.Sm[1] = v[9];  S.Sm[2] = v[11];
+        S.Sm[3] = v[9];  S.Sm[4] = v[7];  S.Sm[5] = v[10];
+        S.Sm[6] = v[11]; S.Sm[7] = v[10]; S.Sm[8] = v[8];
+    }
+    // d x'/d angle = (I+S) dA rel
+    for (int r = 0; r < 3; r++)
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                double s = dA[r][i * 3 + j];
+                for (int l = 0; l < 3; l++) s += S.Sm[i * 3 + l] * dA[r][l * 3 + j];
+                S.M[r][i * 3 + j] = (float)s;
+            }
+    for (int q = 0; q < 9; q++) S.Af[q] = (float)S.A[q];
+}
+
+// Column-pivoted Householder QR solve (Eigen 3.4 ColPivHouseholderQR::computeInPlace
+// and _solve_impl, as used at Search.cpp:234).  A is n x n row-major in LDS; thread 0.
+__device__ void colpiv_solve(const double *Arow, const double *b, int n, double *x) {
+    double Q[144], hc[12], nu[12], nd[12], c[12];
+    int tr[12], perm[12];
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) Q[j * n + i] = Arow[i * n + j];
+    for (int k = 0; k < n; k++) {
+        double s = 0.0;
+        for (int i = 0; i < n; i++) s += Q[k * n + i] * Q[k * n + i];
+        nd[k] = sqrt(s);
+        nu[k] = nd[k];
+    }
+    double maxn = 0.0;
+    for (int k = 0; k < n; k++) maxn = fmax(maxn, nu[k]);
+    const double eps = 2.220446049250313e-16;
+    double thr = (maxn * eps) * (maxn * eps) / (double)n;
+    double dthr = sqrt(eps);
+    int nonzero = n;
+    for (int k = 0; k < n; k++) {
+        int big = k;
+        double bn = nu[k];
+        for (int j = k + 1; j < n; j++)
+            if (nu[j] > bn) { bn = nu[j]; big = j; }
+        if (nonzero == n && bn * bn < thr * (double)(n - k)) nonzero = k;
+        tr[k] = big;
+        if (k != big) {
+            for (int i = 0; i < n; i++) { double t = Q[k * n + i]; Q[k * n + i] = Q[big * n + i]; Q[big * n + i] = t; }
+            double t = nu[k]; nu[k] = nu[big]; nu[big] = t;
+            t = nd[k]; nd[k] = nd[big]; nd[big] = t;
+        }
+        // makeHouseholder of column k rows k..n-1
+        double *col = &Q[k * n + k];
+        int m = n - k;
+        double tail = 0.0;
+        for (int i = 1; i < m; i++) tail += col[i] * col[i];
+        double c0 = col[0], tau, beta;
+        if (tail <= 2.2250738585072014e-308) {
+            tau = 0.0; beta = c0;
+            for (int i = 1; i < m; i++) col[i] = 0.0;
+        } else {
+            beta = sqrt(c0 * c0 + tail);
+            if (c0 >= 0.0) beta = -beta;
+            for (int i = 1; i < m; i++) col[i] = col[i] / (c0 - beta);
+            tau = (beta - c0) / beta;
+        }
+        hc[k] = tau;
+        col[0] = beta;
+        if (m > 1 && tau != 0.0) {
+            for (int j = k + 1; j < n; j++) {
+                double *cj = &Q[j * n + k];
+                double t = 0.0;
+                for (int i = 1; i < m; i++) t += col[i] * cj[i];
+                t += cj[0];
+                cj[0] -= tau * t;
+                for (int i = 1; i < m; i++) cj[i] -= tau * col[i] * t;
+            }
+        } else if (m == 1) {
+            for (int j = k + 1; j < n; j++) Q[j * n + k] *= (1.0 - tau);
+        }
+        for (int j = k + 1; j < n; j++) {
+            if (nu[j] != 0.0) {
+                double t = fabs(Q[j * n + k]) / nu[j];
+                t = (1.0 + t) * (1.0 - t);
+                t = t < 0.0 ? 0.0 : t;
+                double r = nu[j] / nd[j];
+                double t2 = t * (r * r);
+                if (t2 <= dthr) {
+                    double s = 0.0;
+                    for (int i = k + 1; i < n; i++) s += Q[j * n + i] * Q[j * n + i];
+                    nd[j] = sqrt(s);
+                    nu[j] = nd[j];
+                } else {
+                    nu[j] *= sqrt(t);
+                }
+            }
+        }
+    }
+    for (int i = 0; i < n; i++) perm[i] = i;
+    for (int k = 0; k < n; k++) { int t = perm[k]; perm[k] = perm[tr[k]]; perm[tr[k]] = t; }
+    for (int i = 0; i < n; i++) x[i] = 0.0;
+    if (nonzero == 0) return;
+    for (int i = 0; i < n; i++) c[i] = b[i];
+    for (int k = 0; k < nonzero; k++) {
+        int m = n - k;
+        double tau = hc[k];
+        if (m == 1) { c[k] *= (1.0 - tau); continue; }
+        if (tau == 0.0) continue;
+        double t = 0.0;
+        for (int i = 1; i < m; i++) t += Q[k * n + k + i] * c[k + i];
+        t += c[k];
+        c[k] -= tau * t;
+        for (int i = 1; i < m; i++) c[k + i] -= tau * Q[k * n + k + i] * t;
+    }
+    for (int i = nonzero - 1; i >= 0; i--) {
+        double s = c[i];
+        for (int j = i + 1; j < nonzero; j++) s -= Q[j * n + i] * c[j];
+        c[i] = s / Q[i * n + i];
+    }
+    for (int i = 0; i < nonzero; i++) x[perm[i]] = c[i];
+}
+
+// ------------------------------------------------------------- evaluation
+
+// One objective evaluation of the moving cloud at S.evalpar (first ndof entries):
+// Search::obj_val_at (Search.cpp:448-488).  With need_j the analytic-Jacobian moments
+// are left in V (16x16, FP64).  Returns false on Range_Fail.
+template <int BPV, int B>
+__device__ bool evaluate(const SearchArgs &a, PtShared &S, int ndof, bool need_j,
+                         float *rcb, float *tcb, float *stage, double *red, double *V) {
+    constexpr int NW = B / 64;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int n = a.nsamp;
+    if (tid == 0) set_warp(S, ndof);
+    __syncthreads();
+    // moving bounding box
+    if (ndof > 3) {
+        double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
+        for (int j = tid; j < n; j += B) {
+            double xs, ys, zs, x, y, z;
+            stable_pt(a, S, j, xs, ys, zs);
+            warp_pt(S, ndof, xs, ys, zs, x, y, z);
+            mn[0] = fmin(mn[0], x); mn[1] = fmin(mn[1], y); mn[2] = fmin(mn[2], z);
+            mx[0] = fmax(mx[0], x); mx[1] = fmax(mx[1], y); mx[2] = fmax(mx[2], z);
+        }
+        for (int q = 0; q < 3; q++) { mn[q] = wave_min(mn[q]); mx[q] = wave_max(mx[q]); }
+        if (lane == 0)
+            for (int q = 0; q < 3; q++) { red[wid * 6 + q] = mn[q]; red[wid * 6 + 3 + q] = mx[q]; }
+        __syncthreads();
+        if (tid == 0) {
+            for (int q = 0; q < 3; q++) {
+                double lo = red[q], hi = red[3 + q];
+                for (int w = 1; w < NW; w++) { lo = fmin(lo, red[w * 6 + q]); hi = fmax(hi, red[w * 6 + 3 + q]); }
+                // resize_bbox sentinels (FloatingCloud.cpp:208-214)
+                S.bb[q] = fmin(lo, 9999.9);
+                S.bb[3 + q] = fmax(hi, -9999.9);
+            }
+            S.fail = range_ok(a, S, S.bb) ? 0 : 1;
+        }
+    } else if (tid == 0) {
+#pragma clang fp contract(off)
+        for (int q = 0; q < 3; q++) { S.bb[q] = S.sbb_mn[q] + S.u[q]; S.bb[3 + q] = S.sbb_mx[q] + S.u[q]; }
+        S.fail = range_ok(a, S, S.bb) ? 0 : 1;
+    }
+    __syncthreads();
+    if (S.fail) return false;
+
+    const VolView vv = {a.vol[1], a.W, (long long)a.W * a.H};
+    const BoxView bx = {{S.b0[0], S.b0[1], S.b0[2]}, {a.box[0], a.box[1], a.box[2]}};
+    const double rmean = S.rmean;
+    const int method = a.interp;
+    const bool jrow = need_j;
+    floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    double acc64[4] = {0.0, 0.0, 0.0, 0.0};
+    float *wst = stage + wid * 64 * 17;
+    double s_tc = 0.0, s_d2 = 0.0, s_ad = 0.0;
+    const int steps = (n + B - 1) / B;
+    for (int it = 0; it < steps; it++) {
+        int j = tid + it * B;
+        float v[16];
+#pragma unroll
+        for (int q = 0; q < 16; q++) v[q] = 0.0f;
+        if (j < n) {
+            double xs, ys, zs, x, y, z;
+            stable_pt(a, S, j, xs, ys, zs);
+            warp_pt(S, ndof, xs, ys, zs, x, y, z);
+            float gx, gy, gz;
+            float t = interp_at<BPV>(vv, bx, x, y, z, method, jrow, gx, gy, gz);
+            float tc = (float)((double)t - rmean);
+            float rc = rcb[j];
+            tcb[j] = tc;
+            double d = (double)tc - (double)rc;
+            s_tc += (double)tc;
+            s_d2 += d * d;
+            s_ad += fabs(d);
+            if (jrow) {
+                v[0] = gx; v[1] = gy; v[2] = gz;
+                if (ndof > 3) {
+                    float rx = (float)(xs - S.sref[0]), ry = (float)(ys - S.sref[1]), rz = (float)(zs - S.sref[2]);
+#pragma unroll
+                    for (int r = 0; r < 3; r++) {
+                        const float *M = S.M[r];
+                        float dx = M[0] * rx + M[1] * ry + M[2] * rz;
+                        float dy = M[3] * rx + M[4] * ry + M[5] * rz;
+                        float dz = M[6] * rx + M[7] * ry + M[8] * rz;
+                        v[3 + r] = gx * dx + gy * dy + gz * dz;
+                    }
+                    if (ndof > 6) {
+                        float w0 = S.Af[0] * rx + S.Af[1] * ry + S.Af[2] * rz;
+                        float w1 = S.Af[3] * rx + S.Af[4] * ry + S.Af[5] * rz;
+                        float w2 = S.Af[6] * rx + S.Af[7] * ry + S.Af[8] * rz;
+                        v[6] = gx * w0;
+                        v[7] = gy * w1;
+                        v[8] = gz * w2;
+                        v[9] = gx * w1 + gy * w0;
+                        v[10] = gy * w2 + gz * w1;
+                        v[11] = gx * w2 + gz * w0;
+                    }
+                }
+                v[12] = tc;
+                v[13] = rc;
+                v[14] = 1.0f;
+            }
+        }
+        if (jrow) {
+            // stage the 64 rows of this wave, then V += rows^T rows on the matrix cores
+#pragma unroll
+            for (int q = 0; q < 16; q++) wst[lane * 17 + q] = v[q];
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+            for (int s = 0; s < 16; s++) {
+                float xv = wst[(4 * s + (lane >> 4)) * 17 + (lane & 15)];
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(xv, xv, acc, 0, 0, 0);
+            }
+#pragma unroll
+            for (int r = 0; r < 4; r++) { acc64[r] += (double)acc[r]; acc[r] = 0.0f; }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+    }
+    {
+        double v3[3] = {s_tc, s_d2, s_ad};
+        block_sum<B, 3>(v3, red, S.sums);
+    }
+    if (jrow) {
+        double *st64 = (double *)stage;
+        // D row = 4*(lane/16) + r, column = lane%16 (symmetric, so either reading works)
+#pragma unroll
+        for (int r = 0; r < 4; r++) st64[wid * 256 + (4 * (lane >> 4) + r) * 16 + (lane & 15)] = acc64[r];
+        __syncthreads();
+        for (int e = tid; e < 256; e += B) {
+            double s = 0.0;
+            for (int w = 0; w < NW; w++) s += st64[w * 256 + e];
+            V[e] = s;
+        }
+        __syncthreads();
+    }
+    const int kind = a.obj;
+    const double nn = (double)n;
+    if (kind == DVC_OBJ_SAD || kind == DVC_OBJ_SSD) {
+        if (tid == 0) {
+            S.tmean = S.sums[0] / nn;
+            S.obj = (kind == DVC_OBJ_SAD) ? S.sums[2] : S.sums[1];
+        }
+        __syncthreads();
+        return true;
+    }
+    const double tm = S.sums[0] / nn;
+    // pass B: centred sums
+    {
+        double b[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int j = tid; j < n; j += B) {
+            double tc = (double)tcb[j], rc = (double)rcb[j];
+            double tt = tc - tm;
+            double t = tc + rmean, r = rc + rmean;
+            double dz = tt - rc;
+            b[0] += tt * tt;
+            b[1] += tt * rc;
+            b[2] += t * t;
+            b[3] += t * r;
+            b[4] += dz * dz;
+        }
+        block_sum<B, 5>(b, red, &S.sums[3]);
+    }
+    if (kind == DVC_OBJ_ZSSD) {
+        if (tid == 0) { S.tmean = tm; S.N2 = S.sums[3]; S.obj = S.sums[7]; }
+        __syncthreads();
+        return true;
+    }
+    // pass C: normalised residual sum of squares
+    {
+        double N = sqrt(S.sums[3]), T = sqrt(S.sums[5]);
+        double R = sqrt(S.R2), Rn = sqrt(S.Rn2);
+        double c[1] = {0.0};
+        for (int j = tid; j < n; j += B) {
+            double tc = (double)tcb[j], rc = (double)rcb[j];
+            double d;
+            if (kind == DVC_OBJ_ZNSSD) d = ((tc - tm) / N) - (rc / R);
+            else d = (tc + rmean) / T - (rc + rmean) / Rn;
+            c[0] += d * d;
+        }
+        block_sum<B, 1>(c, red, &S.csum);
+        double out[1] = {S.csum};
+        if (tid == 0) {
+            S.tmean = tm;
+            S.N2 = S.sums[3]; S.Str = S.sums[4]; S.T2 = S.sums[5]; S.Strr = S.sums[6];
+            S.obj = (kind == DVC_OBJ_ZNSSD) ? out[0] / 4.0 : out[0] / 2.0;
+        }
+        __syncthreads();
+    }
+    return true;
+}
+
+// Normal equations J^T J and -J^T e of the objective's residual from the moments in V.
+__device__ void assemble(const SearchArgs &a, PtShared &S, const double *V, int nd) {
+    const double n = (double)a.nsamp;
+    double Sg[12], Sgtc[12], Sgrc[12], c[12];
+    for (int p = 0; p < nd; p++) { Sg[p] = V[p * 16 + 14]; Sgtc[p] = V[p * 16 + 12]; Sgrc[p] = V[p * 16 + 13]; }
+    const int kind = a.obj;
+    if (kind == DVC_OBJ_SAD || kind == DVC_OBJ_SSD) {
+        for (int p = 0; p < nd; p++) {
+            for (int q = 0; q < nd; q++) S.JTJ[p * nd + q] = V[p * 16 + q];
+            S.rhs[p] = -(Sgtc[p] - Sgrc[p]);
+        }
+    } else if (kind == DVC_OBJ_ZSSD) {
+        double tm = S.tmean, Src = V[13 * 16 + 14];
+        for (int p = 0; p < nd; p++) {
+            for (int q = 0; q < nd; q++) S.JTJ[p * nd + q] = V[p * 16 + q] - Sg[p] * Sg[q] / n;
+            S.rhs[p] = -((Sgtc[p] - tm * Sg[p]) - Sgrc[p] + (Sg[p] / n) * Src);
+        }
+    } else if (kind == DVC_OBJ_NSSD) {
+        double T2 = S.T2, T = sqrt(T2), Rn = sqrt(S.Rn2), rm = S.rmean;
+        for (int p = 0; p < nd; p++) c[p] = Sgtc[p] + rm * Sg[p];
+        for (int p = 0; p < nd; p++) {
+            for (int q = 0; q < nd; q++) S.JTJ[p * nd + q] = V[p * 16 + q] / T2 - c[p] * c[q] / (T2 * T2);
+            double Sgr = Sgrc[p] + rm * Sg[p];
+            S.rhs[p] = -(-Sgr / (T * Rn) + c[p] * S.Strr / (T2 * T * Rn));
+        }
+    } else {
+        double tm = S.tmean, N2 = S.N2, N = sqrt(N2), R = sqrt(S.R2);
+        for (int p = 0; p < nd; p++) c[p] = Sgtc[p] - tm * Sg[p];
+        for (int p = 0; p < nd; p++) {
+            for (int q = 0; q < nd; q++)
+                S.JTJ[p * nd + q] = (V[p * 16 + q] - Sg[p] * Sg[q] / n) / N2 - c[p] * c[q] / (N2 * N2);
+            S.rhs[p] = -(-Sgrc[p] / (N * R) + c[p] * S.Str / (N2 * N * R));
+        }
+    }
+}
+
+// ------------------------------------------------------------- the search
+
+template <int BPV, int B>
+__device__ void point_search(const SearchArgs &a, int p, int n, PtShared &S, float *rcb, float *tcb,
+                             float *stage, double *red, double *V) {
+#pragma clang fp contract(off)
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int ns = a.nsamp;
+    // ---- cloud geometry and reference box (Search.cpp:98-119, 252-258)
+    if (tid == 0) {
+#pragma clang fp contract(off)
+        S.status = DVC_POINT_GOOD; S.iters = 0; S.fail = 0; S.stop = 0;
+        for (int q = 0; q < 12; q++) { S.par[q] = 0.0; S.upd[q] = 0.0; }
+        for (int q = 0; q < 3; q++) S.sp[q] = a.pts[3 * n + q];
+        if (a.geom == DVC_GEOM_CUBE) {
+            double smn[3], smx[3];
+            for (int q = 0; q < 3; q++) {
+                smn[q] = S.sp[q] + -a.subv_rad * a.aspect[q];
+                smx[q] = S.sp[q] + a.subv_rad * a.aspect[q];
+            }
+            for (int q = 0; q < 3; q++) {
+                S.sref[q] = (smx[q] + smn[q]) / 2.0;
+                S.inc[q] = (smx[q] - smn[q]) / (a.g - 1);
+                S.smin[q] = smn[q];
+                S.sbb_mn[q] = smn[q] + 0.0 * S.inc[q];
+                S.sbb_mx[q] = smn[q] + (double)(a.g - 1) * S.inc[q];
+            }
+        } else {
+            for (int q = 0; q < 3; q++) {
+                S.sref[q] = S.sp[q];
+                S.sbb_mn[q] = S.sp[q] + a.off_min[q];
+                S.sbb_mx[q] = S.sp[q] + a.off_max[q];
+            }
+        }
+        place_box(a, S, S.sp[0], S.sp[1], S.sp[2]);
+        double bb[6] = {S.sbb_mn[0], S.sbb_mn[1], S.sbb_mn[2], S.sbb_mx[0], S.sbb_mx[1], S.sbb_mx[2]};
+        S.fail = range_ok(a, S, bb) ? 0 : 1;
+    }
+    __syncthreads();
+    if (S.fail) {
+        if (tid == 0) S.status = DVC_RANGE_FAIL;
+        return;
+    }
+    // ---- reference sub-volume
+    {
+        const VolView vr = {a.vol[0], a.W, (long long)a.W * a.H};
+        const BoxView bx = {{S.b0[0], S.b0[1], S.b0[2]}, {a.box[0], a.box[1], a.box[2]}};
+        double s[1] = {0.0};
+        for (int j = tid; j < ns; j += B) {
+            double x, y, z;
+            stable_pt(a, S, j, x, y, z);
+            float gx, gy, gz;
+            float r = interp_at<BPV>(vr, bx, x, y, z, a.interp, false, gx, gy, gz);
+            rcb[j] = r;
+            s[0] += (double)r;
+        }
+        block_sum<B, 1>(s, red, S.sums);
+        const double rmean = S.sums[0] / (double)ns;
+        double q3[3] = {0.0, 0.0, 0.0};
+        for (int j = tid; j < ns; j += B) {
+            float rc = (float)((double)rcb[j] - rmean);
+            rcb[j] = rc;
+            q3[0] += (double)rc * (double)rc;
+            q3[1] += (double)rc;
+            double r = (double)rc + rmean;
+            q3[2] += r * r;
+        }
+        block_sum<B, 3>(q3, red, &S.sums[1]);
+        if (tid == 0) { S.rmean = rmean; S.R2 = S.sums[1]; S.Src = S.sums[2]; S.Rn2 = S.sums[3]; }
+    }
+    // ---- starting_param: neighbours searched earlier (wave 0 waits on their flags)
+    if (tid < 64) {
+        for (int q = lane; q < a.k; q += 64) {
+            int m = a.neigh[(long long)n * a.k + q];
+            int st = DVC_NOT_SEARCH;
+            double px = 0.0, py = 0.0, pz = 0.0;
+            if (m >= 0 && m < a.npts && a.pos[m] < p) {
+                long long t0 = wall_clock64();
+                bool ok = true;
+                while (__hip_atomic_load(&a.done[m], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+                    __builtin_amdgcn_s_sleep(2);
+                    if (wall_clock64() - t0 > a.wait_ticks) { ok = false; break; }
+                }
+                if (ok) {
+                    st = a.res[m].status;
+                    px = a.res[m].par[0]; py = a.res[m].par[1]; pz = a.res[m].par[2];
+                } else {
+                    atomicOr(a.error, 1);
+                }
+            }
+            S.nb_status[q] = st;
+            S.nb_par[q][0] = px; S.nb_par[q][1] = py; S.nb_par[q][2] = pz;
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+#pragma clang fp contract(off)
+        double sx = 0.0, sy = 0.0, sz = 0.0;
+        int good = 0;
+        for (int q = 0; q < a.k; q++)
+            if (S.nb_status[q] == DVC_POINT_GOOD) {
+                sx += S.nb_par[q][0]; sy += S.nb_par[q][1]; sz += S.nb_par[q][2]; good += 1;
+            }
+        if (good != 0) { sx /= good; sy /= good; sz /= good; }
+        if (good != 0) { S.par[0] = sx; S.par[1] = sy; S.par[2] = sz; }
+        else { S.par[0] = a.rigid[0]; S.par[1] = a.rigid[1]; S.par[2] = a.rigid[2]; }
+        // target box on the offset point (Search.cpp:267-273)
+        double ox = S.sp[0] + S.par[0], oy = S.sp[1] + S.par[1], oz = S.sp[2] + S.par[2];
+        place_box(a, S, ox, oy, oz);
+    }
+    __syncthreads();
+    // ---- translation grid (Search.cpp:934-1043)
+    if (a.basin != 0.0) {
+        const int ni = (int)(a.disp_max / a.basin);
+        const double inc = a.disp_max / ni;
+        if (tid == 0) { S.omin = 1.7976931348623157e308; S.gmin[0] = S.gmin[1] = S.gmin[2] = 0; }
+        for (int z = -ni; z <= ni; z++)
+            for (int y = -ni; y <= ni; y++)
+                for (int x = -ni; x <= ni; x++) {
+                    if (tid == 0) {
+#pragma clang fp contract(off)
+                        S.evalpar[0] = S.par[0] + (double)x * inc;
+                        S.evalpar[1] = S.par[1] + (double)y * inc;
+                        S.evalpar[2] = S.par[2] + (double)z * inc;
+                    }
+                    __syncthreads();
+                    if (!evaluate<BPV, B>(a, S, 3, false, rcb, tcb, stage, red, V)) {
+                        if (tid == 0) S.status = DVC_RANGE_FAIL;
+                        return;
+                    }
+                    if (tid == 0 && S.obj < S.omin) { S.omin = S.obj; S.gmin[0] = x; S.gmin[1] = y; S.gmin[2] = z; }
+                    __syncthreads();
+                }
+        if (tid == 0) {
+#pragma clang fp contract(off)
+            S.par[0] += (double)S.gmin[0] * inc;
+            S.par[1] += (double)S.gmin[1] * inc;
+            S.par[2] += (double)S.gmin[2] * inc;
+        }
+        __syncthreads();
+    }
+    // ---- Gauss-Newton (Search.cpp:194-246)
+    const int nd = a.ndof;
+    double obj_old = 0.0;
+    bool converged = false;
+    for (int i = 0; i < 20; i++) {
+        if (tid == 0)
+            for (int q = 0; q < 12; q++) S.evalpar[q] = S.par[q];
+        __syncthreads();
+        if (!evaluate<BPV, B>(a, S, nd, true, rcb, tcb, stage, red, V)) {
+            if (tid == 0) S.status = DVC_RANGE_FAIL;
+            return;
+        }
+        if (tid == 0) {
+            S.iters += 1;
+            S.stop = 0;
+            if (i > 0) {
+                double del_obj = fabs(S.obj - obj_old);
+                double del_mag = sqrt(S.upd[0] * S.upd[0] + S.upd[1] * S.upd[1] + S.upd[2] * S.upd[2]);
+                if ((del_obj < 0.000001) || (del_mag < 0.01)) S.stop = 1;
+            }
+            if (!S.stop) {
+                assemble(a, S, V, nd);
+                double x[12];
+                colpiv_solve(S.JTJ, S.rhs, nd, x);
+                for (int q = 0; q < nd; q++) { S.upd[q] = x[q]; S.par[q] += x[q]; }
+            }
+        }
+        __syncthreads();
+        if (S.stop) { converged = true; break; }
+        obj_old = S.obj;
+    }
+    // ---- final objective (Search.cpp:187): needed when the iteration limit ended the loop
+    if (!converged) {
+        if (tid == 0)
+            for (int q = 0; q < 12; q++) S.evalpar[q] = S.par[q];
+        __syncthreads();
+        if (!evaluate<BPV, B>(a, S, nd, false, rcb, tcb, stage, red, V)) {
+            if (tid == 0) S.status = DVC_RANGE_FAIL;
+            return;
+        }
+    }
+}
+
+template <int BPV, int B>
+__global__ void __launch_bounds__(B) search_kernel(SearchArgs a) {
+    extern __shared__ __align__(16) float dyn[];
+    float *rcb = dyn;
+    float *tcb = dyn + a.nsamp_pad;
+    float *stage = dyn + 2 * a.nsamp_pad;
+    __shared__ PtShared S;
+    __shared__ double red[(B / 64) * 8];
+    __shared__ double V[256];
+    __shared__ int s_p;
+    const int tid = threadIdx.x;
+    for (;;) {
+        if (tid == 0) s_p = atomicAdd(a.counter, 1);
+        __syncthreads();
+        const int p = s_p;
+        if (p >= a.nproc) break;
+        const int n = a.order[p];
+        point_search<BPV, B>(a, p, n, S, rcb, tcb, stage, red, V);
+        __syncthreads();
+        if (tid == 0) {
+            dvc_point_result *r = &a.res[n];
+            r->status = S.status;
+            r->iters = S.iters;
+            if (S.status == DVC_POINT_GOOD) {
+                r->obj = S.obj;
+                for (int q = 0; q < 12; q++) r->par[q] = (q < a.ndof) ? S.par[q] : 0.0;
+            } else {
+                // dvc.cpp:268-283 records only the status of a failed point
+                r->obj = 0.0;
+                for (int q = 0; q < 12; q++) r->par[q] = 0.0;
+            }
+            __threadfence();
+            __hip_atomic_store(&a.done[n], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+size_t search_lds_bytes(int nsamp_pad, int block_threads) {
+    return (size_t)nsamp_pad * 2 * sizeof(float) + (size_t)(block_threads / 64) * 64 * 17 * sizeof(float);
+}
+
+template <int BPV, int B>
+static int launch_search_t(const SearchArgs &a, int grid, size_t lds, hipStream_t st) {
+    auto k = search_kernel<BPV, B>;
+    if (hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) return -1;
+    hipLaunchKernelGGL(k, dim3(grid), dim3(B), lds, st, a);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+template <int BPV, int B>
+static int occ_t(size_t lds, int *bpc) {
+    auto k = search_kernel<BPV, B>;
+    if (hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) return -1;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(bpc, (const void *)k, B, lds) == hipSuccess ? 0 : -1;
+}
+
+#define DVC_DISPATCH(BPV_, B_, CALL)                                  \
+    if (bpv == BPV_ && block_threads == B_) return CALL<BPV_, B_>
+
+int launch_search(const SearchArgs &a, int block_threads, int grid, size_t lds, void *stream) {
+    hipStream_t st = (hipStream_t)stream;
+    int bpv = a.bpv;
+    DVC_DISPATCH(1, 64, launch_search_t)(a, grid, lds, st);
+    DVC_DISPATCH(1, 128, launch_search_t)(a, grid, lds, st);
+    DVC_DISPATCH(1, 256, launch_search_t)(a, grid, lds, st);
+    DVC_DISPATCH(1, 512, launch_search_t)(a, grid, lds, st);
+    DVC_DISPATCH(1, 1024, launch_search_t)(a, grid, lds, st);
+    DVC_DISPATCH(2, 64, launch_search_t)(a, grid, lds, st);
+    DVC_DISPATCH(2, 128, launch_search_t)(a, grid, lds, st);
+    DVC_DISPATCH(2, 256, launch_search_t)(a, grid, lds, st);
+    DVC_DISPATCH(2, 512, launch_search_t)(a, grid, lds, st);
+    DVC_DISPATCH(2, 1024, launch_search_t)(a, grid, lds, st);
+    return -1;
+}
+
+int search_max_blocks(int bpv, int block_threads, size_t lds, int *bpc) {
+    DVC_DISPATCH(1, 64, occ_t)(lds, bpc);
+    DVC_DISPATCH(1, 128, occ_t)(lds, bpc);
+    DVC_DISPATCH(1, 256, occ_t)(lds, bpc);
+    DVC_DISPATCH(1, 512, occ_t)(lds, bpc);
+    DVC_DISPATCH(1, 1024, occ_t)(lds, bpc);
+    DVC_DISPATCH(2, 64, occ_t)(lds, bpc);
+    DVC_DISPATCH(2, 128, occ_t)(lds, bpc);
+    DVC_DISPATCH(2, 256, occ_t)(lds, bpc);
+    DVC_DISPATCH(2, 512, occ_t)(lds, bpc);
+    DVC_DISPATCH(2, 1024, occ_t)(lds, bpc);
+    return -1;
+}
+
+// ------------------------------------------------------ interpolation kernel
+
+template <int BPV>
+__global__ void interp_kernel(VolView v, BoxView bx, const double *xyz, int n, int method,
+                              double *out, double *grad) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float gx, gy, gz;
+    float val = interp_at<BPV>(v, bx, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], method, grad != nullptr,
+                               gx, gy, gz);
+    out[i] = (double)val;
+    if (grad) { grad[3 * i] = gx; grad[3 * i + 1] = gy; grad[3 * i + 2] = gz; }
+}
+
+int launch_interp(const uint8_t *vol, int bpv, int W, int H, int T, const int b0[3], const int box[3],
+                  const double *xyz, int n, int method, double *out, double *grad, void *stream) {
+    (void)T;
+    VolView v = {vol, W, (long long)W * H};
+    BoxView bx = {{b0[0], b0[1], b0[2]}, {box[0], box[1], box[2]}};
+    int blocks = (n + 255) / 256;
+    if (blocks == 0) return 0;
+    if (bpv == 1) hipLaunchKernelGGL(interp_kernel<1>, dim3(blocks), dim3(256), 0, (hipStream_t)stream, v, bx, xyz, n, method, out, grad);
+    else hipLaunchKernelGGL(interp_kernel<2>, dim3(blocks), dim3(256), 0, (hipStream_t)stream, v, bx, xyz, n, method, out, grad);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// ------------------------------------------------------------ objective kernel
+
+// obj_* of ObjectiveFunctions.cpp:24-263 for one pair of sample vectors (FP64), one
+// workgroup of 1024 threads.
+__global__ void __launch_bounds__(1024) objective_kernel(int kind, const double *r, const double *t, int n,
+                                                         double *obj, double *res) {
+    __shared__ double red[16 * 8];
+    __shared__ double sums[8];
+    const int tid = threadIdx.x;
+    double s[2] = {0.0, 0.0};
+    for (int i = tid; i < n; i += 1024) { s[0] += r[i]; s[1] += t[i]; }
+    block_sum<1024, 2>(s, red, sums);
+    double ar = sums[0] / n, at = sums[1] / n;
+    double q[2] = {0.0, 0.0};
+    for (int i = tid; i < n; i += 1024) {
+        if (kind == DVC_OBJ_NSSD) { q[0] += r[i] * r[i]; q[1] += t[i] * t[i]; }
+        else { q[0] += (r[i] - ar) * (r[i] - ar); q[1] += (t[i] - at) * (t[i] - at); }
+    }
+    block_sum<1024, 2>(q, red, &sums[2]);
+    double sr = sqrt(sums[2]), stt = sqrt(sums[3]);
+    double o[1] = {0.0};
+    for (int i = tid; i < n; i += 1024) {
+        double d;
+        if (kind == DVC_OBJ_SAD || kind == DVC_OBJ_SSD) d = t[i] - r[i];
+        else if (kind == DVC_OBJ_ZSSD) d = (t[i] - at) - (r[i] - ar);
+        else if (kind == DVC_OBJ_NSSD) d = t[i] / stt - r[i] / sr;
+        else d = ((t[i] - at) / stt) - ((r[i] - ar) / sr);
+        if (res) res[i] = d;
+        o[0] += (kind == DVC_OBJ_SAD) ? fabs(d) : d * d;
+    }
+    block_sum<1024, 1>(o, red, &sums[4]);
+    if (tid == 0) {
+        double v = sums[4];
+        if (kind == DVC_OBJ_NSSD) v /= 2.0;
+        if (kind == DVC_OBJ_ZNSSD) v /= 4.0;
+        *obj = v;
+    }
+}
+
+int launch_objective(int kind, const double *ref, const double *tar, int n, double *obj, double *res, void *stream) {
+    hipLaunchKernelGGL(objective_kernel, dim3(1), dim3(1024), 0, (hipStream_t)stream, kind, ref, tar, n, obj, res);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// ------------------------------------------------------------- small kernels
+
+__global__ void zero_int_kernel(int *p, long long n) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long st = (long long)gridDim.x * blockDim.x;
+    for (; i < n; i += st) p[i] = 0;
+}
+
+int launch_zero_int(int *p, long long n, void *stream) {
+    if (n <= 0) return 0;
+    long long blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(zero_int_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, p, n);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+__global__ void swap16_kernel(uint16_t *p, long long n) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long st = (long long)gridDim.x * blockDim.x;
+    for (; i < n; i += st) { uint16_t v = p[i]; p[i] = (uint16_t)((v >> 8) | (v << 8)); }
+}
+
+int launch_swap16(uint16_t *p, long long n, void *stream) {
+    if (n <= 0) return 0;
+    long long blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(swap16_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, p, n);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// ---------------------------------------------------------------- strain fit
+
+// FullPivHouseholderQR(A).solve(b) for the 10x10 normal matrix (Eigen 3.4
+// FullPivHouseholderQR, used at strain.cpp:533).  Q is column major, overwritten.
+__device__ void fullpiv_factor(double *Q, int n, double *hc, int *rt, int *ct, int *rank_out) {
+    const double prec = 2.220446049250313e-16 * (double)n;
+    double biggest = 0.0, maxpivot = 0.0;
+    int nonzero = n;
+    for (int k = 0; k < n; k++) {
+        int rb = k, cb = k;
+        double sc = -1.0;
+        for (int j = k; j < n; j++)
+            for (int i = k; i < n; i++) {
+                double a = fabs(Q[j * n + i]);
+                if (a > sc) { sc = a; rb = i; cb = j; }
+            }
+        if (k == 0) biggest = sc;
+        if (sc <= biggest * prec) {
+            nonzero = k;
+            for (int i = k; i < n; i++) { rt[i] = i; ct[i] = i; hc[i] = 0.0; }
+            break;
+        }
+        rt[k] = rb; ct[k] = cb;
+        if (k != rb)
+            for (int j = k; j < n; j++) { double t = Q[j * n + k]; Q[j * n + k] = Q[j * n + rb]; Q[j * n + rb] = t; }
+        if (k != cb)
+            for (int i = 0; i < n; i++) { double t = Q[k * n + i]; Q[k * n + i] = Q[cb * n + i]; Q[cb * n + i] = t; }
+        double *col = &Q[k * n + k];
+        int m = n - k;
+        double tail = 0.0;
+        for (int i = 1; i < m; i++) tail += col[i] * col[i];
+        double c0 = col[0], tau, beta;
+        if (tail <= 2.2250738585072014e-308) {
+            tau = 0.0; beta = c0;
+            for (int i = 1; i < m; i++) col[i] = 0.0;
+        } else {
+            beta = sqrt(c0 * c0 + tail);
+            if (c0 >= 0.0) beta = -beta;
+            for (int i = 1; i < m; i++) col[i] = col[i] / (c0 - beta);
+            tau = (beta - c0) / beta;
+        }
+        hc[k] = tau;
+        col[0] = beta;
+        if (fabs(beta) > maxpivot) maxpivot = fabs(beta);
+        if (m == 1) {
+            for (int j = k + 1; j < n; j++) Q[j * n + k] *= (1.0 - tau);
+        } else if (tau != 0.0) {
+            for (int j = k + 1; j < n; j++) {
+                double *cj = &Q[j * n + k];
+                double t = 0.0;
+                for (int i = 1; i < m; i++) t += col[i] * cj[i];
+                t += cj[0];
+                cj[0] -= tau * t;
+                for (int i = 1; i < m; i++) cj[i] -= tau * col[i] * t;
+            }
+        }
+    }
+    double pthr = fabs(maxpivot) * (2.220446049250313e-16 * (double)n);
+    int rank = 0;
+    for (int i = 0; i < nonzero; i++) rank += (fabs(Q[i * n + i]) > pthr);
+    *rank_out = rank;
+}
+
+__device__ void fullpiv_apply(const double *Q, int n, const double *hc, const int *rt, const int *ct, int rank,
+                              const double *b, double *x) {
+    double c[10];
+    int perm[10];
+    for (int i = 0; i < n; i++) { perm[i] = i; x[i] = 0.0; c[i] = b[i]; }
+    for (int k = 0; k < n; k++) { int t = perm[k]; perm[k] = perm[ct[k]]; perm[ct[k]] = t; }
+    if (rank == 0) return;
+    for (int k = 0; k < rank; k++) {
+        double t0 = c[k]; c[k] = c[rt[k]]; c[rt[k]] = t0;
+        int m = n - k;
+        double tau = hc[k];
+        if (m == 1) { c[k] *= (1.0 - tau); continue; }
+        if (tau == 0.0) continue;
+        double t = 0.0;
+        for (int i = 1; i < m; i++) t += Q[k * n + k + i] * c[k + i];
+        t += c[k];
+        c[k] -= tau * t;
+        for (int i = 1; i < m; i++) c[k + i] -= tau * Q[k * n + k + i] * t;
+    }
+    for (int i = rank - 1; i >= 0; i--) {
+        double s = c[i];
+        for (int j = i + 1; j < rank; j++) s -= Q[j * n + i] * c[j];
+        c[i] = s / Q[i * n + i];
+    }
+    for (int i = 0; i < rank; i++) x[perm[i]] = c[i];
+}
+
+// Eigenvalues of a symmetric 3x3 (cyclic Jacobi), sorted descending
+// (strain.cpp:553-556 takes EigenSolver real parts and sorts them).
+__device__ void sym3_eigen(const double *S, double *lam) {
+    double a[3][3];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) a[i][j] = 0.5 * (S[i * 3 + j] + S[j * 3 + i]);
+    for (int sweep = 0; sweep < 50; sweep++) {
+        double off = fabs(a[0][1]) + fabs(a[0][2]) + fabs(a[1][2]);
+        if (off == 0.0) break;
+        for (int p = 0; p < 2; p++)
+            for (int q = p + 1; q < 3; q++) {
+                if (a[p][q] == 0.0) continue;
+                double theta = (a[q][q] - a[p][p]) / (2.0 * a[p][q]);
+                double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+                for (int k = 0; k < 3; k++) {
+                    double akp = a[k][p], akq = a[k][q];
+                    a[k][p] = c * akp - s * akq;
+                    a[k][q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < 3; k++) {
+                    double apk = a[p][k], aqk = a[q][k];
+                    a[p][k] = c * apk - s * aqk;
+                    a[q][k] = s * apk + c * aqk;
+                }
+            }
+    }
+    double l0 = a[0][0], l1 = a[1][1], l2 = a[2][2], t;
+    if (l0 < l1) { t = l0; l0 = l1; l1 = t; }
+    if (l1 < l2) { t = l1; l1 = l2; l2 = t; }
+    if (l0 < l1) { t = l0; l0 = l1; l1 = t; }
+    lam[0] = l0; lam[1] = l1; lam[2] = l2;
+}
+
+// One wavefront per point: window selection (strain.cpp:424-448), design rows
+// (:478-527), normal equations by lanes, FP64 solve and strain tensors (:529-590).
+constexpr int kStrainWaves = 4;
+__global__ void __launch_bounds__(64 * kStrainWaves) strain_kernel(StrainArgs a) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int i = blockIdx.x * kStrainWaves + w;
+    __shared__ double Xs[kStrainWaves][75][10];
+    __shared__ double Ds[kStrainWaves][75][3];
+    __shared__ double Ms[kStrainWaves][100 + 30];
+    __shared__ int sel[kStrainWaves][76];
+    if (i >= a.n) return;      // whole wave leaves together; no block barrier below
+    const int *nb = a.neigh + (long long)i * a.k;
+    double px = a.pts[3 * i], py = a.pts[3 * i + 1], pz = a.pts[3 * i + 2];
+    double distance = sqrt(px * px + py * py + pz * pz);
+    double scale = 1.0 / distance;
+    double bx = scale * px, by = scale * py, bz = scale * pz;
+    if (lane == 0) {
+        int lim = a.refill ? (a.k < 75 ? a.k : 75) : (a.ndp < a.k ? a.ndp : a.k);
+        int np = 0;
+        for (int q = 0; q < lim && np < a.ndp; q++) {
+            int m = nb[q];
+            if (m >= 0 && m < a.n && a.status[m] == 0 && a.objmin[m] <= a.objt) sel[w][np++] = m;
+        }
+        sel[w][75] = np;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int ngp = sel[w][75];
+    if (lane < ngp) {
+        int m = sel[w][lane];
+        double x = scale * a.pts[3 * m], y = scale * a.pts[3 * m + 1], z = scale * a.pts[3 * m + 2];
+        double *X = Xs[w][lane];
+        X[0] = 1; X[1] = x; X[2] = y; X[3] = z;
+        X[4] = x * x; X[5] = y * y; X[6] = z * z;
+        X[7] = x * y; X[8] = y * z; X[9] = x * z;
+        double *D = Ds[w][lane];
+        D[0] = scale * a.disp[3 * m]; D[1] = scale * a.disp[3 * m + 1]; D[2] = scale * a.disp[3 * m + 2];
+        if (a.yz) { X[1] = 0.0; X[4] = 0.0; X[7] = 0.0; X[9] = 0.0; D[0] = 0.0; }
+        if (a.zx) { X[2] = 0.0; X[5] = 0.0; X[7] = 0.0; X[8] = 0.0; D[1] = 0.0; }
+        if (a.xy) { X[3] = 0.0; X[6] = 0.0; X[8] = 0.0; X[9] = 0.0; D[2] = 0.0; }
+    }
+    if (lane + 64 < ngp) {
+        // windows are at most 75 points: lanes 0..10 also build rows 64..74
+        int r = lane + 64;
+        int m = sel[w][r];
+        double x = scale * a.pts[3 * m], y = scale * a.pts[3 * m + 1], z = scale * a.pts[3 * m + 2];
+        double *X = Xs[w][r];
+        X[0] = 1; X[1] = x; X[2] = y; X[3] = z;
+        X[4] = x * x; X[5] = y * y; X[6] = z * z;
+        X[7] = x * y; X[8] = y * z; X[9] = x * z;
+        double *D = Ds[w][r];
+        D[0] = scale * a.disp[3 * m]; D[1] = scale * a.disp[3 * m + 1]; D[2] = scale * a.disp[3 * m + 2];
+        if (a.yz) { X[1] = 0.0; X[4] = 0.0; X[7] = 0.0; X[9] = 0.0; D[0] = 0.0; }
+        if (a.zx) { X[2] = 0.0; X[5] = 0.0; X[7] = 0.0; X[8] = 0.0; D[1] = 0.0; }
+        if (a.xy) { X[3] = 0.0; X[6] = 0.0; X[8] = 0.0; X[9] = 0.0; D[2] = 0.0; }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // X^T X (100 entries, column-major into Ms[0..99]) and X^T d (30 entries)
+    for (int e = lane; e < 130; e += 64) {
+        double s = 0.0;
+        if (e < 100) {
+            int r = e % 10, c = e / 10;
+            for (int j = 0; j < ngp; j++) s += Xs[w][j][r] * Xs[w][j][c];
+        } else {
+            int r = (e - 100) % 10, c = (e - 100) / 10;
+            for (int j = 0; j < ngp; j++) s += Xs[w][j][r] * Ds[w][j][c];
+        }
+        Ms[w][e] = s;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (lane != 0) return;
+    double Q[100], hc[10];
+    int rt[10], ct[10], rank;
+    for (int e = 0; e < 100; e++) Q[e] = Ms[w][e];
+    fullpiv_factor(Q, 10, hc, rt, ct, &rank);
+    double DG[9], displ[3];
+    for (int c = 0; c < 3; c++) {
+        double par[10];
+        fullpiv_apply(Q, 10, hc, rt, ct, rank, &Ms[w][100 + 10 * c], par);
+        DG[c * 3 + 0] = par[1] + 2 * par[4] * bx + par[7] * by + par[9] * bz;
+        DG[c * 3 + 1] = par[2] + 2 * par[5] * by + par[7] * bx + par[8] * bz;
+        DG[c * 3 + 2] = par[3] + 2 * par[6] * bz + par[8] * by + par[9] * bx;
+        displ[c] = distance * (par[0] + par[1] * bx + par[2] * by + par[3] * bz + par[4] * bx * bx +
+                               par[5] * by * by + par[6] * bz * bz + par[7] * bx * by + par[8] * by * bz +
+                               par[9] * bx * bz);
+    }
+    a.out_npts[i] = ngp;
+    if (ngp == 0) {
+        a.out_swr[i] = 0.0;
+    } else {
+        int m = sel[w][ngp - 1];
+        double dx = bx - scale * a.pts[3 * m], dy = by - scale * a.pts[3 * m + 1], dz = bz - scale * a.pts[3 * m + 2];
+        a.out_swr[i] = distance * sqrt(dx * dx + dy * dy + dz * dz);
+    }
+    for (int c = 0; c < 3; c++) a.out_disp[3 * i + c] = displ[c];
+    for (int q = 0; q < 9; q++) a.out_DG[9 * i + q] = DG[q];
+    const bool planar = a.xy || a.yz || a.zx;
+    for (int pass = 0; pass < 2; pass++) {
+        double ST[9];
+        for (int r = 0; r < 3; r++)
+            for (int c = 0; c < 3; c++) {
+                double v = DG[r * 3 + c] + DG[c * 3 + r];
+                if (pass == 1) {
+                    double s = 0.0;
+                    for (int l = 0; l < 3; l++) s += DG[l * 3 + r] * DG[l * 3 + c];
+                    v += s;
+                }
+                ST[r * 3 + c] = 0.5 * v;
+            }
+        double lam[3];
+        sym3_eigen(ST, lam);
+        if (planar) {
+            if (lam[0] == 0.0) { lam[0] = lam[1]; lam[1] = lam[2]; }
+            if (lam[1] == 0.0) { lam[1] = lam[2]; }
+            lam[2] = 0.0;
+        }
+        double *o = (pass == 0 ? a.out_E : a.out_L) + 9 * i;
+        o[0] = ST[0]; o[1] = ST[4]; o[2] = ST[8]; o[3] = ST[1]; o[4] = ST[5]; o[5] = ST[2];
+        o[6] = lam[0]; o[7] = lam[1]; o[8] = lam[2];
+    }
+}
+
+int launch_strain(const StrainArgs &a, void *stream) {
+    if (a.n <= 0) return 0;
+    int blocks = (a.n + kStrainWaves - 1) / kStrainWaves;
+    hipLaunchKernelGGL(strain_kernel, dim3(blocks), dim3(64 * kStrainWaves), 0, (hipStream_t)stream, a);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+}  // namespace dvc

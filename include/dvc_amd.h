@@ -1,0 +1,178 @@
+/*
+ * dvc_amd.h -- C ABI of the MI355X (gfx950) implementation of the CCPi DVC
+ * per-point correlation path.
+ *
+ * Plain pointers and sizes only.  All entry points return 0 on success and a
+ * negative DVC_ERR_* code on failure; dvc_last_error() describes the last failure.
+ * No C++ exception crosses this boundary.  One context per host thread.
+ *
+ * Each declaration names the reference interface it replaces (file:line in the
+ * TomographicImaging/DigitalVolumeCorrelation sources, Core/ directory).
+ */
+#ifndef DVC_AMD_H
+#define DVC_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Point status codes, identical to Core/Utility.h:38-41. */
+#define DVC_POINT_GOOD   0
+#define DVC_RANGE_FAIL  -1
+#define DVC_CONVG_FAIL  -2
+#define DVC_NOT_SEARCH  -3
+
+/* Return codes. */
+#define DVC_OK            0
+#define DVC_ERR_ARG      -1   /* invalid argument or size */
+#define DVC_ERR_HIP      -2   /* HIP runtime failure */
+#define DVC_ERR_STATE    -3   /* call out of order (e.g. no volume uploaded) */
+#define DVC_ERR_TIMEOUT  -4   /* a device-side dependency wait exceeded its spin limit */
+#define DVC_ERR_NODEV    -5   /* no usable gfx950 device */
+#define DVC_ERR_RANGE    -6   /* dvc_interpolate: points outside the active box (Intrp_Fail) */
+
+/* Objective functions, in the order of Core/Utility.h:65-68. */
+#define DVC_OBJ_SAD   0
+#define DVC_OBJ_SSD   1
+#define DVC_OBJ_ZSSD  2
+#define DVC_OBJ_NSSD  3
+#define DVC_OBJ_ZNSSD 4
+
+/* Interpolation methods, in the order of Core/Utility.h:70-73. */
+#define DVC_INTERP_NEAREST   0
+#define DVC_INTERP_TRILINEAR 1
+#define DVC_INTERP_TRICUBIC  2
+
+/* Sub-volume geometry, Core/Utility.h:55-58. */
+#define DVC_GEOM_CUBE   0
+#define DVC_GEOM_SPHERE 1
+
+/* Volume slots: the reference (RunControl::ref_fname) and the correlated
+ * (RunControl::cor_fname) image volumes. */
+#define DVC_VOL_REF 0
+#define DVC_VOL_COR 1
+
+typedef struct dvc_ctx dvc_ctx;
+
+/* The RunControl fields read by Search/Interpolate (Core/Utility.h:90-134). */
+typedef struct {
+    int32_t ndof;            /* num_srch_dof: 3, 6 or 12 */
+    int32_t obj;             /* DVC_OBJ_* */
+    int32_t interp;          /* DVC_INTERP_* */
+    int32_t geom;            /* DVC_GEOM_* */
+    int32_t subvol_size;     /* side or diameter in voxels */
+    int32_t subvol_npts;     /* requested samples (cube: rounded up to a cube, Search.cpp:30) */
+    int32_t disp_max;        /* voxels */
+    uint32_t sphere_seed;    /* mt19937 seed of every sphere cloud (Search.cpp:102-105 uses 1) */
+    double basin_radius;     /* 0 disables the translation grid (Search.cpp:941) */
+    double rigid_trans[3];   /* start guess without converged neighbours (Search.cpp:351-356) */
+    double aspect[3];        /* subvol_aspect */
+} dvc_run_params;
+
+/* One search result: ResultRecord (Core/DataCloud.h:45-56) plus the iteration count. */
+typedef struct {
+    int32_t status;          /* DVC_POINT_GOOD / DVC_RANGE_FAIL / DVC_NOT_SEARCH */
+    int32_t iters;           /* Gauss-Newton evaluations (LM_prep_at calls) */
+    double  obj;             /* obj_min, 0 unless good */
+    double  par[12];         /* par_min: u v w phi the psi exx eyy ezz exy eyz exz */
+} dvc_point_result;
+
+/* ---------------------------------------------------------------- context */
+
+/* Create a context on HIP device `device` (replaces nothing: the reference is a
+ * single-process CPU program; this owns the device buffers, stream and events). */
+int  dvc_create(dvc_ctx **ctx, int device);
+void dvc_destroy(dvc_ctx *ctx);
+const char *dvc_last_error(const dvc_ctx *ctx);
+/* Run subsequent work on an existing hipStream_t (NULL: the context's own stream). */
+int  dvc_set_stream(dvc_ctx *ctx, void *hip_stream);
+/* Milliseconds of the last search / strain kernel, timed with HIP events on the
+ * stream the kernel ran on. */
+int  dvc_last_kernel_ms(const dvc_ctx *ctx, float *ms);
+
+/* ---------------------------------------------------------------- volumes */
+
+/* Make an image volume resident in HBM in its native uint8 / uint16 form.
+ * Replaces the per-point disk reads of Interpolate::kernels (Interpolate.cpp:144-193):
+ * `host` holds the raw file contents after `header_bytes`; big_endian swaps uint16
+ * as Interpolate.cpp:233-242 does.  Layout: x fastest, then y, then z. */
+int dvc_volume_upload(dvc_ctx *ctx, int slot, const void *host, int64_t header_bytes,
+                      int bytes_per, int big_endian, int64_t wide, int64_t high, int64_t tall);
+/* Borrow a volume that is already in device memory (x fastest, native endianness,
+ * readable for 16 bytes past its end).  The caller keeps ownership. */
+int dvc_volume_attach(dvc_ctx *ctx, int slot, const void *device_ptr, int bytes_per,
+                      int64_t wide, int64_t high, int64_t tall);
+
+/* ---------------------------------------------------------- interpolation */
+
+/* Interpolate::center_on + Interpolate::kernels + tri_cub_Lek / tri_lin / nearest
+ * (Interpolate.cpp:569-574, 144-258, 472-567, 429-470, 404-427): place the
+ * interpolation box of `rc` (Search.cpp:64-80) on `center`, then evaluate the volume
+ * in `slot` at n points xyz[3n] whose bounding box is bbox[6] (min xyz, max xyz).
+ * out[n] receives values; grad[3n] (may be NULL, tricubic only) the x,y,z derivatives.
+ * Returns DVC_ERR_RANGE where tri_cub_Lek throws Intrp_Fail. */
+int dvc_interpolate(dvc_ctx *ctx, int slot, const dvc_run_params *rc, const double center[3],
+                    const double *xyz, int32_t n, const double bbox[6], int method,
+                    double *out, double *grad);
+
+/* obj_SAD / obj_SSD / obj_ZSSD / obj_NSSD / obj_ZNSSD with residual
+ * (ObjectiveFunctions.h:31-45, ObjectiveFunctions.cpp:24-263), on the device in FP64.
+ * residual[n] may be NULL. */
+int dvc_objective(dvc_ctx *ctx, int kind, const double *ref, const double *tar, int32_t n,
+                  double *obj, double *residual);
+
+/* ----------------------------------------------------------------- search */
+
+/* The dvc point loop (dvc.cpp:229-303) around Search::process_point
+ * (Search.cpp:95-192): points[3*npts]; order[npts] = DataCloud::order; neigh[npts*k]
+ * = DataCloud::neigh (k = min(75, npts), self first); the first nproc entries of order
+ * are searched (nproc <= 1 or >= npts: all, dvc.cpp:229-232).  Every point's start
+ * guess averages the converged translations of its neighbours searched before it
+ * (Search.cpp:277-377), so results match the serial loop.  results[npts]. */
+int dvc_search_points(dvc_ctx *ctx, const dvc_run_params *rc, const double *points,
+                      int32_t npts, const int32_t *order, int32_t nproc,
+                      const int32_t *neigh, int32_t k, dvc_point_result *results);
+
+/* The same loop split for callers that keep inputs resident: prepare uploads the
+ * point cloud and its order/neighbour tables; execute launches the search on the
+ * context stream without synchronising; fetch waits and copies results back. */
+int dvc_search_prepare(dvc_ctx *ctx, const dvc_run_params *rc, const double *points,
+                       int32_t npts, const int32_t *order, int32_t nproc,
+                       const int32_t *neigh, int32_t k);
+int dvc_search_execute(dvc_ctx *ctx);
+int dvc_search_fetch(dvc_ctx *ctx, dvc_point_result *results);
+
+/* Launch tuning: threads per point (64, 128, 256, 512 or 1024; 0 = automatic). */
+int dvc_set_block_threads(dvc_ctx *ctx, int threads);
+/* Upper bound, in seconds, on one device-side wait for a neighbour's result
+ * (default 30).  A wait that exceeds it ends with DVC_ERR_TIMEOUT from the fetch. */
+int dvc_set_wait_limit(dvc_ctx *ctx, double seconds);
+
+/* ----------------------------------------------------------------- strain */
+
+/* The strain-window quadratic fit of strain.cpp:402-591 for n points: pts[3n],
+ * disp[3n], status[n], objmin[n], neigh[n*k]; ndp window size (10..75,
+ * strain.cpp:204-207), objt threshold, refill (-r), planar bit mask (1 yz, 2 zx,
+ * 4 xy; auto-detected planes are added as strain.cpp:340-365).  Outputs:
+ * fit displacement [3n], points in window [n], window radius [n], engineering and
+ * Lagrangian strain [9n] (exx eyy ezz exy eyz exz p1 p2 p3), gradient [9n]. */
+int dvc_strain_fit(dvc_ctx *ctx, const double *pts, const double *disp, const int32_t *status,
+                   const double *objmin, const int32_t *neigh, int32_t k, int32_t n,
+                   int32_t ndp, double objt, int32_t refill, int32_t planar,
+                   double *out_disp, int32_t *out_npts, double *out_swr,
+                   double *out_E, double *out_L, double *out_DG);
+
+/* --------------------------------------------------------- point ordering */
+
+/* DataCloud::sort_order_neighbors (DataCloud.cpp:69-144) on the host: the processing
+ * order by distance from start[3] and the k = min(75, n) nearest neighbours of every
+ * point, with std::sort tie order.  order[n], neigh[n*k]; returns k. */
+int dvc_sort_order_neighbors(const double *pts, int32_t n, const double start[3],
+                             int32_t *order, int32_t *neigh, int32_t nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
