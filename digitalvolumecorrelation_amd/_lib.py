@@ -18,7 +18,7 @@ INTERP = {"nearest": 0, "trilinear": 1, "tricubic": 2}
 GEOM = {"cube": 0, "sphere": 1}
 
 ERRORS = {0: "ok", -1: "invalid argument", -2: "HIP failure", -3: "bad call order",
-          -4: "dependency wait timed out", -5: "no gfx950 device", -6: "outside the interpolation box"}
+          -5: "no gfx950 device", -6: "outside the interpolation box"}
 ERR_RANGE = -6
 
 
@@ -69,7 +69,6 @@ def load() -> C.CDLL:
         "dvc_search_execute": (I, [VP]),
         "dvc_search_fetch": (I, [VP, VP]),
         "dvc_set_block_threads": (I, [VP, C.c_int]),
-        "dvc_set_wait_limit": (I, [VP, C.c_double]),
         "dvc_strain_fit": (I, [VP, P(D), P(D), P(I), P(D), P(I), I, I, I, D, I, I,
                                P(D), P(I), P(D), P(D), P(D), P(D)]),
         "dvc_sort_order_neighbors": (I, [P(D), I, P(D), P(I), P(I), I]),
@@ -196,9 +195,6 @@ class Context:
     # ---- search
     def set_block_threads(self, n: int):
         self._check(self.lib.dvc_set_block_threads(self.h, n))
-
-    def set_wait_limit(self, seconds: float):
-        self._check(self.lib.dvc_set_wait_limit(self.h, seconds))
 
     def prepare(self, params, points, order, neigh, nproc=0):
         self._pts = np.ascontiguousarray(points, dtype=np.float64)

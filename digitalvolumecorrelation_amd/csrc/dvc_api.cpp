@@ -16,31 +16,11 @@
 #include <thread>
 #include <vector>
 
-#include "dvc_internal.h"
+#include "dvc_geometry.h"
 
 using namespace dvc;
 
 namespace {
-
-// Point / BoundBox arithmetic of Core/Point.cpp:57-66 and Core/BoundBox.cpp.
-inline int trunc_i(double v) { return (int)(v / 1); }
-
-struct HBox {
-    double mn[3], mx[3], cn[3];
-    HBox(const double a[3], const double b[3]) {
-        for (int q = 0; q < 3; q++) { mn[q] = a[q]; mx[q] = b[q]; cn[q] = (a[q] + b[q]) / 2.0; }
-    }
-    void grow_by(double f) {
-        for (int q = 0; q < 3; q++) { mn[q] = mn[q] + -f; mx[q] = mx[q] + f; }
-        for (int q = 0; q < 3; q++) cn[q] = (mn[q] + mx[q]) / 2.0;
-    }
-    void center_on(const double c[3]) {
-        for (int q = 0; q < 3; q++) {
-            double d = c[q] - cn[q];
-            mn[q] = mn[q] + d; mx[q] = mx[q] + d; cn[q] = cn[q] + d;
-        }
-    }
-};
 
 struct Volume {
     uint8_t *d = nullptr;
@@ -48,37 +28,6 @@ struct Volume {
     int bpv = 0;
     int64_t W = 0, H = 0, T = 0;
 };
-
-// Run geometry derived from dvc_run_params (Search.cpp:28-80).
-struct Geometry {
-    double subv_rad;
-    int subv_num, g;
-    double est_mn[3], est_mx[3], est_cn[3];
-    double act_mn[3], act_mx[3], act_cn[3];
-    int box[3];
-};
-
-Geometry make_geometry(const dvc_run_params &rc) {
-    Geometry G;
-    G.subv_rad = rc.subvol_size / 2.0;
-    if (rc.geom == DVC_GEOM_CUBE) G.subv_num = (int)pow(ceil(pow((double)rc.subvol_npts, 1.0 / 3.0)), 3.0);
-    else G.subv_num = rc.subvol_npts;
-    G.g = (int)ceil(pow((double)G.subv_num, 1.0 / 3.0));
-    double z[3] = {0.0, 0.0, 0.0};
-    double m[3] = {2 * G.subv_rad * rc.aspect[0], 2 * G.subv_rad * rc.aspect[1], 2 * G.subv_rad * rc.aspect[2]};
-    HBox est(z, m);
-    est.grow_by(rc.disp_max);
-    est.grow_by(2.0);
-    HBox e2(est.mn, est.mx);                     // Interpolate.cpp:51
-    HBox act(est.mn, est.mx);                    // Interpolate.cpp:52-55
-    act.grow_by(-1.0);
-    for (int q = 0; q < 3; q++) {
-        G.est_mn[q] = e2.mn[q]; G.est_mx[q] = e2.mx[q]; G.est_cn[q] = e2.cn[q];
-        G.act_mn[q] = act.mn[q]; G.act_mx[q] = act.mx[q]; G.act_cn[q] = act.cn[q];
-        G.box[q] = trunc_i(e2.mx[q]) - trunc_i(e2.mn[q]);
-    }
-    return G;
-}
 
 }  // namespace
 
@@ -91,9 +40,7 @@ struct dvc_ctx {
     Volume vol[2];
     int cus = 0;
     size_t lds_max = 0;
-    int wall_khz = 0;
     int block_threads = 0;
-    double wait_seconds = 30.0;
     float last_ms = 0.0f;
     // search plan
     bool plan_ready = false;
@@ -101,11 +48,12 @@ struct dvc_ctx {
     Geometry geo{};
     int npts = 0, nproc = 0, k = 0;
     double *d_pts = nullptr, *d_sph = nullptr;
-    int *d_order = nullptr, *d_pos = nullptr, *d_neigh = nullptr, *d_done = nullptr, *d_ctrl = nullptr;
+    int *d_order = nullptr, *d_pos = nullptr, *d_neigh = nullptr, *d_levels = nullptr;
     dvc_point_result *d_res = nullptr;
     double off_min[3]{}, off_max[3]{};
     int chosen_threads = 256;
     std::vector<char> searched;
+    std::vector<int> level_off;      // dependency levels (dvc_geometry.h)
 };
 
 namespace {
@@ -135,9 +83,9 @@ int dev_alloc(dvc_ctx *ctx, T **p, size_t count) {
 
 void free_plan(dvc_ctx *c) {
     hipFree(c->d_pts); hipFree(c->d_sph); hipFree(c->d_order); hipFree(c->d_pos);
-    hipFree(c->d_neigh); hipFree(c->d_done); hipFree(c->d_ctrl); hipFree(c->d_res);
+    hipFree(c->d_neigh); hipFree(c->d_levels); hipFree(c->d_res);
     c->d_pts = c->d_sph = nullptr;
-    c->d_order = c->d_pos = c->d_neigh = c->d_done = c->d_ctrl = nullptr;
+    c->d_order = c->d_pos = c->d_neigh = c->d_levels = nullptr;
     c->d_res = nullptr;
     c->plan_ready = false;
 }
@@ -186,9 +134,6 @@ int dvc_create(dvc_ctx **out, int device) {
     if (hipSetDevice(device) != hipSuccess) { delete ctx; return DVC_ERR_NODEV; }
     ctx->cus = prop.multiProcessorCount;
     ctx->lds_max = prop.sharedMemPerBlock;
-    int khz = 0;
-    hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device);
-    ctx->wall_khz = khz > 0 ? khz : 100000;
     if (hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
         delete ctx;
@@ -231,12 +176,6 @@ int dvc_set_block_threads(dvc_ctx *ctx, int threads) {
     if (threads != 0 && threads != 64 && threads != 128 && threads != 256 && threads != 512 && threads != 1024)
         return fail(ctx, DVC_ERR_ARG, "threads per point must be 0, 64, 128, 256, 512 or 1024");
     ctx->block_threads = threads;
-    return DVC_OK;
-}
-
-int dvc_set_wait_limit(dvc_ctx *ctx, double seconds) {
-    if (!ctx || !(seconds > 0.0)) return DVC_ERR_ARG;
-    ctx->wait_seconds = seconds;
     return DVC_OK;
 }
 
@@ -361,14 +300,8 @@ int dvc_search_prepare(dvc_ctx *ctx, const dvc_run_params *rc, const double *poi
     // dvc.cpp:229-232
     int N = npts;
     if (nproc > 1 && nproc < npts) N = nproc;
-    std::vector<int> pos(npts, npts);
-    for (int i = 0; i < npts; i++) {
-        int o = order[i];
-        if (o < 0 || o >= npts || pos[o] != npts) return fail(ctx, DVC_ERR_ARG, "order must be a permutation of 0..npts-1");
-        pos[o] = i;
-    }
-    for (int i = 0; i < npts; i++)
-        if (pos[order[i]] >= N) pos[order[i]] = npts;   // never searched
+    std::vector<int> pos;
+    if (!order_positions(order, npts, N, pos)) return fail(ctx, DVC_ERR_ARG, "order must be a permutation of 0..npts-1");
     for (int64_t i = 0; i < (int64_t)npts * k; i++)
         if (neigh[i] < 0 || neigh[i] >= npts) return fail(ctx, DVC_ERR_ARG, "neighbour index out of range");
     for (int i = 0; i < npts * 3; i++)
@@ -382,44 +315,30 @@ int dvc_search_prepare(dvc_ctx *ctx, const dvc_run_params *rc, const double *poi
     // sphere template: identical offsets for every point (one seed, Search.cpp:102-105)
     std::vector<double> sph;
     double omn[3] = {0, 0, 0}, omx[3] = {0, 0, 0};
-    if (rc->geom == DVC_GEOM_SPHERE) {
-        sph.resize((size_t)3 * G.subv_num);
-        std::mt19937 mt(rc->sphere_seed);
-        std::uniform_real_distribution<double> dist(-G.subv_rad, G.subv_rad);
-        int count = 0;
-        while (count < G.subv_num) {
-            double rx = dist(mt), ry = dist(mt), rz = dist(mt);
-            double radius = std::sqrt(rx * rx + ry * ry + rz * rz);
-            if (radius <= G.subv_rad) {
-                sph[3 * count + 0] = rc->aspect[0] * (rx);
-                sph[3 * count + 1] = rc->aspect[1] * (ry);
-                sph[3 * count + 2] = rc->aspect[2] * (rz);
-                count += 1;
-            }
-        }
-        for (int q = 0; q < 3; q++) { omn[q] = 1e300; omx[q] = -1e300; }
-        for (int i = 0; i < G.subv_num; i++)
-            for (int q = 0; q < 3; q++) { omn[q] = std::min(omn[q], sph[3 * i + q]); omx[q] = std::max(omx[q], sph[3 * i + q]); }
-    }
+    if (rc->geom == DVC_GEOM_SPHERE) sphere_template(*rc, G, sph, omn, omx);
     if (int e = dev_alloc(ctx, &ctx->d_pts, (size_t)3 * npts)) return e;
     if (int e = dev_alloc(ctx, &ctx->d_sph, sph.size())) return e;
     if (int e = dev_alloc(ctx, &ctx->d_order, (size_t)npts)) return e;
     if (int e = dev_alloc(ctx, &ctx->d_pos, (size_t)npts)) return e;
     if (int e = dev_alloc(ctx, &ctx->d_neigh, (size_t)npts * k)) return e;
-    if (int e = dev_alloc(ctx, &ctx->d_done, (size_t)npts)) return e;
-    if (int e = dev_alloc(ctx, &ctx->d_ctrl, (size_t)4)) return e;
+    std::vector<int> level_pts, level_off;
+    dependency_levels(order, pos, neigh, npts, k, N, level_pts, level_off);
+    if (int e = dev_alloc(ctx, &ctx->d_levels, level_pts.size())) return e;
     if (int e = dev_alloc(ctx, &ctx->d_res, (size_t)npts)) return e;
     HIPCHK(hipMemcpyAsync(ctx->d_pts, points, sizeof(double) * 3 * npts, hipMemcpyHostToDevice, ctx->stream));
     if (!sph.empty()) HIPCHK(hipMemcpyAsync(ctx->d_sph, sph.data(), sizeof(double) * sph.size(), hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipMemcpyAsync(ctx->d_order, order, sizeof(int) * npts, hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipMemcpyAsync(ctx->d_pos, pos.data(), sizeof(int) * npts, hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipMemcpyAsync(ctx->d_neigh, neigh, sizeof(int) * (size_t)npts * k, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(ctx->d_levels, level_pts.data(), sizeof(int) * level_pts.size(), hipMemcpyHostToDevice,
+                          ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->rc = *rc;
     ctx->geo = G;
     ctx->npts = npts; ctx->nproc = N; ctx->k = k;
     for (int q = 0; q < 3; q++) { ctx->off_min[q] = omn[q]; ctx->off_max[q] = omx[q]; }
     ctx->chosen_threads = bt;
+    ctx->level_off = level_off;
     ctx->searched.assign(npts, 0);
     for (int i = 0; i < npts; i++) ctx->searched[i] = pos[i] < N ? 1 : 0;
     ctx->plan_ready = true;
@@ -434,38 +353,23 @@ int dvc_search_execute(dvc_ctx *ctx) {
     const Geometry &G = ctx->geo;
     const dvc_run_params &rc = ctx->rc;
     SearchArgs a;
-    std::memset(&a, 0, sizeof(a));
+    fill_search_args(a, rc, G, ctx->vol[0].bpv, (int)ctx->vol[0].W, (int)ctx->vol[0].H, (int)ctx->vol[0].T,
+                     ctx->npts, ctx->nproc, ctx->k, ctx->off_min, ctx->off_max);
     a.vol[0] = ctx->vol[0].d; a.vol[1] = ctx->vol[1].d;
-    a.W = (int)ctx->vol[0].W; a.H = (int)ctx->vol[0].H; a.T = (int)ctx->vol[0].T; a.bpv = ctx->vol[0].bpv;
-    a.ndof = rc.ndof; a.obj = rc.obj; a.interp = rc.interp; a.geom = rc.geom;
-    a.nsamp = G.subv_num; a.g = G.g;
-    a.npts = ctx->npts; a.nproc = ctx->nproc; a.k = ctx->k;
-    a.nsamp_pad = (G.subv_num + 3) & ~3;
-    for (int q = 0; q < 3; q++) {
-        a.box[q] = G.box[q];
-        a.est_mn[q] = G.est_mn[q]; a.est_mx[q] = G.est_mx[q]; a.est_cn[q] = G.est_cn[q];
-        a.act_mn[q] = G.act_mn[q]; a.act_mx[q] = G.act_mx[q]; a.act_cn[q] = G.act_cn[q];
-        a.aspect[q] = rc.aspect[q]; a.rigid[q] = rc.rigid_trans[q];
-        a.off_min[q] = ctx->off_min[q]; a.off_max[q] = ctx->off_max[q];
-    }
-    a.subv_rad = G.subv_rad; a.basin = rc.basin_radius; a.disp_max = (double)rc.disp_max;
     a.sph = ctx->d_sph; a.pts = ctx->d_pts; a.order = ctx->d_order; a.pos = ctx->d_pos; a.neigh = ctx->d_neigh;
-    a.res = ctx->d_res; a.done = ctx->d_done; a.counter = ctx->d_ctrl; a.error = ctx->d_ctrl + 1;
-    a.wait_ticks = (long long)(ctx->wait_seconds * (double)ctx->wall_khz * 1000.0);
+    a.res = ctx->d_res;
     int bt = ctx->chosen_threads;
     size_t lds = search_lds_bytes(a.nsamp_pad, bt);
-    int bpc = 0;
-    if (search_max_blocks(a.bpv, bt, lds, &bpc) != 0 || bpc < 1) return fail(ctx, DVC_ERR_HIP, "search kernel does not fit on a CU");
-    long long grid = (long long)bpc * ctx->cus;
-    if (grid > ctx->nproc) grid = ctx->nproc;
-    if (grid < 1) grid = 1;
-    // results start as not_search (DataCloud.cpp:163-172): zero then mark via the kernel
-    if (launch_zero_int(ctx->d_done, ctx->npts, ctx->stream) != 0) return fail(ctx, DVC_ERR_HIP, "zero launch");
-    if (launch_zero_int(ctx->d_ctrl, 4, ctx->stream) != 0) return fail(ctx, DVC_ERR_HIP, "zero launch");
+    // results start zeroed; points never searched become not_search in fetch
     if (launch_zero_int((int *)ctx->d_res, (long long)ctx->npts * (long long)(sizeof(dvc_point_result) / 4), ctx->stream) != 0)
         return fail(ctx, DVC_ERR_HIP, "zero launch");
     HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
-    if (launch_search(a, bt, (int)grid, lds, ctx->stream) != 0) return fail(ctx, DVC_ERR_HIP, "search kernel launch failed");
+    const int depth = (int)ctx->level_off.size() - 1;
+    for (int L = 0; L < depth; L++) {
+        int cnt = ctx->level_off[L + 1] - ctx->level_off[L];
+        if (launch_search(a, ctx->d_levels + ctx->level_off[L], bt, cnt, lds, ctx->stream) != 0)
+            return fail(ctx, DVC_ERR_HIP, "search kernel launch failed");
+    }
     HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
     return DVC_OK;
 }
@@ -474,9 +378,7 @@ int dvc_search_fetch(dvc_ctx *ctx, dvc_point_result *results) {
     if (!ctx || !results) return DVC_ERR_ARG;
     if (!ctx->plan_ready) return fail(ctx, DVC_ERR_STATE, "dvc_search_prepare first");
     HIPCHK(hipSetDevice(ctx->device));
-    int ctrl[4] = {0, 0, 0, 0};
     HIPCHK(hipMemcpyAsync(results, ctx->d_res, sizeof(dvc_point_result) * ctx->npts, hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipMemcpyAsync(ctrl, ctx->d_ctrl, sizeof(ctrl), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     float ms = 0.0f;
     if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) == hipSuccess) ctx->last_ms = ms;
@@ -488,7 +390,6 @@ int dvc_search_fetch(dvc_ctx *ctx, dvc_point_result *results) {
             results[i].obj = 0.0;
             for (int q = 0; q < 12; q++) results[i].par[q] = 0.0;
         }
-    if (ctrl[1] & 1) return fail(ctx, DVC_ERR_TIMEOUT, "a dependency wait exceeded its time limit");
     return DVC_OK;
 }
 

@@ -1,0 +1,158 @@
+// dvc_geometry.h -- host-side FP64 run geometry shared by the C ABI (dvc_api.cpp) and
+// the CPU build of the kernels used by the tests.  No HIP dependency.
+//
+// Mirrors Search::Search (Core/Search.cpp:24-82), Interpolate::Interpolate
+// (Core/Interpolate.cpp:34-58) and the sphere cloud of FloatingCloud
+// (Core/FloatingCloud.cpp:40-82).
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <vector>
+
+#include "dvc_internal.h"
+
+namespace dvc {
+
+// Point truncation of Core/Point.cpp:59-61.
+inline int trunc_i(double v) { return (int)(v / 1); }
+
+// BoundBox arithmetic (Core/BoundBox.cpp:33-85).
+struct HBox {
+    double mn[3], mx[3], cn[3];
+    HBox(const double a[3], const double b[3]) {
+        for (int q = 0; q < 3; q++) { mn[q] = a[q]; mx[q] = b[q]; cn[q] = (a[q] + b[q]) / 2.0; }
+    }
+    void grow_by(double f) {
+        for (int q = 0; q < 3; q++) { mn[q] = mn[q] + -f; mx[q] = mx[q] + f; }
+        for (int q = 0; q < 3; q++) cn[q] = (mn[q] + mx[q]) / 2.0;
+    }
+    void center_on(const double c[3]) {
+        for (int q = 0; q < 3; q++) {
+            double d = c[q] - cn[q];
+            mn[q] = mn[q] + d; mx[q] = mx[q] + d; cn[q] = cn[q] + d;
+        }
+    }
+};
+
+struct Geometry {
+    double subv_rad;
+    int subv_num, g;
+    double est_mn[3], est_mx[3], est_cn[3];
+    double act_mn[3], act_mx[3], act_cn[3];
+    int box[3];
+};
+
+inline Geometry make_geometry(const dvc_run_params &rc) {
+    Geometry G;
+    G.subv_rad = rc.subvol_size / 2.0;                                        // Search.cpp:29
+    if (rc.geom == DVC_GEOM_CUBE) G.subv_num = (int)pow(ceil(pow((double)rc.subvol_npts, 1.0 / 3.0)), 3.0);
+    else G.subv_num = rc.subvol_npts;                                         // Search.cpp:30-31
+    G.g = (int)ceil(pow((double)G.subv_num, 1.0 / 3.0));                      // Search.cpp:117
+    double z[3] = {0.0, 0.0, 0.0};
+    double m[3] = {2 * G.subv_rad * rc.aspect[0], 2 * G.subv_rad * rc.aspect[1], 2 * G.subv_rad * rc.aspect[2]};
+    HBox est(z, m);                                                           // Search.cpp:66-76
+    est.grow_by(rc.disp_max);
+    est.grow_by(2.0);
+    HBox e2(est.mn, est.mx);                                                  // Interpolate.cpp:51
+    HBox act(est.mn, est.mx);                                                 // Interpolate.cpp:52-55
+    act.grow_by(-1.0);
+    for (int q = 0; q < 3; q++) {
+        G.est_mn[q] = e2.mn[q]; G.est_mx[q] = e2.mx[q]; G.est_cn[q] = e2.cn[q];
+        G.act_mn[q] = act.mn[q]; G.act_mx[q] = act.mx[q]; G.act_cn[q] = act.cn[q];
+        G.box[q] = trunc_i(e2.mx[q]) - trunc_i(e2.mn[q]);                     // kern_4d extent, Interpolate.cpp:58
+    }
+    return G;
+}
+
+// Sphere sample offsets aspect*rel of one seed (FloatingCloud.cpp:51-78): the same
+// libstdc++ mt19937 and uniform_real_distribution<double> stream, rejection in the
+// unit-aspect ball, then the aspect scaling.
+inline void sphere_template(const dvc_run_params &rc, const Geometry &G, std::vector<double> &sph,
+                            double omn[3], double omx[3]) {
+    sph.assign((size_t)3 * G.subv_num, 0.0);
+    std::mt19937 mt(rc.sphere_seed);
+    std::uniform_real_distribution<double> dist(-G.subv_rad, G.subv_rad);
+    int count = 0;
+    while (count < G.subv_num) {
+        double rx = dist(mt), ry = dist(mt), rz = dist(mt);
+        double radius = std::sqrt(rx * rx + ry * ry + rz * rz);
+        if (radius <= G.subv_rad) {
+            sph[3 * count + 0] = rc.aspect[0] * (rx);
+            sph[3 * count + 1] = rc.aspect[1] * (ry);
+            sph[3 * count + 2] = rc.aspect[2] * (rz);
+            count += 1;
+        }
+    }
+    for (int q = 0; q < 3; q++) { omn[q] = 1e300; omx[q] = -1e300; }
+    for (int i = 0; i < G.subv_num; i++)
+        for (int q = 0; q < 3; q++) {
+            omn[q] = std::min(omn[q], sph[3 * i + q]);
+            omx[q] = std::max(omx[q], sph[3 * i + q]);
+        }
+}
+
+// Kernel arguments of one search (pointers filled by the caller).
+inline void fill_search_args(SearchArgs &a, const dvc_run_params &rc, const Geometry &G, int bpv, int W, int H,
+                             int T, int npts, int nproc, int k, const double omn[3], const double omx[3]) {
+    std::memset(&a, 0, sizeof(a));
+    a.W = W; a.H = H; a.T = T; a.bpv = bpv;
+    a.ndof = rc.ndof; a.obj = rc.obj; a.interp = rc.interp; a.geom = rc.geom;
+    a.nsamp = G.subv_num; a.g = G.g;
+    a.npts = npts; a.nproc = nproc; a.k = k;
+    a.nsamp_pad = (G.subv_num + 3) & ~3;
+    for (int q = 0; q < 3; q++) {
+        a.box[q] = G.box[q];
+        a.est_mn[q] = G.est_mn[q]; a.est_mx[q] = G.est_mx[q]; a.est_cn[q] = G.est_cn[q];
+        a.act_mn[q] = G.act_mn[q]; a.act_mx[q] = G.act_mx[q]; a.act_cn[q] = G.act_cn[q];
+        a.aspect[q] = rc.aspect[q]; a.rigid[q] = rc.rigid_trans[q];
+        a.off_min[q] = omn[q]; a.off_max[q] = omx[q];
+    }
+    a.subv_rad = G.subv_rad; a.basin = rc.basin_radius; a.disp_max = (double)rc.disp_max;
+}
+
+// Position of every point in order[] (npts when it is never searched, dvc.cpp:229-232).
+inline bool order_positions(const int *order, int npts, int N, std::vector<int> &pos) {
+    pos.assign(npts, npts);
+    for (int i = 0; i < npts; i++) {
+        int o = order[i];
+        if (o < 0 || o >= npts || pos[o] != npts) return false;
+        pos[o] = i;
+    }
+    for (int i = 0; i < npts; i++)
+        if (pos[i] >= N) pos[i] = npts;
+    return true;
+}
+
+// Dependency levels of the serial point loop (dvc.cpp:233-303 with the start guess of
+// Search.cpp:277-377): the point at position i of order[] reads the results of its
+// neighbours at earlier positions, so its level is one more than the highest level
+// among them.  Points of one level are independent; running the levels in order
+// reproduces the serial loop exactly.  level_pts lists the searched points level by
+// level (ascending position inside a level), level_off[L]..level_off[L+1] indexes it.
+inline int dependency_levels(const int *order, const std::vector<int> &pos, const int *neigh, int npts, int k, int N,
+                             std::vector<int> &level_pts, std::vector<int> &level_off) {
+    std::vector<int> lv(N, 0);
+    int depth = 0;
+    for (int i = 0; i < N; i++) {
+        int n = order[i], l = 0;
+        for (int q = 0; q < k; q++) {
+            int m = neigh[(size_t)n * k + q];
+            int pm = pos[m];
+            if (pm < i && lv[pm] + 1 > l) l = lv[pm] + 1;
+        }
+        lv[i] = l;
+        if (l + 1 > depth) depth = l + 1;
+    }
+    level_off.assign(depth + 1, 0);
+    for (int i = 0; i < N; i++) level_off[lv[i] + 1] += 1;
+    for (int L = 0; L < depth; L++) level_off[L + 1] += level_off[L];
+    std::vector<int> fill(level_off.begin(), level_off.end() - 1);
+    level_pts.assign(N, 0);
+    for (int i = 0; i < N; i++) level_pts[fill[lv[i]]++] = order[i];
+    return depth;
+}
+
+}  // namespace dvc

@@ -28,10 +28,6 @@ struct SearchArgs {
     const int *pos;                  // position of each point in order[] (npts if unsearched)
     const int *neigh;                // [npts*k]
     dvc_point_result *res;           // [npts]
-    int *done;                       // [npts] completion flags
-    int *counter;                    // work counter
-    int *error;                      // bit0: dependency wait timed out
-    long long wait_ticks;            // wall_clock64 ticks a dependency wait may take
 };
 
 struct StrainArgs {
@@ -45,7 +41,8 @@ struct StrainArgs {
 };
 
 // launchers implemented in dvc_kernels.hip
-int launch_search(const SearchArgs &a, int block_threads, int grid_blocks, size_t lds_bytes, void *stream);
+int launch_search(const SearchArgs &a, const int *level_pts, int block_threads, int grid_blocks, size_t lds_bytes,
+                  void *stream);
 int search_max_blocks(int bpv, int block_threads, size_t lds_bytes, int *blocks_per_cu);
 size_t search_lds_bytes(int nsamp_pad, int block_threads);
 int launch_interp(const uint8_t *vol, int bpv, int W, int H, int T, const int b0[3], const int box[3],

@@ -30,13 +30,25 @@
 //     all point geometry (cloud, warp, bounding boxes, range checks) are FP64 with
 //     the reference's operation order and no fused multiply-add, so statuses follow
 //     the reference's comparisons.
+#ifdef DVC_EMU
+#include "dvc_emu.h"
+#else
 #include <hip/hip_runtime.h>
+#define DVC_DYN_LDS(type, name) extern __shared__ __align__(16) type name[]
+#define DVC_SHARED(type, name) \
+    using name##_t = type;     \
+    __shared__ name##_t name
+#endif
 #include <math.h>
 #include "dvc_internal.h"
 
 namespace dvc {
 
+#ifdef DVC_EMU
+typedef emu::f4 floatx4;
+#else
 typedef float floatx4 __attribute__((ext_vector_type(4)));
+#endif
 
 // ------------------------------------------------------------------ helpers
 
@@ -298,6 +310,9 @@ struct PtShared {
     int gmin[3];
     double sums[8];
     double JTJ[144], rhs[12];
+    // column-pivoted QR workspace (kept in LDS: no private-memory arrays)
+    double qr[144], qhc[12], qnu[12], qnd[12], qc[12], qx[12];
+    int qtr[12], qperm[12];
 };
 
 // Stable cloud sample j (FloatingCloud.cpp:84-128 cube grid, z outer / x inner;
@@ -402,7 +417,7 @@ __device__ void rotation(double phi, double the, double psi, double A[9], double
 }
 
 // Warp state for parameter vector evalpar (thread 0).  SearchParams.cpp:53-107.
-__device__ void set_warp(PtShared &S, int ndof) {
+__device__ __forceinline__ void set_warp(PtShared &S, int ndof) {
 #pragma clang fp contract(off)
     for (int q = 0; q < 3; q++) {
         S.u[q] = S.evalpar[q];
@@ -434,12 +449,13 @@ __device__ void set_warp(PtShared &S, int ndof) {
 }
 
 // Column-pivoted Householder QR solve (Eigen 3.4 ColPivHouseholderQR::computeInPlace
-// and _solve_impl, as used at Search.cpp:234).  A is n x n row-major in LDS; thread 0.
-__device__ void colpiv_solve(const double *Arow, const double *b, int n, double *x) {
-    double Q[144], hc[12], nu[12], nd[12], c[12];
-    int tr[12], perm[12];
+// and _solve_impl, as used at Search.cpp:234) of S.JTJ (n x n, row-major) x = S.rhs,
+// result in S.qx.  Thread 0; the workspace lives in LDS (PtShared::q*).
+__device__ void colpiv_solve(PtShared &S, int n) {
+    double *Q = S.qr, *hc = S.qhc, *nu = S.qnu, *nd = S.qnd, *c = S.qc, *x = S.qx;
+    int *tr = S.qtr, *perm = S.qperm;
     for (int i = 0; i < n; i++)
-        for (int j = 0; j < n; j++) Q[j * n + i] = Arow[i * n + j];
+        for (int j = 0; j < n; j++) Q[j * n + i] = S.JTJ[i * n + j];
     for (int k = 0; k < n; k++) {
         double s = 0.0;
         for (int i = 0; i < n; i++) s += Q[k * n + i] * Q[k * n + i];
@@ -464,7 +480,7 @@ __device__ void colpiv_solve(const double *Arow, const double *b, int n, double 
             double t = nu[k]; nu[k] = nu[big]; nu[big] = t;
             t = nd[k]; nd[k] = nd[big]; nd[big] = t;
         }
-        // makeHouseholder of column k rows k..n-1
+        // makeHouseholder of column k, rows k..n-1
         double *col = &Q[k * n + k];
         int m = n - k;
         double tail = 0.0;
@@ -501,9 +517,9 @@ __device__ void colpiv_solve(const double *Arow, const double *b, int n, double 
                 double r = nu[j] / nd[j];
                 double t2 = t * (r * r);
                 if (t2 <= dthr) {
-                    double s = 0.0;
-                    for (int i = k + 1; i < n; i++) s += Q[j * n + i] * Q[j * n + i];
-                    nd[j] = sqrt(s);
+                    double s2 = 0.0;
+                    for (int i = k + 1; i < n; i++) s2 += Q[j * n + i] * Q[j * n + i];
+                    nd[j] = sqrt(s2);
                     nu[j] = nd[j];
                 } else {
                     nu[j] *= sqrt(t);
@@ -515,7 +531,7 @@ __device__ void colpiv_solve(const double *Arow, const double *b, int n, double 
     for (int k = 0; k < n; k++) { int t = perm[k]; perm[k] = perm[tr[k]]; perm[tr[k]] = t; }
     for (int i = 0; i < n; i++) x[i] = 0.0;
     if (nonzero == 0) return;
-    for (int i = 0; i < n; i++) c[i] = b[i];
+    for (int i = 0; i < n; i++) c[i] = S.rhs[i];
     for (int k = 0; k < nonzero; k++) {
         int m = n - k;
         double tau = hc[k];
@@ -528,9 +544,9 @@ __device__ void colpiv_solve(const double *Arow, const double *b, int n, double 
         for (int i = 1; i < m; i++) c[k + i] -= tau * Q[k * n + k + i] * t;
     }
     for (int i = nonzero - 1; i >= 0; i--) {
-        double s = c[i];
-        for (int j = i + 1; j < nonzero; j++) s -= Q[j * n + i] * c[j];
-        c[i] = s / Q[i * n + i];
+        double sacc = c[i];
+        for (int j = i + 1; j < nonzero; j++) sacc -= Q[j * n + i] * c[j];
+        c[i] = sacc / Q[i * n + i];
     }
     for (int i = 0; i < nonzero; i++) x[perm[i]] = c[i];
 }
@@ -835,25 +851,16 @@ __device__ void point_search(const SearchArgs &a, int p, int n, PtShared &S, flo
         block_sum<B, 3>(q3, red, &S.sums[1]);
         if (tid == 0) { S.rmean = rmean; S.R2 = S.sums[1]; S.Src = S.sums[2]; S.Rn2 = S.sums[3]; }
     }
-    // ---- starting_param: neighbours searched earlier (wave 0 waits on their flags)
+    // ---- starting_param: neighbours searched earlier.  Launches run the dependency
+    // levels in order, so every such neighbour's result is already in memory.
     if (tid < 64) {
         for (int q = lane; q < a.k; q += 64) {
             int m = a.neigh[(long long)n * a.k + q];
             int st = DVC_NOT_SEARCH;
             double px = 0.0, py = 0.0, pz = 0.0;
             if (m >= 0 && m < a.npts && a.pos[m] < p) {
-                long long t0 = wall_clock64();
-                bool ok = true;
-                while (__hip_atomic_load(&a.done[m], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
-                    __builtin_amdgcn_s_sleep(2);
-                    if (wall_clock64() - t0 > a.wait_ticks) { ok = false; break; }
-                }
-                if (ok) {
-                    st = a.res[m].status;
-                    px = a.res[m].par[0]; py = a.res[m].par[1]; pz = a.res[m].par[2];
-                } else {
-                    atomicOr(a.error, 1);
-                }
+                st = a.res[m].status;
+                px = a.res[m].par[0]; py = a.res[m].par[1]; pz = a.res[m].par[2];
             }
             S.nb_status[q] = st;
             S.nb_par[q][0] = px; S.nb_par[q][1] = py; S.nb_par[q][2] = pz;
@@ -928,9 +935,8 @@ __device__ void point_search(const SearchArgs &a, int p, int n, PtShared &S, flo
             }
             if (!S.stop) {
                 assemble(a, S, V, nd);
-                double x[12];
-                colpiv_solve(S.JTJ, S.rhs, nd, x);
-                for (int q = 0; q < nd; q++) { S.upd[q] = x[q]; S.par[q] += x[q]; }
+                colpiv_solve(S, nd);
+                for (int q = 0; q < nd; q++) { S.upd[q] = S.qx[q]; S.par[q] += S.qx[q]; }
             }
         }
         __syncthreads();
@@ -949,52 +955,47 @@ __device__ void point_search(const SearchArgs &a, int p, int n, PtShared &S, flo
     }
 }
 
+// One dependency level of the point loop: workgroup b searches point level_pts[b].
+// All neighbours a point depends on belong to earlier levels (earlier launches).
 template <int BPV, int B>
-__global__ void __launch_bounds__(B) search_kernel(SearchArgs a) {
-    extern __shared__ __align__(16) float dyn[];
+__global__ void __launch_bounds__(B) search_kernel(SearchArgs a, const int *level_pts) {
+    DVC_DYN_LDS(float, dyn);
     float *rcb = dyn;
     float *tcb = dyn + a.nsamp_pad;
     float *stage = dyn + 2 * a.nsamp_pad;
-    __shared__ PtShared S;
-    __shared__ double red[(B / 64) * 8];
-    __shared__ double V[256];
-    __shared__ int s_p;
+    DVC_SHARED(PtShared, S);
+    DVC_SHARED(double[(B / 64) * 8], red);
+    DVC_SHARED(double[256], V);
     const int tid = threadIdx.x;
-    for (;;) {
-        if (tid == 0) s_p = atomicAdd(a.counter, 1);
-        __syncthreads();
-        const int p = s_p;
-        if (p >= a.nproc) break;
-        const int n = a.order[p];
-        point_search<BPV, B>(a, p, n, S, rcb, tcb, stage, red, V);
-        __syncthreads();
-        if (tid == 0) {
-            dvc_point_result *r = &a.res[n];
-            r->status = S.status;
-            r->iters = S.iters;
-            if (S.status == DVC_POINT_GOOD) {
-                r->obj = S.obj;
-                for (int q = 0; q < 12; q++) r->par[q] = (q < a.ndof) ? S.par[q] : 0.0;
-            } else {
-                // dvc.cpp:268-283 records only the status of a failed point
-                r->obj = 0.0;
-                for (int q = 0; q < 12; q++) r->par[q] = 0.0;
-            }
-            __threadfence();
-            __hip_atomic_store(&a.done[n], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    const int n = level_pts[blockIdx.x];
+    const int p = a.pos[n];
+    point_search<BPV, B>(a, p, n, S, rcb, tcb, stage, red, V);
+    __syncthreads();
+    if (tid == 0) {
+        dvc_point_result *r = &a.res[n];
+        r->status = S.status;
+        r->iters = S.iters;
+        if (S.status == DVC_POINT_GOOD) {
+            r->obj = S.obj;
+            for (int q = 0; q < 12; q++) r->par[q] = (q < a.ndof) ? S.par[q] : 0.0;
+        } else {
+            // dvc.cpp:268-283 records only the status of a failed point
+            r->obj = 0.0;
+            for (int q = 0; q < 12; q++) r->par[q] = 0.0;
         }
     }
 }
 
+#ifndef DVC_EMU
 size_t search_lds_bytes(int nsamp_pad, int block_threads) {
     return (size_t)nsamp_pad * 2 * sizeof(float) + (size_t)(block_threads / 64) * 64 * 17 * sizeof(float);
 }
 
 template <int BPV, int B>
-static int launch_search_t(const SearchArgs &a, int grid, size_t lds, hipStream_t st) {
+static int launch_search_t(const SearchArgs &a, const int *level_pts, int grid, size_t lds, hipStream_t st) {
     auto k = search_kernel<BPV, B>;
     if (hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) return -1;
-    hipLaunchKernelGGL(k, dim3(grid), dim3(B), lds, st, a);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(B), lds, st, a, level_pts);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -1008,19 +1009,19 @@ static int occ_t(size_t lds, int *bpc) {
 #define DVC_DISPATCH(BPV_, B_, CALL)                                  \
     if (bpv == BPV_ && block_threads == B_) return CALL<BPV_, B_>
 
-int launch_search(const SearchArgs &a, int block_threads, int grid, size_t lds, void *stream) {
+int launch_search(const SearchArgs &a, const int *level_pts, int block_threads, int grid, size_t lds, void *stream) {
     hipStream_t st = (hipStream_t)stream;
     int bpv = a.bpv;
-    DVC_DISPATCH(1, 64, launch_search_t)(a, grid, lds, st);
-    DVC_DISPATCH(1, 128, launch_search_t)(a, grid, lds, st);
-    DVC_DISPATCH(1, 256, launch_search_t)(a, grid, lds, st);
-    DVC_DISPATCH(1, 512, launch_search_t)(a, grid, lds, st);
-    DVC_DISPATCH(1, 1024, launch_search_t)(a, grid, lds, st);
-    DVC_DISPATCH(2, 64, launch_search_t)(a, grid, lds, st);
-    DVC_DISPATCH(2, 128, launch_search_t)(a, grid, lds, st);
-    DVC_DISPATCH(2, 256, launch_search_t)(a, grid, lds, st);
-    DVC_DISPATCH(2, 512, launch_search_t)(a, grid, lds, st);
-    DVC_DISPATCH(2, 1024, launch_search_t)(a, grid, lds, st);
+    DVC_DISPATCH(1, 64, launch_search_t)(a, level_pts, grid, lds, st);
+    DVC_DISPATCH(1, 128, launch_search_t)(a, level_pts, grid, lds, st);
+    DVC_DISPATCH(1, 256, launch_search_t)(a, level_pts, grid, lds, st);
+    DVC_DISPATCH(1, 512, launch_search_t)(a, level_pts, grid, lds, st);
+    DVC_DISPATCH(1, 1024, launch_search_t)(a, level_pts, grid, lds, st);
+    DVC_DISPATCH(2, 64, launch_search_t)(a, level_pts, grid, lds, st);
+    DVC_DISPATCH(2, 128, launch_search_t)(a, level_pts, grid, lds, st);
+    DVC_DISPATCH(2, 256, launch_search_t)(a, level_pts, grid, lds, st);
+    DVC_DISPATCH(2, 512, launch_search_t)(a, level_pts, grid, lds, st);
+    DVC_DISPATCH(2, 1024, launch_search_t)(a, level_pts, grid, lds, st);
     return -1;
 }
 
@@ -1038,6 +1039,8 @@ int search_max_blocks(int bpv, int block_threads, size_t lds, int *bpc) {
     return -1;
 }
 
+#endif  // DVC_EMU
+
 // ------------------------------------------------------ interpolation kernel
 
 template <int BPV>
@@ -1052,6 +1055,7 @@ __global__ void interp_kernel(VolView v, BoxView bx, const double *xyz, int n, i
     if (grad) { grad[3 * i] = gx; grad[3 * i + 1] = gy; grad[3 * i + 2] = gz; }
 }
 
+#ifndef DVC_EMU
 int launch_interp(const uint8_t *vol, int bpv, int W, int H, int T, const int b0[3], const int box[3],
                   const double *xyz, int n, int method, double *out, double *grad, void *stream) {
     (void)T;
@@ -1064,14 +1068,16 @@ int launch_interp(const uint8_t *vol, int bpv, int W, int H, int T, const int b0
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+#endif  // DVC_EMU
+
 // ------------------------------------------------------------ objective kernel
 
 // obj_* of ObjectiveFunctions.cpp:24-263 for one pair of sample vectors (FP64), one
 // workgroup of 1024 threads.
 __global__ void __launch_bounds__(1024) objective_kernel(int kind, const double *r, const double *t, int n,
                                                          double *obj, double *res) {
-    __shared__ double red[16 * 8];
-    __shared__ double sums[8];
+    DVC_SHARED(double[16 * 8], red);
+    DVC_SHARED(double[8], sums);
     const int tid = threadIdx.x;
     double s[2] = {0.0, 0.0};
     for (int i = tid; i < n; i += 1024) { s[0] += r[i]; s[1] += t[i]; }
@@ -1103,10 +1109,13 @@ __global__ void __launch_bounds__(1024) objective_kernel(int kind, const double 
     }
 }
 
+#ifndef DVC_EMU
 int launch_objective(int kind, const double *ref, const double *tar, int n, double *obj, double *res, void *stream) {
     hipLaunchKernelGGL(objective_kernel, dim3(1), dim3(1024), 0, (hipStream_t)stream, kind, ref, tar, n, obj, res);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
+
+#endif  // DVC_EMU
 
 // ------------------------------------------------------------- small kernels
 
@@ -1116,6 +1125,7 @@ __global__ void zero_int_kernel(int *p, long long n) {
     for (; i < n; i += st) p[i] = 0;
 }
 
+#ifndef DVC_EMU
 int launch_zero_int(int *p, long long n, void *stream) {
     if (n <= 0) return 0;
     long long blocks = (n + 255) / 256;
@@ -1124,12 +1134,15 @@ int launch_zero_int(int *p, long long n, void *stream) {
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+#endif  // DVC_EMU
+
 __global__ void swap16_kernel(uint16_t *p, long long n) {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     long long st = (long long)gridDim.x * blockDim.x;
     for (; i < n; i += st) { uint16_t v = p[i]; p[i] = (uint16_t)((v >> 8) | (v << 8)); }
 }
 
+#ifndef DVC_EMU
 int launch_swap16(uint16_t *p, long long n, void *stream) {
     if (n <= 0) return 0;
     long long blocks = (n + 255) / 256;
@@ -1137,6 +1150,8 @@ int launch_swap16(uint16_t *p, long long n, void *stream) {
     hipLaunchKernelGGL(swap16_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, p, n);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
+
+#endif  // DVC_EMU
 
 // ---------------------------------------------------------------- strain fit
 
@@ -1268,10 +1283,10 @@ constexpr int kStrainWaves = 4;
 __global__ void __launch_bounds__(64 * kStrainWaves) strain_kernel(StrainArgs a) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int i = blockIdx.x * kStrainWaves + w;
-    __shared__ double Xs[kStrainWaves][75][10];
-    __shared__ double Ds[kStrainWaves][75][3];
-    __shared__ double Ms[kStrainWaves][100 + 30];
-    __shared__ int sel[kStrainWaves][76];
+    DVC_SHARED(double[kStrainWaves][75][10], Xs);
+    DVC_SHARED(double[kStrainWaves][75][3], Ds);
+    DVC_SHARED(double[kStrainWaves][100 + 30], Ms);
+    DVC_SHARED(int[kStrainWaves][76], sel);
     if (i >= a.n) return;      // whole wave leaves together; no block barrier below
     const int *nb = a.neigh + (long long)i * a.k;
     double px = a.pts[3 * i], py = a.pts[3 * i + 1], pz = a.pts[3 * i + 2];
@@ -1389,11 +1404,14 @@ __global__ void __launch_bounds__(64 * kStrainWaves) strain_kernel(StrainArgs a)
     }
 }
 
+#ifndef DVC_EMU
 int launch_strain(const StrainArgs &a, void *stream) {
     if (a.n <= 0) return 0;
     int blocks = (a.n + kStrainWaves - 1) / kStrainWaves;
     hipLaunchKernelGGL(strain_kernel, dim3(blocks), dim3(64 * kStrainWaves), 0, (hipStream_t)stream, a);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
+
+#endif  // DVC_EMU
 
 }  // namespace dvc

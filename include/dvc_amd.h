@@ -29,7 +29,6 @@ extern "C" {
 #define DVC_ERR_ARG      -1   /* invalid argument or size */
 #define DVC_ERR_HIP      -2   /* HIP runtime failure */
 #define DVC_ERR_STATE    -3   /* call out of order (e.g. no volume uploaded) */
-#define DVC_ERR_TIMEOUT  -4   /* a device-side dependency wait exceeded its spin limit */
 #define DVC_ERR_NODEV    -5   /* no usable gfx950 device */
 #define DVC_ERR_RANGE    -6   /* dvc_interpolate: points outside the active box (Intrp_Fail) */
 
@@ -130,14 +129,16 @@ int dvc_objective(dvc_ctx *ctx, int kind, const double *ref, const double *tar, 
  * = DataCloud::neigh (k = min(75, npts), self first); the first nproc entries of order
  * are searched (nproc <= 1 or >= npts: all, dvc.cpp:229-232).  Every point's start
  * guess averages the converged translations of its neighbours searched before it
- * (Search.cpp:277-377), so results match the serial loop.  results[npts]. */
+ * (Search.cpp:277-377); the points run level by level of that dependency, so results
+ * match the serial loop.  results[npts]. */
 int dvc_search_points(dvc_ctx *ctx, const dvc_run_params *rc, const double *points,
                       int32_t npts, const int32_t *order, int32_t nproc,
                       const int32_t *neigh, int32_t k, dvc_point_result *results);
 
 /* The same loop split for callers that keep inputs resident: prepare uploads the
- * point cloud and its order/neighbour tables; execute launches the search on the
- * context stream without synchronising; fetch waits and copies results back. */
+ * point cloud and its order/neighbour tables and derives the dependency levels;
+ * execute enqueues one launch per level on the context stream without synchronising;
+ * fetch waits and copies results back. */
 int dvc_search_prepare(dvc_ctx *ctx, const dvc_run_params *rc, const double *points,
                        int32_t npts, const int32_t *order, int32_t nproc,
                        const int32_t *neigh, int32_t k);
@@ -146,9 +147,6 @@ int dvc_search_fetch(dvc_ctx *ctx, dvc_point_result *results);
 
 /* Launch tuning: threads per point (64, 128, 256, 512 or 1024; 0 = automatic). */
 int dvc_set_block_threads(dvc_ctx *ctx, int threads);
-/* Upper bound, in seconds, on one device-side wait for a neighbour's result
- * (default 30).  A wait that exceeds it ends with DVC_ERR_TIMEOUT from the fetch. */
-int dvc_set_wait_limit(dvc_ctx *ctx, double seconds);
 
 /* ----------------------------------------------------------------- strain */
 

@@ -24,6 +24,5 @@ def oracle():
 def ctx():
     import digitalvolumecorrelation_amd as D
     c = D.Context(0)
-    c.set_wait_limit(20.0)
     yield c
     c.close()

@@ -1,0 +1,78 @@
+// emu_driver.cpp -- runs the search and strain kernels of dvc_kernels.hip on the CPU
+// (see dvc_emu.h).  Test infrastructure: tests/test_emu_kernels.py compares these
+// runs with the oracle, so kernel logic is checked without a GPU.
+#define DVC_EMU 1
+#include "dvc_kernels.hip"
+#include "dvc_geometry.h"
+
+using namespace dvc;
+
+// One launch per dependency level; the workgroups of a level run concurrently.
+template <int BPV, int B>
+static void run_search(const SearchArgs &a, size_t lds, const std::vector<int> &lp, const std::vector<int> &lo) {
+    for (size_t L = 0; L + 1 < lo.size(); L++) {
+        int cnt = lo[L + 1] - lo[L];
+        const int *pts = lp.data() + lo[L];
+        emu::grid_dim.x = cnt;
+        for (int b0 = 0; b0 < cnt; b0 += 8) {
+            int nb = cnt - b0 < 8 ? cnt - b0 : 8;
+            emu::run_grid(b0, nb, B, lds, [&] { search_kernel<BPV, B>(a, pts); });
+        }
+    }
+}
+
+extern "C" int emu_search(const dvc_run_params *rc, const void *ref, const void *tar, int bpv, int W, int H, int T,
+                          const double *pts, int npts, const int *order, int nproc, const int *neigh, int k,
+                          int block_threads, dvc_point_result *out) {
+    Geometry G = make_geometry(*rc);
+    int N = npts;
+    if (nproc > 1 && nproc < npts) N = nproc;
+    std::vector<int> pos;
+    if (!order_positions(order, npts, N, pos)) return -1;
+    std::vector<double> sph;
+    double omn[3] = {0, 0, 0}, omx[3] = {0, 0, 0};
+    if (rc->geom == DVC_GEOM_SPHERE) sphere_template(*rc, G, sph, omn, omx);
+    size_t vbytes = (size_t)W * H * T * bpv;
+    std::vector<uint32_t> v0(vbytes / 4 + 32), v1(vbytes / 4 + 32);
+    std::memcpy(v0.data(), ref, vbytes);
+    std::memcpy(v1.data(), tar, vbytes);
+    std::vector<dvc_point_result> res(npts);
+    std::vector<int> lp, lo;
+    dependency_levels(order, pos, neigh, npts, k, N, lp, lo);
+    std::memset(res.data(), 0, sizeof(dvc_point_result) * npts);
+    SearchArgs a;
+    fill_search_args(a, *rc, G, bpv, W, H, T, npts, N, k, omn, omx);
+    a.vol[0] = (const uint8_t *)v0.data();
+    a.vol[1] = (const uint8_t *)v1.data();
+    a.sph = sph.data(); a.pts = pts; a.order = order; a.pos = pos.data(); a.neigh = neigh;
+    a.res = res.data();
+    size_t lds = (size_t)a.nsamp_pad * 8 + (size_t)(block_threads / 64) * 64 * 17 * 4;
+    if (bpv == 1 && block_threads == 64) run_search<1, 64>(a, lds, lp, lo);
+    else if (bpv == 1 && block_threads == 256) run_search<1, 256>(a, lds, lp, lo);
+    else if (bpv == 2 && block_threads == 64) run_search<2, 64>(a, lds, lp, lo);
+    else if (bpv == 2 && block_threads == 256) run_search<2, 256>(a, lds, lp, lo);
+    else return -1;
+    for (int i = 0; i < npts; i++) {
+        out[i] = res[i];
+        if (pos[i] >= N) {
+            out[i].status = DVC_NOT_SEARCH; out[i].iters = 0; out[i].obj = 0.0;
+            for (int q = 0; q < 12; q++) out[i].par[q] = 0.0;
+        }
+    }
+    return 0;
+}
+
+extern "C" int emu_strain(const double *pts, const double *disp, const int *status, const double *objmin,
+                          const int *neigh, int k, int n, int ndp, double objt, int refill, int yz, int zx, int xy,
+                          double *out_disp, int *out_npts, double *out_swr, double *out_E, double *out_L,
+                          double *out_DG) {
+    StrainArgs a;
+    a.pts = pts; a.disp = disp; a.objmin = objmin; a.status = status; a.neigh = neigh;
+    a.k = k; a.n = n; a.ndp = ndp; a.refill = refill; a.yz = yz; a.zx = zx; a.xy = xy; a.objt = objt;
+    a.out_disp = out_disp; a.out_swr = out_swr; a.out_E = out_E; a.out_L = out_L; a.out_DG = out_DG;
+    a.out_npts = out_npts;
+    int blocks = (n + kStrainWaves - 1) / kStrainWaves;
+    emu::grid_dim.x = blocks;
+    for (int b = 0; b < blocks; b++) emu::run_grid(b, 1, 64 * kStrainWaves, 0, [&] { strain_kernel(a); });
+    return 0;
+}

@@ -1,0 +1,97 @@
+"""GPU search (dvc_search_points) against the oracle's serial dvc loop
+(Core/dvc.cpp:229-320 around Search::process_point, Core/Search.cpp:95-192).
+
+Bar (tolerances written per assertion): statuses and iteration counts equal,
+displacements within 1e-3 voxel, objectives within 1e-5 relative, the oracle run in
+its reference mode (forward-difference Jacobian, h = 1e-10).  The error of both
+against the imposed deformation is asserted too.
+"""
+import numpy as np
+import pytest
+
+import digitalvolumecorrelation_amd as D
+from digitalvolumecorrelation_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _compare(ctx, oracle, case, params, start=None, nproc=0, threads=None, truth_tol=None, jacobian=0):
+    pts = case.points
+    order, neigh = D.sort_order_neighbors(pts, start=start if start is not None else case.start)
+    run = oracle.make_run(shape=case.ref.shape, bytes_per=case.ref.dtype.itemsize, jacobian=jacobian, **params)
+    ores, _, _ = oracle.dvc_run(run, case.ref, case.tar, pts, order, neigh, nproc=nproc, nthreads=8)
+    ctx.upload(0, case.ref)
+    ctx.upload(1, case.tar)
+    if threads:
+        ctx.set_block_threads(threads)
+    try:
+        gres = ctx.search(D.make_params(**params), pts, order, neigh, nproc=nproc)
+    finally:
+        ctx.set_block_threads(0)
+    print(f"{case.name}: status {np.unique(gres['status'], return_counts=True)}")
+    assert np.array_equal(gres["status"], ores["status"])
+    good = ores["status"] == 0
+    du = np.abs(gres["par"][:, :3] - ores["par"][:, :3]).max()
+    rel = (np.abs(gres["obj"][good] - ores["obj"][good]) / np.abs(ores["obj"][good])).max() if good.any() else 0.0
+    print(f"  max |du| {du:.3e}  max obj rel {rel:.3e}  iters gpu {gres['iters'].sum()} oracle {ores['iters'].sum()}")
+    assert du <= 1e-3
+    assert rel <= 1e-5
+    assert np.array_equal(gres["iters"], ores["iters"])
+    if truth_tol is not None and good.any():
+        assert np.abs(gres["par"][good, :3] - case.truth[good]).max() <= truth_tol
+        assert np.abs(ores["par"][good, :3] - case.truth[good]).max() <= truth_tol
+    return gres, ores
+
+
+@pytest.fixture(scope="module")
+def case1():
+    return synth.config1(size=80, n=4, spacing=8)
+
+
+def test_translation_cube_u8(ctx, oracle, case1):
+    _compare(ctx, oracle, case1, case1.params, truth_tol=0.05)
+
+
+@pytest.mark.parametrize("threads", [64, 128, 512, 1024])
+def test_block_sizes_agree(ctx, oracle, case1, threads):
+    _compare(ctx, oracle, case1, case1.params, threads=threads)
+
+
+@pytest.fixture(scope="module")
+def case12():
+    return synth.config2(size=96, n=3, spacing=12, npts=2000, diameter=25)
+
+
+def test_affine_sphere_u16_12dof(ctx, oracle, case12):
+    p = dict(case12.params, disp_max=6)
+    _compare(ctx, oracle, case12, p, truth_tol=0.1)
+
+
+@pytest.mark.parametrize("obj", ["ssd", "zssd", "nssd", "znssd", "sad"])
+def test_objectives_6dof(ctx, oracle, case12, obj):
+    p = dict(case12.params, disp_max=6, ndof=6, obj=obj, subvol_npts=1500)
+    _compare(ctx, oracle, case12, p)
+
+
+@pytest.mark.parametrize("interp", ["trilinear", "nearest"])
+def test_other_interpolators(ctx, oracle, case1, interp):
+    p = dict(case1.params, interp=interp)
+    _compare(ctx, oracle, case1, p)
+
+
+def test_range_fail_partial_and_grid(ctx, oracle, case1):
+    # start far off: every point fails its range check
+    g, o = _compare(ctx, oracle, case1, dict(case1.params, rigid_trans=(0.0, 0.0, 8.0), disp_max=2))
+    assert np.all(g["status"] == D.RANGE_FAIL)
+    # num_points_to_process
+    g, o = _compare(ctx, oracle, case1, case1.params, nproc=7)
+    assert (g["status"] == D.NOT_SEARCH).sum() == len(case1.points) - 7
+    # translation grid from a zero start
+    _compare(ctx, oracle, case1, dict(case1.params, rigid_trans=(0.0, 0.0, 0.0), basin_radius=1.0, disp_max=3),
+             nproc=3, truth_tol=0.05)
+
+
+def test_points_near_volume_edge(ctx, oracle):
+    case = synth.config1(size=64, n=4, spacing=14)   # outer points' boxes leave the volume
+    g, o = _compare(ctx, oracle, case, case.params)
+    assert (g["status"] == D.RANGE_FAIL).any() and (g["status"] == D.POINT_GOOD).any()
