@@ -114,7 +114,7 @@ __device__ __forceinline__ float voxel(const VolView &v, int x, int y, int z) {
 // Four consecutive voxels x0..x0+3 of row (y, z).  The volume buffer is dword aligned
 // and padded by 16 bytes, so the aligned window read stays inside the allocation.
 template <int BPV>
-__device__ __forceinline__ void load_row(const VolView &v, int x0, int y, int z, float r[4]) {
+__device__ __forceinline__ void load_row(const VolView &v, int x0, int y, int z, float bias, float r[4]) {
     long long i = (long long)z * v.WH + (long long)y * v.W + x0;
     if (BPV == 1) {
         uintptr_t ad = (uintptr_t)(v.p + i);
@@ -122,10 +122,10 @@ __device__ __forceinline__ void load_row(const VolView &v, int x0, int y, int z,
         uint32_t s = (uint32_t)(ad & 3);
         uint32_t lo = a4[0], hi = a4[1];
         uint32_t w = __builtin_amdgcn_alignbyte(hi, lo, s);
-        r[0] = (float)(w & 0xffu);
-        r[1] = (float)((w >> 8) & 0xffu);
-        r[2] = (float)((w >> 16) & 0xffu);
-        r[3] = (float)(w >> 24);
+        r[0] = (float)(w & 0xffu) - bias;
+        r[1] = (float)((w >> 8) & 0xffu) - bias;
+        r[2] = (float)((w >> 16) & 0xffu) - bias;
+        r[3] = (float)(w >> 24) - bias;
     } else {
         uintptr_t ad = (uintptr_t)(v.p + 2 * i);
         const uint32_t *a4 = (const uint32_t *)(ad & ~(uintptr_t)3);
@@ -133,10 +133,10 @@ __device__ __forceinline__ void load_row(const VolView &v, int x0, int y, int z,
         uint32_t d0 = a4[0], d1 = a4[1], d2 = a4[2];
         uint32_t w0 = __builtin_amdgcn_alignbyte(d1, d0, s);
         uint32_t w1 = __builtin_amdgcn_alignbyte(d2, d1, s);
-        r[0] = (float)(w0 & 0xffffu);
-        r[1] = (float)(w0 >> 16);
-        r[2] = (float)(w1 & 0xffffu);
-        r[3] = (float)(w1 >> 16);
+        r[0] = (float)(w0 & 0xffffu) - bias;
+        r[1] = (float)(w0 >> 16) - bias;
+        r[2] = (float)(w1 & 0xffffu) - bias;
+        r[3] = (float)(w1 >> 16) - bias;
     }
 }
 
@@ -168,7 +168,7 @@ __device__ __forceinline__ void hermite(float t, float h[2][2], float dh[2][2]) 
 // face copies of Interpolate.cpp:293-335 leave them, combined with the Hermite basis.
 template <int BPV>
 __device__ float tricubic_faces(const VolView &v, const BoxView &bx, int ix, int iy, int iz,
-                                float tx, float ty, float tz, bool want_grad,
+                                float tx, float ty, float tz, float bias, bool want_grad,
                                 float &gx, float &gy, float &gz) {
     float hx[2][2], hy[2][2], hz[2][2], dhx[2][2], dhy[2][2], dhz[2][2];
     hermite(tx, hx, dhx);
@@ -182,9 +182,9 @@ __device__ float tricubic_faces(const VolView &v, const BoxView &bx, int ix, int
         int qx = clampi(vx, bx.b[0] + 1, bx.b[0] + bx.n[0] - 2);
         int qy = clampi(vy, bx.b[1] + 1, bx.b[1] + bx.n[1] - 2);
         int qz = clampi(vz, bx.b[2] + 1, bx.b[2] + bx.n[2] - 2);
-#define F(dx, dy, dz) voxel<BPV>(v, qx + (dx), qy + (dy), qz + (dz))
+#define F(dx, dy, dz) (voxel<BPV>(v, qx + (dx), qy + (dy), qz + (dz)) - bias)
         float D[8];
-        D[0] = voxel<BPV>(v, vx, vy, vz);
+        D[0] = voxel<BPV>(v, vx, vy, vz) - bias;
         D[1] = (F(1, 0, 0) - F(-1, 0, 0)) / 2.0f;
         D[2] = (F(0, 1, 0) - F(0, -1, 0)) / 2.0f;
         D[3] = (F(0, 0, 1) - F(0, 0, -1)) / 2.0f;
@@ -209,13 +209,16 @@ __device__ float tricubic_faces(const VolView &v, const BoxView &bx, int ix, int
     return val;
 }
 
-// Interpolated value at (x, y, z) with the reference's cell split (Point.cpp:57-66:
-// truncated integer part, remainder) and the box semantics above.  method: 0 nearest,
-// 1 trilinear, 2 tricubic.  Callers guarantee every sample lies in a valid cell; the
-// clamps below only keep stray (NaN) coordinates inside the box.
+// Interpolated value minus `bias` at (x, y, z) with the reference's cell split
+// (Point.cpp:57-66: truncated integer part, remainder) and the box semantics above.
+// method: 0 nearest, 1 trilinear, 2 tricubic.  An integer bias near the local mean is
+// subtracted from the voxels exactly before the weighted sums, which keeps the FP32
+// rounding proportional to the local contrast instead of the full intensity range.
+// Callers guarantee every sample lies in a valid cell; the clamps below only keep
+// stray (NaN) coordinates inside the box.
 template <int BPV>
 __device__ __forceinline__ float interp_at(const VolView &v, const BoxView &bx, double x, double y,
-                                           double z, int method, bool want_grad,
+                                           double z, int method, float bias, bool want_grad,
                                            float &gx, float &gy, float &gz) {
     int ix = (int)x, iy = (int)y, iz = (int)z;
     float tx = (float)(x - (double)ix), ty = (float)(y - (double)iy), tz = (float)(z - (double)iz);
@@ -225,7 +228,7 @@ __device__ __forceinline__ float interp_at(const VolView &v, const BoxView &bx, 
         ix = bx.b[0] + clampi(cx, 0, bx.n[0] - 1);
         iy = bx.b[1] + clampi(cy, 0, bx.n[1] - 1);
         iz = bx.b[2] + clampi(cz, 0, bx.n[2] - 1);
-        return voxel<BPV>(v, ix, iy, iz);
+        return voxel<BPV>(v, ix, iy, iz) - bias;
     }
     if (cx < 0 || cx > bx.n[0] - 2 || cy < 0 || cy > bx.n[1] - 2 || cz < 0 || cz > bx.n[2] - 2 ||
         !(tx >= 0.0f && tx <= 1.0f && ty >= 0.0f && ty <= 1.0f && tz >= 0.0f && tz <= 1.0f)) {
@@ -242,10 +245,10 @@ __device__ __forceinline__ float interp_at(const VolView &v, const BoxView &bx, 
     }
     if (method == 1) {
         // tri_lin (Interpolate.cpp:429-470)
-        float c000 = voxel<BPV>(v, ix, iy, iz), c100 = voxel<BPV>(v, ix + 1, iy, iz);
-        float c010 = voxel<BPV>(v, ix, iy + 1, iz), c110 = voxel<BPV>(v, ix + 1, iy + 1, iz);
-        float c001 = voxel<BPV>(v, ix, iy, iz + 1), c101 = voxel<BPV>(v, ix + 1, iy, iz + 1);
-        float c011 = voxel<BPV>(v, ix, iy + 1, iz + 1), c111 = voxel<BPV>(v, ix + 1, iy + 1, iz + 1);
+        float c000 = voxel<BPV>(v, ix, iy, iz) - bias, c100 = voxel<BPV>(v, ix + 1, iy, iz) - bias;
+        float c010 = voxel<BPV>(v, ix, iy + 1, iz) - bias, c110 = voxel<BPV>(v, ix + 1, iy + 1, iz) - bias;
+        float c001 = voxel<BPV>(v, ix, iy, iz + 1) - bias, c101 = voxel<BPV>(v, ix + 1, iy, iz + 1) - bias;
+        float c011 = voxel<BPV>(v, ix, iy + 1, iz + 1) - bias, c111 = voxel<BPV>(v, ix + 1, iy + 1, iz + 1) - bias;
         float x0 = 1.0f - tx, y0 = 1.0f - ty, z0 = 1.0f - tz;
         float a00 = c000 * x0 + c100 * tx, a10 = c010 * x0 + c110 * tx;
         float a01 = c001 * x0 + c101 * tx, a11 = c011 * x0 + c111 * tx;
@@ -260,7 +263,7 @@ __device__ __forceinline__ float interp_at(const VolView &v, const BoxView &bx, 
     }
     cx = ix - bx.b[0]; cy = iy - bx.b[1]; cz = iz - bx.b[2];
     if (cx < 1 || cx > bx.n[0] - 3 || cy < 1 || cy > bx.n[1] - 3 || cz < 1 || cz > bx.n[2] - 3)
-        return tricubic_faces<BPV>(v, bx, ix, iy, iz, tx, ty, tz, want_grad, gx, gy, gz);
+        return tricubic_faces<BPV>(v, bx, ix, iy, iz, tx, ty, tz, bias, want_grad, gx, gy, gz);
     float wx[4], dx[4], wy[4], dy[4], wz[4], dz[4];
     cr_weights(tx, wx, dx);
     cr_weights(ty, wy, dy);
@@ -272,7 +275,7 @@ __device__ __forceinline__ float interp_at(const VolView &v, const BoxView &bx, 
 #pragma unroll
         for (int ky = 0; ky < 4; ky++) {
             float r[4];
-            load_row<BPV>(v, ix - 1, iy - 1 + ky, iz - 1 + kz, r);
+            load_row<BPV>(v, ix - 1, iy - 1 + ky, iz - 1 + kz, bias, r);
             float rv = r[0] * wx[0] + r[1] * wx[1] + r[2] * wx[2] + r[3] * wx[3];
             pv += wy[ky] * rv;
             if (want_grad) {
@@ -294,7 +297,7 @@ __device__ __forceinline__ float interp_at(const VolView &v, const BoxView &bx, 
 // ------------------------------------------------------------- point state
 
 struct PtShared {
-    int status, iters, fail, stop;
+    int status, iters, fail, stop, pfail;
     int b0[3], box_ok;
     int nb_status[kMaxNeigh];
     double nb_par[kMaxNeigh][3];
@@ -305,6 +308,8 @@ struct PtShared {
     float M[3][9], Af[9];
     double bb[6];
     double rmean, R2, Rn2, Src;
+    float fbias;                     // integer intensity subtracted before interpolation sums
+    double rbmean;                   // mean of the biased reference samples
     double tmean, N2, Str, T2, Strr, Szssd;
     double obj, csum, omin;
     int gmin[3];
@@ -562,8 +567,16 @@ __device__ bool evaluate(const SearchArgs &a, PtShared &S, int ndof, bool need_j
     constexpr int NW = B / 64;
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int n = a.nsamp;
-    if (tid == 0) set_warp(S, ndof);
+    if (tid == 0) {
+        set_warp(S, ndof);
+        // non-finite parameters (singular normal matrix) are a range failure; a flag of
+        // its own, so that thread 0 writing S.fail below cannot race with this read
+        S.pfail = 0;
+        for (int q = 0; q < ndof; q++)
+            if (!isfinite(S.evalpar[q])) S.pfail = 1;
+    }
     __syncthreads();
+    if (S.pfail) return false;
     // moving bounding box
     if (ndof > 3) {
         double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
@@ -598,7 +611,8 @@ __device__ bool evaluate(const SearchArgs &a, PtShared &S, int ndof, bool need_j
 
     const VolView vv = {a.vol[1], a.W, (long long)a.W * a.H};
     const BoxView bx = {{S.b0[0], S.b0[1], S.b0[2]}, {a.box[0], a.box[1], a.box[2]}};
-    const double rmean = S.rmean;
+    const double rbmean = S.rbmean;
+    const float fbias = S.fbias;
     const int method = a.interp;
     const bool jrow = need_j;
     floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -616,8 +630,8 @@ __device__ bool evaluate(const SearchArgs &a, PtShared &S, int ndof, bool need_j
             stable_pt(a, S, j, xs, ys, zs);
             warp_pt(S, ndof, xs, ys, zs, x, y, z);
             float gx, gy, gz;
-            float t = interp_at<BPV>(vv, bx, x, y, z, method, jrow, gx, gy, gz);
-            float tc = (float)((double)t - rmean);
+            float t = interp_at<BPV>(vv, bx, x, y, z, method, fbias, jrow, gx, gy, gz);
+            float tc = (float)((double)t - rbmean);
             float rc = rcb[j];
             tcb[j] = tc;
             double d = (double)tc - (double)rc;
@@ -706,7 +720,7 @@ __device__ bool evaluate(const SearchArgs &a, PtShared &S, int ndof, bool need_j
         for (int j = tid; j < n; j += B) {
             double tc = (double)tcb[j], rc = (double)rcb[j];
             double tt = tc - tm;
-            double t = tc + rmean, r = rc + rmean;
+            double t = tc + S.rmean, r = rc + S.rmean;
             double dz = tt - rc;
             b[0] += tt * tt;
             b[1] += tt * rc;
@@ -730,7 +744,7 @@ __device__ bool evaluate(const SearchArgs &a, PtShared &S, int ndof, bool need_j
             double tc = (double)tcb[j], rc = (double)rcb[j];
             double d;
             if (kind == DVC_OBJ_ZNSSD) d = ((tc - tm) / N) - (rc / R);
-            else d = (tc + rmean) / T - (rc + rmean) / Rn;
+            else d = (tc + S.rmean) / T - (rc + S.rmean) / Rn;
             c[0] += d * d;
         }
         block_sum<B, 1>(c, red, &S.csum);
@@ -818,6 +832,12 @@ __device__ void point_search(const SearchArgs &a, int p, int n, PtShared &S, flo
         place_box(a, S, S.sp[0], S.sp[1], S.sp[2]);
         double bb[6] = {S.sbb_mn[0], S.sbb_mn[1], S.sbb_mn[2], S.sbb_mx[0], S.sbb_mx[1], S.sbb_mx[2]};
         S.fail = range_ok(a, S, bb) ? 0 : 1;
+        S.fbias = 0.0f;
+        if (!S.fail) {
+            // the search point lies inside the (valid) reference box
+            const VolView vr = {a.vol[0], a.W, (long long)a.W * a.H};
+            S.fbias = voxel<BPV>(vr, (int)(S.sp[0] / 1), (int)(S.sp[1] / 1), (int)(S.sp[2] / 1));
+        }
     }
     __syncthreads();
     if (S.fail) {
@@ -828,20 +848,22 @@ __device__ void point_search(const SearchArgs &a, int p, int n, PtShared &S, flo
     {
         const VolView vr = {a.vol[0], a.W, (long long)a.W * a.H};
         const BoxView bx = {{S.b0[0], S.b0[1], S.b0[2]}, {a.box[0], a.box[1], a.box[2]}};
+        const float fbias = S.fbias;
         double s[1] = {0.0};
         for (int j = tid; j < ns; j += B) {
             double x, y, z;
             stable_pt(a, S, j, x, y, z);
             float gx, gy, gz;
-            float r = interp_at<BPV>(vr, bx, x, y, z, a.interp, false, gx, gy, gz);
+            float r = interp_at<BPV>(vr, bx, x, y, z, a.interp, fbias, false, gx, gy, gz);
             rcb[j] = r;
             s[0] += (double)r;
         }
         block_sum<B, 1>(s, red, S.sums);
-        const double rmean = S.sums[0] / (double)ns;
+        const double rbmean = S.sums[0] / (double)ns;
+        const double rmean = rbmean + (double)fbias;
         double q3[3] = {0.0, 0.0, 0.0};
         for (int j = tid; j < ns; j += B) {
-            float rc = (float)((double)rcb[j] - rmean);
+            float rc = (float)((double)rcb[j] - rbmean);
             rcb[j] = rc;
             q3[0] += (double)rc * (double)rc;
             q3[1] += (double)rc;
@@ -849,7 +871,10 @@ __device__ void point_search(const SearchArgs &a, int p, int n, PtShared &S, flo
             q3[2] += r * r;
         }
         block_sum<B, 3>(q3, red, &S.sums[1]);
-        if (tid == 0) { S.rmean = rmean; S.R2 = S.sums[1]; S.Src = S.sums[2]; S.Rn2 = S.sums[3]; }
+        if (tid == 0) {
+            S.rmean = rmean; S.rbmean = rbmean;
+            S.R2 = S.sums[1]; S.Src = S.sums[2]; S.Rn2 = S.sums[3];
+        }
     }
     // ---- starting_param: neighbours searched earlier.  Launches run the dependency
     // levels in order, so every such neighbour's result is already in memory.
@@ -918,15 +943,16 @@ __device__ void point_search(const SearchArgs &a, int p, int n, PtShared &S, flo
     double obj_old = 0.0;
     bool converged = false;
     for (int i = 0; i < 20; i++) {
-        if (tid == 0)
+        if (tid == 0) {
             for (int q = 0; q < 12; q++) S.evalpar[q] = S.par[q];
+            S.iters += 1;                 // LM_prep_at calls, counted when attempted
+        }
         __syncthreads();
         if (!evaluate<BPV, B>(a, S, nd, true, rcb, tcb, stage, red, V)) {
             if (tid == 0) S.status = DVC_RANGE_FAIL;
             return;
         }
         if (tid == 0) {
-            S.iters += 1;
             S.stop = 0;
             if (i > 0) {
                 double del_obj = fabs(S.obj - obj_old);
@@ -1049,9 +1075,12 @@ __global__ void interp_kernel(VolView v, BoxView bx, const double *xyz, int n, i
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float gx, gy, gz;
-    float val = interp_at<BPV>(v, bx, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], method, grad != nullptr,
-                               gx, gy, gz);
-    out[i] = (double)val;
+    double x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+    int cx = clampi((int)x - bx.b[0], 0, bx.n[0] - 1), cy = clampi((int)y - bx.b[1], 0, bx.n[1] - 1);
+    int cz = clampi((int)z - bx.b[2], 0, bx.n[2] - 1);
+    float bias = voxel<BPV>(v, bx.b[0] + cx, bx.b[1] + cy, bx.b[2] + cz);
+    float val = interp_at<BPV>(v, bx, x, y, z, method, bias, grad != nullptr, gx, gy, gz);
+    out[i] = (double)val + (double)bias;
     if (grad) { grad[3 * i] = gx; grad[3 * i + 1] = gy; grad[3 * i + 2] = gz; }
 }
 
@@ -1158,6 +1187,7 @@ int launch_swap16(uint16_t *p, long long n, void *stream) {
 // FullPivHouseholderQR(A).solve(b) for the 10x10 normal matrix (Eigen 3.4
 // FullPivHouseholderQR, used at strain.cpp:533).  Q is column major, overwritten.
 __device__ void fullpiv_factor(double *Q, int n, double *hc, int *rt, int *ct, int *rank_out) {
+#pragma clang fp contract(off)
     const double prec = 2.220446049250313e-16 * (double)n;
     double biggest = 0.0, maxpivot = 0.0;
     int nonzero = n;
@@ -1218,6 +1248,7 @@ __device__ void fullpiv_factor(double *Q, int n, double *hc, int *rt, int *ct, i
 
 __device__ void fullpiv_apply(const double *Q, int n, const double *hc, const int *rt, const int *ct, int rank,
                               const double *b, double *x) {
+#pragma clang fp contract(off)
     double c[10];
     int perm[10];
     for (int i = 0; i < n; i++) { perm[i] = i; x[i] = 0.0; c[i] = b[i]; }
@@ -1246,6 +1277,7 @@ __device__ void fullpiv_apply(const double *Q, int n, const double *hc, const in
 // Eigenvalues of a symmetric 3x3 (cyclic Jacobi), sorted descending
 // (strain.cpp:553-556 takes EigenSolver real parts and sorts them).
 __device__ void sym3_eigen(const double *S, double *lam) {
+#pragma clang fp contract(off)
     double a[3][3];
     for (int i = 0; i < 3; i++)
         for (int j = 0; j < 3; j++) a[i][j] = 0.5 * (S[i * 3 + j] + S[j * 3 + i]);
@@ -1279,8 +1311,12 @@ __device__ void sym3_eigen(const double *S, double *lam) {
 
 // One wavefront per point: window selection (strain.cpp:424-448), design rows
 // (:478-527), normal equations by lanes, FP64 solve and strain tensors (:529-590).
+// All FP64 arithmetic keeps the reference's operation order without fused
+// multiply-add, so windows too small to determine the 10-term fit (rank detection of
+// the full-pivot QR) resolve exactly as the serial restatement does.
 constexpr int kStrainWaves = 4;
 __global__ void __launch_bounds__(64 * kStrainWaves) strain_kernel(StrainArgs a) {
+#pragma clang fp contract(off)
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int i = blockIdx.x * kStrainWaves + w;
     DVC_SHARED(double[kStrainWaves][75][10], Xs);

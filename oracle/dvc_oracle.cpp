@@ -421,6 +421,12 @@ struct Interp {
      * box inside the volume, and every sample cell with its upper corner inside the box
      * (the reference indexes one cell past kern_4d otherwise, Interpolate.cpp:494-500). */
     bool range_ok(const Pt *pts, int n, const Box &bbox, bool need_upper) const {
+        // A non-finite box (parameters gone NaN after a singular solve) would pass the
+        // comparisons of BoundBox::contains and index kern_4d out of range in the
+        // reference; it is a range failure here.
+        const double b6[6] = {bbox.mn.x, bbox.mn.y, bbox.mn.z, bbox.mx.x, bbox.mx.y, bbox.mx.z};
+        for (int q = 0; q < 6; q++)
+            if (!(b6[q] == b6[q]) || std::fabs(b6[q]) > 1.0e15) return false;
         if (!act.contains(bbox)) return false;
         if (!box_ok) return false;
         int bx = est.mn.ix, by = est.mn.iy, bz = est.mn.iz;
@@ -796,6 +802,10 @@ struct Searcher {
 
     /* obj_val_at (Search.cpp:448-488); returns 0 or -1 (Range_Fail). */
     int obj_val_at(const double *x, int xs, double &obj, double *res, double *grad) {
+        // Non-finite parameters (a singular normal matrix) make every sample NaN; the
+        // reference then indexes kern_4d with undefined cells.  Range failure here.
+        for (int q = 0; q < xs; q++)
+            if (!std::isfinite(x[q])) return -1;
         fc->affine_to(x, xs, xs);
         if (sample(fc->moving, tar_sv.data(), grad) != 0) return -1;
         obj = objective(rc.obj, ref_sv.data(), tar_sv.data(), subv_num, res);

@@ -63,7 +63,7 @@ def strain(pts, disp, status, objmin, neigh, ndp=25, objt=1.0, refill=False, pla
     o = {"disp": np.zeros((n, 3)), "npts": np.zeros(n, np.int32), "swr": np.zeros(n),
          "E": np.zeros((n, 9)), "L": np.zeros((n, 9)), "DG": np.zeros((n, 9))}
     vp = lambda a: a.ctypes.data_as(C.c_void_p)
-    lib.emu_strain.argtypes = [C.c_void_p] * 5 + [C.c_int] * 4 + [C.c_double] + [C.c_int] * 4 + [C.c_void_p] * 6
+    lib.emu_strain.argtypes = [C.c_void_p] * 5 + [C.c_int] * 3 + [C.c_double] + [C.c_int] * 4 + [C.c_void_p] * 6
     lib.emu_strain(vp(pts), vp(disp), vp(st), vp(om), vp(nb), k, n, ndp, float(objt), int(refill), *planar,
                    vp(o["disp"]), vp(o["npts"]), vp(o["swr"]), vp(o["E"]), vp(o["L"]), vp(o["DG"]))
     return o

@@ -49,6 +49,8 @@ extern "C" int emu_search(const dvc_run_params *rc, const void *ref, const void 
     size_t lds = (size_t)a.nsamp_pad * 8 + (size_t)(block_threads / 64) * 64 * 17 * 4;
     if (bpv == 1 && block_threads == 64) run_search<1, 64>(a, lds, lp, lo);
     else if (bpv == 1 && block_threads == 256) run_search<1, 256>(a, lds, lp, lo);
+    else if (bpv == 1 && block_threads == 128) run_search<1, 128>(a, lds, lp, lo);
+    else if (bpv == 2 && block_threads == 128) run_search<2, 128>(a, lds, lp, lo);
     else if (bpv == 2 && block_threads == 64) run_search<2, 64>(a, lds, lp, lo);
     else if (bpv == 2 && block_threads == 256) run_search<2, 256>(a, lds, lp, lo);
     else return -1;
