@@ -1,0 +1,189 @@
+"""Throughput of the DVC correlation path on one MI355X.
+
+Workload: 256^3 uint16 reference/target pair (smoothed noise, sigma 1.5; target under
+an affine map: 0.5 % strain along z, 1 degree about x, translation (2.4, 0.3, -1.1)),
+a 15^3 grid of 3375 points at spacing 12, sphere subvolumes of diameter 41 with 5000
+samples, ZNSSD, tricubic, 12-DOF Gauss-Newton search.  One step = one full search of
+every point (dvc_search_execute: the point loop of Core/dvc.cpp:233-303), volumes and
+point tables already resident in HBM.
+
+Prints one JSON line:  metric/value (points/s), roofline of the search kernel (HIP
+events around each step on the library's stream), and the oracle's CPU throughput on
+a bounded prefix of the same processing order (same box, stated thread count), with
+the displacement agreement of the GPU run against it.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--cpu-points M] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "points/s at <=1e-3 voxel agreement: plain GPU vs NN-proposal GPU vs CPU OpenMP"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--threads", type=int, default=0, help="threads per point (0 = library default)")
+    ap.add_argument("--cpu-points", type=int, default=384, help="oracle sample: first M points of the order")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (0 = min(16, cores))")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--bytes-sample", type=int, default=400, help="points sampled for the algorithmic byte count")
+    ap.add_argument("--cache", default="", help="directory to keep the generated volumes in between runs")
+    return ap.parse_args()
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def load_case(cache: str, synth):
+    """The synthetic workload; its two volumes optionally kept as .npy under `cache`."""
+    if not cache:
+        return synth.config2()
+    d = Path(cache)
+    r, t = d / "affine_u16_256_ref.npy", d / "affine_u16_256_tar.npy"
+    if r.exists() and t.exists():
+        case = synth.config2(volumes=(np.load(r), np.load(t)))
+    else:
+        case = synth.config2()
+        d.mkdir(parents=True, exist_ok=True)
+        np.save(r, case.ref)
+        np.save(t, case.tar)
+    return case
+
+
+def main():
+    a = parse()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print(f"bench.py measures one GPU only: the search runs on a single device (asked for {a.gpus}); "
+              "run it without --gpus or with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import digitalvolumecorrelation_amd as D
+    from digitalvolumecorrelation_amd import synth, workload
+
+    t0 = time.time()
+    case = load_case(a.cache, synth)
+    p = case.params
+    order, neigh = D.sort_order_neighbors(case.points, case.start)
+    log(f"workload built in {time.time() - t0:.1f} s: {len(case.points)} points, volumes {case.ref.shape} "
+        f"{case.ref.dtype}")
+
+    ctx = D.Context(0)
+    ctx.upload(0, case.ref)
+    ctx.upload(1, case.tar)
+    if a.threads:
+        ctx.set_block_threads(a.threads)
+    params = D.make_params(**p)
+    ctx.prepare(params, case.points, order, neigh)
+
+    for _ in range(a.warmup):
+        ctx.execute()
+        ctx.kernel_ms()
+    log(f"warmup done ({a.warmup} steps)")
+
+    dev_ms = []
+    t_start = time.perf_counter()
+    for _ in range(a.steps):
+        ctx.execute()
+        dev_ms.append(ctx.kernel_ms())       # waits for the step's closing event
+    t_end = time.perf_counter()
+    wall_ms = (t_end - t_start) * 1e3 / a.steps
+    res = ctx.fetch()
+    npts = len(case.points)
+    value = npts / (wall_ms / 1e3)
+    log(f"timed {a.steps} steps: {wall_ms:.2f} ms/step wall, {np.mean(dev_ms):.2f} ms/step device")
+
+    # the strain field of the search (Core/strain.cpp:402-591, 26 grid neighbours + self)
+    disp = res["par"][:, :3].copy()
+    strain_ms = []
+    for _ in range(3):
+        ctx.strain(case.points, disp, res["status"], res["obj"], neigh, ndp=27)
+        strain_ms.append(ctx.kernel_ms())
+
+    good = res["status"] == 0
+    truth_err = float(np.abs(res["par"][good, :3] - case.truth[good]).max()) if good.any() else None
+
+    # roofline of the search kernel: compulsory volume bytes of the searched points over
+    # the device time of the step (all its launches: one per dependency level)
+    offs = workload.sphere_offsets(params.sphere_seed, p["subvol_size"] / 2.0, p["subvol_npts"])
+    per_pt, nsamp = workload.search_bytes(case.points, offs, res, p["ndof"], case.ref.dtype.itemsize,
+                                          sample=a.bytes_sample)
+    step_bytes = per_pt * int(good.sum())
+    dev_s = float(np.mean(dev_ms)) / 1e3
+    achieved = step_bytes / dev_s / 1e9
+    traffic = None
+    tf = ROOT / "profiles" / "search_kernel_affine_u16_256_traffic.json"
+    if tf.exists():
+        traffic = json.loads(tf.read_text()).get("hbm_bytes_per_step")
+    roof = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "kernel": "search_kernel<2,B>", "launches_per_step": ctx.levels(),
+            "algorithmic_bytes_per_point": round(per_pt, 1), "bytes_points_sampled": nsamp}
+
+    cpu = None
+    agree = None
+    if not a.no_cpu:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import oracle as O
+        thr = a.cpu_threads or min(16, os.cpu_count() or 1)
+        M = min(a.cpu_points, npts)
+        run = O.make_run(shape=case.ref.shape, bytes_per=case.ref.dtype.itemsize, jacobian=0, **p)
+        log(f"oracle: first {M} points of the order, {thr} threads")
+        ores, sec, n = O.dvc_run(run, case.ref, case.tar, case.points, order, neigh, nproc=M, nthreads=thr)
+        cpu = {"value": n / sec, "unit": "points/s", "cores": thr, "kind": "port",
+               "sample": f"first {n} points of the processing order of the same workload, serial point loop "
+                         f"with {thr} OpenMP threads inside interpolation, forward-difference Jacobian "
+                         f"(h = 1e-10), {sec:.1f} s"}
+        idx = order[:n]
+        g, o = res[idx], ores[idx]
+        ok = o["status"] == 0
+        du = float(np.abs(g["par"][ok, :3] - o["par"][ok, :3]).max()) if ok.any() else 0.0
+        rel = float((np.abs(g["obj"][ok] - o["obj"][ok]) / np.abs(o["obj"][ok])).max()) if ok.any() else 0.0
+        agree = {"points": int(n), "status_equal": bool(np.array_equal(g["status"], o["status"])),
+                 "iters_equal": int((g["iters"] == o["iters"]).sum()), "max_du_voxel": du, "max_obj_rel": rel,
+                 "within_1e-3": bool(du <= 1e-3)}
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "points/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": wall_ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "plain 12-DOF affine, 256^3 u16 pair, 15^3 grid (3375 points) at spacing 12, "
+                               "sphere d=41 / 5000 samples, ZNSSD, tricubic, disp_max 10",
+                   "points": npts, "good": int(good.sum()), "threads_per_point": ctx.block_threads(),
+                   "device_ms_per_step": float(np.mean(dev_ms)), "max_err_vs_imposed_voxel": truth_err,
+                   "strain_fit_device_ms": float(np.min(strain_ms)),
+                   "arith": "f32 interpolation and sums, f64 warp, normal equations and solve"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "agreement_vs_cpu": agree,
+    }
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

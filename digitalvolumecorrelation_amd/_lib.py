@@ -69,6 +69,8 @@ def load() -> C.CDLL:
         "dvc_search_execute": (I, [VP]),
         "dvc_search_fetch": (I, [VP, VP]),
         "dvc_set_block_threads": (I, [VP, C.c_int]),
+        "dvc_set_launch_mode": (I, [VP, C.c_int]),
+        "dvc_search_info": (I, [VP, C.c_void_p, C.c_void_p, C.c_void_p]),
         "dvc_strain_fit": (I, [VP, P(D), P(D), P(I), P(D), P(I), I, I, I, D, I, I,
                                P(D), P(I), P(D), P(D), P(D), P(D)]),
         "dvc_sort_order_neighbors": (I, [P(D), I, P(D), P(I), P(I), I]),
@@ -196,6 +198,9 @@ class Context:
     def set_block_threads(self, n: int):
         self._check(self.lib.dvc_set_block_threads(self.h, n))
 
+    def set_launch_mode(self, graph: bool):
+        self._check(self.lib.dvc_set_launch_mode(self.h, int(graph)))
+
     def prepare(self, params, points, order, neigh, nproc=0):
         self._pts = np.ascontiguousarray(points, dtype=np.float64)
         self._ord = np.ascontiguousarray(order, dtype=np.int32)
@@ -203,6 +208,18 @@ class Context:
         self._npts = self._pts.shape[0]
         self._check(self.lib.dvc_search_prepare(self.h, C.byref(params), _dp(self._pts), self._npts, _ip(self._ord),
                                                 nproc, _ip(self._nb), self._nb.shape[1]))
+
+    def _info(self):
+        v = [C.c_int32(), C.c_int32(), C.c_int32()]
+        self._check(self.lib.dvc_search_info(self.h, *[C.byref(x) for x in v]))
+        return [x.value for x in v]
+
+    def levels(self) -> int:
+        """Dependency levels of the prepared search (one kernel launch each)."""
+        return self._info()[0]
+
+    def block_threads(self) -> int:
+        return self._info()[1]
 
     def execute(self):
         self._check(self.lib.dvc_search_execute(self.h))

@@ -42,6 +42,7 @@ struct dvc_ctx {
     size_t lds_max = 0;
     int block_threads = 0;
     float last_ms = 0.0f;
+    bool timing_pending = false;     // ev0/ev1 of an execute not yet read
     // search plan
     bool plan_ready = false;
     dvc_run_params rc{};
@@ -54,6 +55,10 @@ struct dvc_ctx {
     int chosen_threads = 256;
     std::vector<char> searched;
     std::vector<int> level_off;      // dependency levels (dvc_geometry.h)
+    // the level launches of a prepared search, captured once and replayed
+    int use_graph = 1;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
 };
 
 namespace {
@@ -81,7 +86,15 @@ int dev_alloc(dvc_ctx *ctx, T **p, size_t count) {
     return 0;
 }
 
+void drop_graph(dvc_ctx *c) {
+    if (c->gexec) hipGraphExecDestroy(c->gexec);
+    if (c->graph) hipGraphDestroy(c->graph);
+    c->gexec = nullptr;
+    c->graph = nullptr;
+}
+
 void free_plan(dvc_ctx *c) {
+    drop_graph(c);
     hipFree(c->d_pts); hipFree(c->d_sph); hipFree(c->d_order); hipFree(c->d_pos);
     hipFree(c->d_neigh); hipFree(c->d_levels); hipFree(c->d_res);
     c->d_pts = c->d_sph = nullptr;
@@ -134,6 +147,7 @@ int dvc_create(dvc_ctx **out, int device) {
     if (hipSetDevice(device) != hipSuccess) { delete ctx; return DVC_ERR_NODEV; }
     ctx->cus = prop.multiProcessorCount;
     ctx->lds_max = prop.sharedMemPerBlock;
+    if (search_set_lds_limit(ctx->lds_max) != 0) { delete ctx; return DVC_ERR_HIP; }
     if (hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
         delete ctx;
@@ -165,8 +179,15 @@ int dvc_set_stream(dvc_ctx *ctx, void *s) {
     return DVC_OK;
 }
 
-int dvc_last_kernel_ms(const dvc_ctx *ctx, float *ms) {
+int dvc_last_kernel_ms(dvc_ctx *ctx, float *ms) {
     if (!ctx || !ms) return DVC_ERR_ARG;
+    if (ctx->timing_pending) {
+        HIPCHK(hipEventSynchronize(ctx->ev1));
+        float t = 0.0f;
+        HIPCHK(hipEventElapsedTime(&t, ctx->ev0, ctx->ev1));
+        ctx->last_ms = t;
+        ctx->timing_pending = false;
+    }
     *ms = ctx->last_ms;
     return DVC_OK;
 }
@@ -187,6 +208,7 @@ int dvc_volume_upload(dvc_ctx *ctx, int slot, const void *host, int64_t header, 
         return fail(ctx, DVC_ERR_ARG, "invalid volume description");
     if (w > 2147483647LL / 2 || h > 2147483647LL || t > 2147483647LL) return fail(ctx, DVC_ERR_ARG, "volume too large");
     HIPCHK(hipSetDevice(ctx->device));
+    drop_graph(ctx);
     Volume &v = ctx->vol[slot];
     if (v.owned) hipFree(v.d);
     v = Volume();
@@ -207,6 +229,7 @@ int dvc_volume_attach(dvc_ctx *ctx, int slot, const void *dptr, int bpv, int64_t
     if (slot != 0 && slot != 1) return fail(ctx, DVC_ERR_ARG, "slot must be DVC_VOL_REF or DVC_VOL_COR");
     if (!dptr || (bpv != 1 && bpv != 2) || w < 4 || h < 4 || t < 4) return fail(ctx, DVC_ERR_ARG, "invalid volume description");
     if (((uintptr_t)dptr & 3) != 0) return fail(ctx, DVC_ERR_ARG, "device volume must be 4-byte aligned");
+    drop_graph(ctx);
     Volume &v = ctx->vol[slot];
     if (v.owned) hipFree(v.d);
     v = Volume();
@@ -293,6 +316,7 @@ int dvc_search_prepare(dvc_ctx *ctx, const dvc_run_params *rc, const double *poi
                        const int32_t *order, int32_t nproc, const int32_t *neigh, int32_t k) {
     if (!ctx) return DVC_ERR_ARG;
     ctx->plan_ready = false;
+    drop_graph(ctx);
     if (int e = check_params(ctx, rc)) return e;
     if (int e = check_volumes(ctx)) return e;
     if (npts < 1 || !points || !order || !neigh) return fail(ctx, DVC_ERR_ARG, "bad point arrays");
@@ -345,11 +369,9 @@ int dvc_search_prepare(dvc_ctx *ctx, const dvc_run_params *rc, const double *poi
     return DVC_OK;
 }
 
-int dvc_search_execute(dvc_ctx *ctx) {
-    if (!ctx) return DVC_ERR_ARG;
-    if (!ctx->plan_ready) return fail(ctx, DVC_ERR_STATE, "dvc_search_prepare first");
-    if (int e = check_volumes(ctx)) return e;
-    HIPCHK(hipSetDevice(ctx->device));
+// Enqueue the whole search on `st`: clear the results, then one launch per dependency
+// level (a level's points only read results of earlier levels).
+static int enqueue_search(dvc_ctx *ctx, hipStream_t st) {
     const Geometry &G = ctx->geo;
     const dvc_run_params &rc = ctx->rc;
     SearchArgs a;
@@ -360,17 +382,58 @@ int dvc_search_execute(dvc_ctx *ctx) {
     a.res = ctx->d_res;
     int bt = ctx->chosen_threads;
     size_t lds = search_lds_bytes(a.nsamp_pad, bt);
-    // results start zeroed; points never searched become not_search in fetch
-    if (launch_zero_int((int *)ctx->d_res, (long long)ctx->npts * (long long)(sizeof(dvc_point_result) / 4), ctx->stream) != 0)
+    if (launch_zero_int((int *)ctx->d_res, (long long)ctx->npts * (long long)(sizeof(dvc_point_result) / 4), st) != 0)
         return fail(ctx, DVC_ERR_HIP, "zero launch");
-    HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
     const int depth = (int)ctx->level_off.size() - 1;
     for (int L = 0; L < depth; L++) {
         int cnt = ctx->level_off[L + 1] - ctx->level_off[L];
-        if (launch_search(a, ctx->d_levels + ctx->level_off[L], bt, cnt, lds, ctx->stream) != 0)
+        if (launch_search(a, ctx->d_levels + ctx->level_off[L], bt, cnt, lds, st) != 0)
             return fail(ctx, DVC_ERR_HIP, "search kernel launch failed");
     }
+    return DVC_OK;
+}
+
+int dvc_search_execute(dvc_ctx *ctx) {
+    if (!ctx) return DVC_ERR_ARG;
+    if (!ctx->plan_ready) return fail(ctx, DVC_ERR_STATE, "dvc_search_prepare first");
+    if (int e = check_volumes(ctx)) return e;
+    HIPCHK(hipSetDevice(ctx->device));
+    if (ctx->use_graph && !ctx->gexec) {
+        if (hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+            return fail(ctx, DVC_ERR_HIP, "stream capture could not start");
+        int e = enqueue_search(ctx, ctx->stream);
+        hipGraph_t g = nullptr;
+        hipError_t ce = hipStreamEndCapture(ctx->stream, &g);
+        if (e != 0) { if (g) hipGraphDestroy(g); return e; }
+        if (ce != hipSuccess) return hip_fail(ctx, ce, "hipStreamEndCapture");
+        ctx->graph = g;
+        hipError_t ie = hipGraphInstantiate(&ctx->gexec, g, nullptr, nullptr, 0);
+        if (ie != hipSuccess) { drop_graph(ctx); return hip_fail(ctx, ie, "hipGraphInstantiate"); }
+    }
+    HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+    if (ctx->use_graph) {
+        HIPCHK(hipGraphLaunch(ctx->gexec, ctx->stream));
+    } else if (int e = enqueue_search(ctx, ctx->stream)) {
+        return e;
+    }
     HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+    ctx->timing_pending = true;
+    return DVC_OK;
+}
+
+int dvc_search_info(dvc_ctx *ctx, int32_t *levels, int32_t *threads, int32_t *searched) {
+    if (!ctx) return DVC_ERR_ARG;
+    if (!ctx->plan_ready) return fail(ctx, DVC_ERR_STATE, "dvc_search_prepare first");
+    if (levels) *levels = (int32_t)ctx->level_off.size() - 1;
+    if (threads) *threads = ctx->chosen_threads;
+    if (searched) *searched = ctx->nproc;
+    return DVC_OK;
+}
+
+int dvc_set_launch_mode(dvc_ctx *ctx, int graph) {
+    if (!ctx) return DVC_ERR_ARG;
+    if ((graph != 0) != (ctx->use_graph != 0)) drop_graph(ctx);
+    ctx->use_graph = graph ? 1 : 0;
     return DVC_OK;
 }
 
@@ -381,7 +444,8 @@ int dvc_search_fetch(dvc_ctx *ctx, dvc_point_result *results) {
     HIPCHK(hipMemcpyAsync(results, ctx->d_res, sizeof(dvc_point_result) * ctx->npts, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     float ms = 0.0f;
-    if (hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) == hipSuccess) ctx->last_ms = ms;
+    if (ctx->timing_pending && hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) == hipSuccess) ctx->last_ms = ms;
+    ctx->timing_pending = false;
     // points never searched keep the not_search record of DataCloud.cpp:168-170
     for (int i = 0; i < ctx->npts; i++)
         if (!ctx->searched[i]) {
@@ -450,6 +514,7 @@ int dvc_strain_fit(dvc_ctx *ctx, const double *pts, const double *disp, const in
     hipError_t e = hipStreamSynchronize(ctx->stream);
     float ms = 0.0f;
     if (e == hipSuccess && hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1) == hipSuccess) ctx->last_ms = ms;
+    ctx->timing_pending = false;
     hipFree(buf);
     if (r != 0) return fail(ctx, DVC_ERR_HIP, "strain launch failed");
     if (e != hipSuccess) return hip_fail(ctx, e, "strain");

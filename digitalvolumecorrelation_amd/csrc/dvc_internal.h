@@ -9,7 +9,7 @@ namespace dvc {
 
 constexpr int kMaxNeigh = 128;       // neighbour slots read per point (reference: 75)
 
-// Arguments of the persistent search kernel.  Geometry mirrors Search::Search
+// Arguments of the search kernel.  Geometry mirrors Search::Search
 // (Search.cpp:24-82) and Interpolate::Interpolate (Interpolate.cpp:34-56).
 struct SearchArgs {
     const uint8_t *vol[2];           // reference and correlated volume bytes
@@ -43,6 +43,7 @@ struct StrainArgs {
 // launchers implemented in dvc_kernels.hip
 int launch_search(const SearchArgs &a, const int *level_pts, int block_threads, int grid_blocks, size_t lds_bytes,
                   void *stream);
+int search_set_lds_limit(size_t lds_total);
 int search_max_blocks(int bpv, int block_threads, size_t lds_bytes, int *blocks_per_cu);
 size_t search_lds_bytes(int nsamp_pad, int block_threads);
 int launch_interp(const uint8_t *vol, int bpv, int W, int H, int T, const int b0[3], const int box[3],

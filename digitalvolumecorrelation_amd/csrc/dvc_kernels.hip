@@ -1,6 +1,7 @@
 // dvc_kernels.hip -- gfx950 kernels of the DVC per-point correlation path.
 //
-// Search kernel (one workgroup per ROI point, persistent, dependency ordered):
+// Search kernel (one workgroup per ROI point, one launch per dependency level of the
+// serial point order):
 //   Search::process_point            Core/Search.cpp:95-192
 //   Search::search_pt_setup          Core/Search.cpp:248-275
 //   Search::starting_param           Core/Search.cpp:277-377
@@ -1020,15 +1021,26 @@ size_t search_lds_bytes(int nsamp_pad, int block_threads) {
 template <int BPV, int B>
 static int launch_search_t(const SearchArgs &a, const int *level_pts, int grid, size_t lds, hipStream_t st) {
     auto k = search_kernel<BPV, B>;
-    if (hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) return -1;
     hipLaunchKernelGGL(k, dim3(grid), dim3(B), lds, st, a, level_pts);
     return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// Raise the dynamic-LDS limit of one instantiation to everything its static LDS leaves
+// of `lds_total`.  Done once per context, outside any stream capture, so that launches
+// (direct or replayed from a graph) never change function attributes.
+template <int BPV, int B>
+static int lds_limit_t(size_t lds_total, int *) {
+    auto k = search_kernel<BPV, B>;
+    hipFuncAttributes fa;
+    if (hipFuncGetAttributes(&fa, (const void *)k) != hipSuccess) return -1;
+    if (fa.sharedSizeBytes >= lds_total) return -1;
+    int dyn = (int)(lds_total - fa.sharedSizeBytes);
+    return hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, dyn) == hipSuccess ? 0 : -1;
 }
 
 template <int BPV, int B>
 static int occ_t(size_t lds, int *bpc) {
     auto k = search_kernel<BPV, B>;
-    if (hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) return -1;
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(bpc, (const void *)k, B, lds) == hipSuccess ? 0 : -1;
 }
 
@@ -1049,6 +1061,27 @@ int launch_search(const SearchArgs &a, const int *level_pts, int block_threads, 
     DVC_DISPATCH(2, 512, launch_search_t)(a, level_pts, grid, lds, st);
     DVC_DISPATCH(2, 1024, launch_search_t)(a, level_pts, grid, lds, st);
     return -1;
+}
+
+static int lds_limit_one(int bpv, int block_threads, size_t lds_total) {
+    DVC_DISPATCH(1, 64, lds_limit_t)(lds_total, nullptr);
+    DVC_DISPATCH(1, 128, lds_limit_t)(lds_total, nullptr);
+    DVC_DISPATCH(1, 256, lds_limit_t)(lds_total, nullptr);
+    DVC_DISPATCH(1, 512, lds_limit_t)(lds_total, nullptr);
+    DVC_DISPATCH(1, 1024, lds_limit_t)(lds_total, nullptr);
+    DVC_DISPATCH(2, 64, lds_limit_t)(lds_total, nullptr);
+    DVC_DISPATCH(2, 128, lds_limit_t)(lds_total, nullptr);
+    DVC_DISPATCH(2, 256, lds_limit_t)(lds_total, nullptr);
+    DVC_DISPATCH(2, 512, lds_limit_t)(lds_total, nullptr);
+    DVC_DISPATCH(2, 1024, lds_limit_t)(lds_total, nullptr);
+    return -1;
+}
+
+int search_set_lds_limit(size_t lds_total) {
+    for (int bpv = 1; bpv <= 2; bpv++)
+        for (int bt = 64; bt <= 1024; bt *= 2)
+            if (lds_limit_one(bpv, bt, lds_total) != 0) return -1;
+    return 0;
 }
 
 int search_max_blocks(int bpv, int block_threads, size_t lds, int *bpc) {

@@ -152,15 +152,20 @@ def config1(size: int = 128, n: int = 10, spacing: float = 8.0, seed: int = 1,
 
 def config2(size: int = 256, n: int = 15, spacing: float = 12.0, seed: int = 2,
             t=(2.4, 0.3, -1.1), rot_deg: float = 1.0, strain_z: float = 0.005, npts: int = 5000,
-            dof: int = 12, diameter: int = 41) -> Case:
+            dof: int = 12, diameter: int = 41, volumes=None) -> Case:
     """Plain 12-DOF affine case: uint16 size^3, sigma 1.5, rotation about x through the
     volume centre, uniaxial strain along z, sphere 41 / 5000 samples, ZNSSD, disp_max 10,
-    processing order starting at the volume centre."""
-    ref = smoothed_noise((size, size, size), 1.5, np.uint16, seed)
+    processing order starting at the volume centre.  `volumes` = (ref, tar) made by an
+    earlier call with the same arguments skips regenerating them."""
     c = np.full(3, (size - 1) / 2.0)
     F = rot_x(rot_deg) @ np.diag([1.0, 1.0, 1.0 + strain_z])
     A = Affine(F, np.asarray(t, dtype=np.float64), c)
-    tar = warp(ref, A.inverse)
+    if volumes is not None:
+        ref, tar = volumes
+        assert ref.shape == (size, size, size) and ref.dtype == np.uint16 and tar.shape == ref.shape
+    else:
+        ref = smoothed_noise((size, size, size), 1.5, np.uint16, seed)
+        tar = warp(ref, A.inverse)
     pts = grid_points(n, spacing, (size / 2.0,) * 3)
     truth = A.displacement(pts)
     start = np.full(3, size / 2.0)

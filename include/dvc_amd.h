@@ -87,9 +87,9 @@ void dvc_destroy(dvc_ctx *ctx);
 const char *dvc_last_error(const dvc_ctx *ctx);
 /* Run subsequent work on an existing hipStream_t (NULL: the context's own stream). */
 int  dvc_set_stream(dvc_ctx *ctx, void *hip_stream);
-/* Milliseconds of the last search / strain kernel, timed with HIP events on the
- * stream the kernel ran on. */
-int  dvc_last_kernel_ms(const dvc_ctx *ctx, float *ms);
+/* Milliseconds of the last search execute / strain fit on the device, timed with HIP
+ * events on the stream the kernels ran on.  Waits for that work to finish. */
+int  dvc_last_kernel_ms(dvc_ctx *ctx, float *ms);
 
 /* ---------------------------------------------------------------- volumes */
 
@@ -145,8 +145,15 @@ int dvc_search_prepare(dvc_ctx *ctx, const dvc_run_params *rc, const double *poi
 int dvc_search_execute(dvc_ctx *ctx);
 int dvc_search_fetch(dvc_ctx *ctx, dvc_point_result *results);
 
-/* Launch tuning: threads per point (64, 128, 256, 512 or 1024; 0 = automatic). */
+/* Launch tuning: threads per point (64, 128, 256, 512 or 1024; 0 = automatic), applied
+ * by the next dvc_search_prepare. */
 int dvc_set_block_threads(dvc_ctx *ctx, int threads);
+/* 1 (default): the level launches of a prepared search are captured into a HIP graph
+ * once and replayed by dvc_search_execute; 0: launched one by one. */
+int dvc_set_launch_mode(dvc_ctx *ctx, int graph);
+/* Shape of the prepared search: dependency levels (one kernel launch each), threads
+ * per point, points searched.  Any pointer may be NULL. */
+int dvc_search_info(dvc_ctx *ctx, int32_t *levels, int32_t *threads_per_point, int32_t *points_searched);
 
 /* ----------------------------------------------------------------- strain */
 
