@@ -315,10 +315,6 @@ struct PtShared {
     double obj, csum, omin;
     int gmin[3];
     double sums[8];
-    double JTJ[144], rhs[12];
-    // column-pivoted QR workspace (kept in LDS: no private-memory arrays)
-    double qr[144], qhc[12], qnu[12], qnd[12], qc[12], qx[12];
-    int qtr[12], qperm[12];
 };
 
 // Stable cloud sample j (FloatingCloud.cpp:84-128 cube grid, z outer / x inner;
@@ -452,109 +448,6 @@ __device__ __forceinline__ void set_warp(PtShared &S, int ndof) {
                 S.M[r][i * 3 + j] = (float)s;
             }
     for (int q = 0; q < 9; q++) S.Af[q] = (float)S.A[q];
-}
-
-// Column-pivoted Householder QR solve (Eigen 3.4 ColPivHouseholderQR::computeInPlace
-// and _solve_impl, as used at Search.cpp:234) of S.JTJ (n x n, row-major) x = S.rhs,
-// result in S.qx.  Thread 0; the workspace lives in LDS (PtShared::q*).
-__device__ void colpiv_solve(PtShared &S, int n) {
-    double *Q = S.qr, *hc = S.qhc, *nu = S.qnu, *nd = S.qnd, *c = S.qc, *x = S.qx;
-    int *tr = S.qtr, *perm = S.qperm;
-    for (int i = 0; i < n; i++)
-        for (int j = 0; j < n; j++) Q[j * n + i] = S.JTJ[i * n + j];
-    for (int k = 0; k < n; k++) {
-        double s = 0.0;
-        for (int i = 0; i < n; i++) s += Q[k * n + i] * Q[k * n + i];
-        nd[k] = sqrt(s);
-        nu[k] = nd[k];
-    }
-    double maxn = 0.0;
-    for (int k = 0; k < n; k++) maxn = fmax(maxn, nu[k]);
-    const double eps = 2.220446049250313e-16;
-    double thr = (maxn * eps) * (maxn * eps) / (double)n;
-    double dthr = sqrt(eps);
-    int nonzero = n;
-    for (int k = 0; k < n; k++) {
-        int big = k;
-        double bn = nu[k];
-        for (int j = k + 1; j < n; j++)
-            if (nu[j] > bn) { bn = nu[j]; big = j; }
-        if (nonzero == n && bn * bn < thr * (double)(n - k)) nonzero = k;
-        tr[k] = big;
-        if (k != big) {
-            for (int i = 0; i < n; i++) { double t = Q[k * n + i]; Q[k * n + i] = Q[big * n + i]; Q[big * n + i] = t; }
-            double t = nu[k]; nu[k] = nu[big]; nu[big] = t;
-            t = nd[k]; nd[k] = nd[big]; nd[big] = t;
-        }
-        // makeHouseholder of column k, rows k..n-1
-        double *col = &Q[k * n + k];
-        int m = n - k;
-        double tail = 0.0;
-        for (int i = 1; i < m; i++) tail += col[i] * col[i];
-        double c0 = col[0], tau, beta;
-        if (tail <= 2.2250738585072014e-308) {
-            tau = 0.0; beta = c0;
-            for (int i = 1; i < m; i++) col[i] = 0.0;
-        } else {
-            beta = sqrt(c0 * c0 + tail);
-            if (c0 >= 0.0) beta = -beta;
-            for (int i = 1; i < m; i++) col[i] = col[i] / (c0 - beta);
-            tau = (beta - c0) / beta;
-        }
-        hc[k] = tau;
-        col[0] = beta;
-        if (m > 1 && tau != 0.0) {
-            for (int j = k + 1; j < n; j++) {
-                double *cj = &Q[j * n + k];
-                double t = 0.0;
-                for (int i = 1; i < m; i++) t += col[i] * cj[i];
-                t += cj[0];
-                cj[0] -= tau * t;
-                for (int i = 1; i < m; i++) cj[i] -= tau * col[i] * t;
-            }
-        } else if (m == 1) {
-            for (int j = k + 1; j < n; j++) Q[j * n + k] *= (1.0 - tau);
-        }
-        for (int j = k + 1; j < n; j++) {
-            if (nu[j] != 0.0) {
-                double t = fabs(Q[j * n + k]) / nu[j];
-                t = (1.0 + t) * (1.0 - t);
-                t = t < 0.0 ? 0.0 : t;
-                double r = nu[j] / nd[j];
-                double t2 = t * (r * r);
-                if (t2 <= dthr) {
-                    double s2 = 0.0;
-                    for (int i = k + 1; i < n; i++) s2 += Q[j * n + i] * Q[j * n + i];
-                    nd[j] = sqrt(s2);
-                    nu[j] = nd[j];
-                } else {
-                    nu[j] *= sqrt(t);
-                }
-            }
-        }
-    }
-    for (int i = 0; i < n; i++) perm[i] = i;
-    for (int k = 0; k < n; k++) { int t = perm[k]; perm[k] = perm[tr[k]]; perm[tr[k]] = t; }
-    for (int i = 0; i < n; i++) x[i] = 0.0;
-    if (nonzero == 0) return;
-    for (int i = 0; i < n; i++) c[i] = S.rhs[i];
-    for (int k = 0; k < nonzero; k++) {
-        int m = n - k;
-        double tau = hc[k];
-        if (m == 1) { c[k] *= (1.0 - tau); continue; }
-        if (tau == 0.0) continue;
-        double t = 0.0;
-        for (int i = 1; i < m; i++) t += Q[k * n + k + i] * c[k + i];
-        t += c[k];
-        c[k] -= tau * t;
-        for (int i = 1; i < m; i++) c[k + i] -= tau * Q[k * n + k + i] * t;
-    }
-    for (int i = nonzero - 1; i >= 0; i--) {
-        double sacc = c[i];
-        for (int j = i + 1; j < nonzero; j++) sacc -= Q[j * n + i] * c[j];
-        c[i] = sacc / Q[i * n + i];
-    }
-    for (int i = 0; i < nonzero; i++) x[perm[i]] = c[i];
 }
 
 // ------------------------------------------------------------- evaluation
@@ -760,40 +653,203 @@ __device__ bool evaluate(const SearchArgs &a, PtShared &S, int ndof, bool need_j
     return true;
 }
 
-// Normal equations J^T J and -J^T e of the objective's residual from the moments in V.
-__device__ void assemble(const SearchArgs &a, PtShared &S, const double *V, int nd) {
+// ------------------------------------------------------------- step solve
+
+// Cross-lane moves of one wavefront.  rdlane: value of lane l (l uniform);
+// shfl_from: value of lane src (src per lane).
+#ifdef DVC_EMU
+__device__ __forceinline__ double rdlane(double v, int l) { return emu::shfl_idx(v, l); }
+__device__ __forceinline__ double shfl_from(double v, int src) { return emu::shfl_idx(v, src); }
+__device__ __forceinline__ int shfl_from(int v, int src) { return (int)emu::shfl_idx((double)v, src); }
+#else
+__device__ __forceinline__ double rdlane(double v, int l) {
+    long long b = __double_as_longlong(v);
+    int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
+    int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+__device__ __forceinline__ double shfl_from(double v, int src) { return __shfl(v, src, 64); }
+__device__ __forceinline__ int shfl_from(int v, int src) { return __shfl(v, src, 64); }
+#endif
+
+// One Gauss-Newton step on wavefront 0 (all 64 lanes call it): the normal equations
+// J^T J dx = -J^T e of the objective's residual, assembled from the moments in V
+// (lane j < N builds column j of J^T J, lane N builds -J^T e), then solved by
+// column-pivoted Householder QR (Eigen 3.4 ColPivHouseholderQR::computeInPlace and
+// _solve_impl, as used at Search.cpp:234).  Lane j keeps column j in registers; the
+// pivot scan, the reflector and the back substitution are uniform across lanes, and
+// every column is updated by its own lane with the same operation order as a serial
+// loop.  Adds the step to S.par and stores it in S.upd (Search.cpp:236-240).
+template <int N>
+__device__ void gn_step(const SearchArgs &a, PtShared &S, const double *V) {
+#pragma clang fp contract(off)
+    const int lane = threadIdx.x & 63;
     const double n = (double)a.nsamp;
-    double Sg[12], Sgtc[12], Sgrc[12], c[12];
-    for (int p = 0; p < nd; p++) { Sg[p] = V[p * 16 + 14]; Sgtc[p] = V[p * 16 + 12]; Sgrc[p] = V[p * 16 + 13]; }
     const int kind = a.obj;
-    if (kind == DVC_OBJ_SAD || kind == DVC_OBJ_SSD) {
-        for (int p = 0; p < nd; p++) {
-            for (int q = 0; q < nd; q++) S.JTJ[p * nd + q] = V[p * 16 + q];
-            S.rhs[p] = -(Sgtc[p] - Sgrc[p]);
-        }
-    } else if (kind == DVC_OBJ_ZSSD) {
-        double tm = S.tmean, Src = V[13 * 16 + 14];
-        for (int p = 0; p < nd; p++) {
-            for (int q = 0; q < nd; q++) S.JTJ[p * nd + q] = V[p * 16 + q] - Sg[p] * Sg[q] / n;
-            S.rhs[p] = -((Sgtc[p] - tm * Sg[p]) - Sgrc[p] + (Sg[p] / n) * Src);
-        }
-    } else if (kind == DVC_OBJ_NSSD) {
-        double T2 = S.T2, T = sqrt(T2), Rn = sqrt(S.Rn2), rm = S.rmean;
-        for (int p = 0; p < nd; p++) c[p] = Sgtc[p] + rm * Sg[p];
-        for (int p = 0; p < nd; p++) {
-            for (int q = 0; q < nd; q++) S.JTJ[p * nd + q] = V[p * 16 + q] / T2 - c[p] * c[q] / (T2 * T2);
-            double Sgr = Sgrc[p] + rm * Sg[p];
-            S.rhs[p] = -(-Sgr / (T * Rn) + c[p] * S.Strr / (T2 * T * Rn));
-        }
-    } else {
-        double tm = S.tmean, N2 = S.N2, N = sqrt(N2), R = sqrt(S.R2);
-        for (int p = 0; p < nd; p++) c[p] = Sgtc[p] - tm * Sg[p];
-        for (int p = 0; p < nd; p++) {
-            for (int q = 0; q < nd; q++)
-                S.JTJ[p * nd + q] = (V[p * 16 + q] - Sg[p] * Sg[q] / n) / N2 - c[p] * c[q] / (N2 * N2);
-            S.rhs[p] = -(-Sgrc[p] / (N * R) + c[p] * S.Str / (N2 * N * R));
+    const int j = lane < N ? lane : 0;      // column of this lane (lane N: right-hand side)
+    double col[N];
+    {
+        const double Sgj = V[j * 16 + 14];
+        if (kind == DVC_OBJ_SAD || kind == DVC_OBJ_SSD) {
+#pragma unroll
+            for (int i = 0; i < N; i++) col[i] = V[i * 16 + j];
+            if (lane == N)
+#pragma unroll
+                for (int i = 0; i < N; i++) col[i] = -(V[i * 16 + 12] - V[i * 16 + 13]);
+        } else if (kind == DVC_OBJ_ZSSD) {
+            const double tm = S.tmean, Src = V[13 * 16 + 14];
+#pragma unroll
+            for (int i = 0; i < N; i++) col[i] = V[i * 16 + j] - V[i * 16 + 14] * Sgj / n;
+            if (lane == N)
+#pragma unroll
+                for (int i = 0; i < N; i++) {
+                    const double Sg = V[i * 16 + 14];
+                    col[i] = -((V[i * 16 + 12] - tm * Sg) - V[i * 16 + 13] + (Sg / n) * Src);
+                }
+        } else if (kind == DVC_OBJ_NSSD) {
+            const double T2 = S.T2, T = sqrt(T2), Rn = sqrt(S.Rn2), rm = S.rmean;
+            const double cj = V[j * 16 + 12] + rm * Sgj;
+#pragma unroll
+            for (int i = 0; i < N; i++) {
+                const double ci = V[i * 16 + 12] + rm * V[i * 16 + 14];
+                col[i] = V[i * 16 + j] / T2 - ci * cj / (T2 * T2);
+            }
+            if (lane == N)
+#pragma unroll
+                for (int i = 0; i < N; i++) {
+                    const double Sg = V[i * 16 + 14];
+                    const double ci = V[i * 16 + 12] + rm * Sg;
+                    const double Sgr = V[i * 16 + 13] + rm * Sg;
+                    col[i] = -(-Sgr / (T * Rn) + ci * S.Strr / (T2 * T * Rn));
+                }
+        } else {
+            const double tm = S.tmean, N2 = S.N2, Nn = sqrt(N2), R = sqrt(S.R2);
+            const double cj = V[j * 16 + 12] - tm * Sgj;
+#pragma unroll
+            for (int i = 0; i < N; i++) {
+                const double ci = V[i * 16 + 12] - tm * V[i * 16 + 14];
+                col[i] = (V[i * 16 + j] - V[i * 16 + 14] * Sgj / n) / N2 - ci * cj / (N2 * N2);
+            }
+            if (lane == N)
+#pragma unroll
+                for (int i = 0; i < N; i++) {
+                    const double ci = V[i * 16 + 12] - tm * V[i * 16 + 14];
+                    col[i] = -(-V[i * 16 + 13] / (Nn * R) + ci * S.Str / (N2 * Nn * R));
+                }
         }
     }
+    // column norms (exact and updated), the permutation carried with the columns
+    double cn = 0.0;
+#pragma unroll
+    for (int i = 0; i < N; i++) cn += col[i] * col[i];
+    double nrm_d = sqrt(cn), nrm_u = nrm_d;
+    int orig = lane;
+    double maxn = 0.0;
+#pragma unroll
+    for (int k = 0; k < N; k++) maxn = fmax(maxn, rdlane(nrm_u, k));
+    const double eps = 2.220446049250313e-16;
+    const double thr = (maxn * eps) * (maxn * eps) / (double)N;
+    const double dthr = sqrt(eps);
+    int nonzero = N;
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+        int big = k;
+        double bn = rdlane(nrm_u, k);
+#pragma unroll
+        for (int q = k + 1; q < N; q++) {
+            double v = rdlane(nrm_u, q);
+            if (v > bn) { bn = v; big = q; }
+        }
+        if (nonzero == N && bn * bn < thr * (double)(N - k)) nonzero = k;
+        if (big != k) {
+            const int src = (lane == k) ? big : ((lane == big) ? k : lane);
+#pragma unroll
+            for (int i = 0; i < N; i++) col[i] = shfl_from(col[i], src);
+            nrm_u = shfl_from(nrm_u, src);
+            nrm_d = shfl_from(nrm_d, src);
+            orig = shfl_from(orig, src);
+        }
+        // makeHouseholder of pivot column k, rows k..N-1 (uniform on every lane)
+        const int m = N - k;
+        double ess[N];
+        double tail = 0.0;
+#pragma unroll
+        for (int i = 1; i < m; i++) { ess[i] = rdlane(col[k + i], k); tail += ess[i] * ess[i]; }
+        const double c0 = rdlane(col[k], k);
+        double tau, beta;
+        if (tail <= 2.2250738585072014e-308) {
+            tau = 0.0; beta = c0;
+#pragma unroll
+            for (int i = 1; i < m; i++) ess[i] = 0.0;
+        } else {
+            beta = sqrt(c0 * c0 + tail);
+            if (c0 >= 0.0) beta = -beta;
+#pragma unroll
+            for (int i = 1; i < m; i++) ess[i] = ess[i] / (c0 - beta);
+            tau = (beta - c0) / beta;
+        }
+        if (lane == k) {
+            col[k] = beta;
+#pragma unroll
+            for (int i = 1; i < m; i++) col[k + i] = ess[i];
+        }
+        // apply the reflector to the later columns and, while k < nonzero, to the
+        // right-hand side
+        const bool later = lane > k && lane < N;
+        const bool rhs = lane == N && k < nonzero;
+        if (later || rhs) {
+            if (m > 1 && tau != 0.0) {
+                double t = 0.0;
+#pragma unroll
+                for (int i = 1; i < m; i++) t += ess[i] * col[k + i];
+                t += col[k];
+                col[k] -= tau * t;
+#pragma unroll
+                for (int i = 1; i < m; i++) col[k + i] -= tau * ess[i] * t;
+            } else if (m == 1) {
+                col[k] *= (1.0 - tau);
+            }
+        }
+        // norm downdate of the later columns
+        if (later && nrm_u != 0.0) {
+            double t = fabs(col[k]) / nrm_u;
+            t = (1.0 + t) * (1.0 - t);
+            t = t < 0.0 ? 0.0 : t;
+            const double r = nrm_u / nrm_d;
+            const double t2 = t * (r * r);
+            if (t2 <= dthr) {
+                double s2 = 0.0;
+#pragma unroll
+                for (int i = k + 1; i < N; i++) s2 += col[i] * col[i];
+                nrm_d = sqrt(s2);
+                nrm_u = nrm_d;
+            } else {
+                nrm_u *= sqrt(t);
+            }
+        }
+    }
+    // back substitution with R (upper triangle of the columns), uniform on every lane
+    double c[N];
+#pragma unroll
+    for (int i = 0; i < N; i++) c[i] = rdlane(col[i], N);
+#pragma unroll
+    for (int i = N - 1; i >= 0; i--) {
+        if (i < nonzero) {
+            double sacc = c[i];
+#pragma unroll
+            for (int q = i + 1; q < N; q++)
+                if (q < nonzero) sacc -= rdlane(col[i], q) * c[q];
+            c[i] = sacc / rdlane(col[i], i);
+        }
+    }
+    // x[perm[i]] = c[i] for i < nonzero, 0 otherwise; the step is added to the parameters
+#pragma unroll
+    for (int i = 0; i < N; i++)
+        if (lane == i) {
+            const double x = (i < nonzero) ? c[i] : 0.0;
+            S.upd[orig] = x;
+            S.par[orig] += x;
+        }
 }
 
 // ------------------------------------------------------------- the search
@@ -953,18 +1009,20 @@ __device__ void point_search(const SearchArgs &a, int p, int n, PtShared &S, flo
             if (tid == 0) S.status = DVC_RANGE_FAIL;
             return;
         }
-        if (tid == 0) {
-            S.stop = 0;
+        if (tid < 64) {
+            // stopping test on the previous step (Search.cpp:217-230), uniform on wave 0
+            bool stop = false;
             if (i > 0) {
                 double del_obj = fabs(S.obj - obj_old);
                 double del_mag = sqrt(S.upd[0] * S.upd[0] + S.upd[1] * S.upd[1] + S.upd[2] * S.upd[2]);
-                if ((del_obj < 0.000001) || (del_mag < 0.01)) S.stop = 1;
+                if ((del_obj < 0.000001) || (del_mag < 0.01)) stop = true;
             }
-            if (!S.stop) {
-                assemble(a, S, V, nd);
-                colpiv_solve(S, nd);
-                for (int q = 0; q < nd; q++) { S.upd[q] = S.qx[q]; S.par[q] += S.qx[q]; }
+            if (!stop) {
+                if (nd == 3) gn_step<3>(a, S, V);
+                else if (nd == 6) gn_step<6>(a, S, V);
+                else gn_step<12>(a, S, V);
             }
+            if (tid == 0) S.stop = stop ? 1 : 0;
         }
         __syncthreads();
         if (S.stop) { converged = true; break; }

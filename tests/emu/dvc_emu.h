@@ -67,6 +67,16 @@ inline double shfl_xor(double v, int mask) {
     return r;
 }
 
+// value of lane `src` of this wave (src may differ per lane)
+inline double shfl_idx(double v, int src) {
+    unsigned t = thread_idx.x, w = t / 64, l = t % 64;
+    cur->xd[w * 64 + l] = v;
+    wave_sync();
+    double r = cur->xd[w * 64 + ((unsigned)src & 63u)];
+    wave_sync();
+    return r;
+}
+
 typedef float f4 __attribute__((vector_size(16)));
 inline f4 mfma16x16x4(float a, float b, f4 c) {
     unsigned t = thread_idx.x, w = t / 64, l = t % 64;

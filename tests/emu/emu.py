@@ -52,6 +52,19 @@ def search(params, ref, tar, points, order, neigh, nproc=0, threads=64):
     return out
 
 
+def gn_solve(A, b):
+    """The search kernel's Gauss-Newton solve (gn_step) of A x = b on one emulated wave."""
+    lib = load()
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros(len(b))
+    vp = lambda a: a.ctypes.data_as(C.c_void_p)
+    lib.emu_gn_solve.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+    if lib.emu_gn_solve(len(b), vp(A), vp(b), vp(x)) != 0:
+        raise ValueError("n must be 3, 6 or 12")
+    return x
+
+
 def strain(pts, disp, status, objmin, neigh, ndp=25, objt=1.0, refill=False, planar=(0, 0, 0)):
     lib = load()
     pts = np.ascontiguousarray(pts, dtype=np.float64)

@@ -64,6 +64,35 @@ extern "C" int emu_search(const dvc_run_params *rc, const void *ref, const void 
     return 0;
 }
 
+// gn_step<N> alone on one wave: with the SSD moments V[i][j] = A[i][j], V[i][12] = -b[i],
+// V[i][13] = 0 it solves A x = b by the column-pivoted QR of the search kernel.
+extern "C" int emu_gn_solve(int n, const double *A, const double *b, double *x) {
+    if (n != 3 && n != 6 && n != 12) return -1;
+    SearchArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.obj = DVC_OBJ_SSD;
+    a.nsamp = 1;
+    std::vector<double> V(256, 0.0);
+    for (int i = 0; i < n; i++) {
+        for (int j = 0; j < n; j++) V[i * 16 + j] = A[i * n + j];
+        V[i * 16 + 12] = -b[i];
+    }
+    emu::grid_dim.x = 1;
+    emu::run_grid(0, 1, 64, 0, [&] {
+        DVC_SHARED(PtShared, S);
+        if (threadIdx.x == 0)
+            for (int q = 0; q < 12; q++) { S.par[q] = 0.0; S.upd[q] = 0.0; }
+        emu::wave_sync();
+        if (n == 3) gn_step<3>(a, S, V.data());
+        else if (n == 6) gn_step<6>(a, S, V.data());
+        else gn_step<12>(a, S, V.data());
+        emu::wave_sync();
+        if (threadIdx.x == 0)
+            for (int q = 0; q < n; q++) x[q] = S.par[q];
+    });
+    return 0;
+}
+
 extern "C" int emu_strain(const double *pts, const double *disp, const int *status, const double *objmin,
                           const int *neigh, int k, int n, int ndp, double objt, int refill, int yz, int zx, int xy,
                           double *out_disp, int *out_npts, double *out_swr, double *out_E, double *out_L,

@@ -45,9 +45,40 @@ def test_affine_sphere_u16_12dof(oracle):
     assert np.all(o["status"] == 0)
 
 
-def test_ssd_6dof_and_partial(oracle):
+@pytest.mark.parametrize("obj", ["ssd", "zssd", "nssd", "sad"])
+def test_6dof_objectives_and_partial(oracle, obj):
     case = synth.config2(size=72, n=2, spacing=10, npts=600, diameter=21)
-    _cmp(oracle, case, dict(case.params, disp_max=5, ndof=6, obj="ssd"), nproc=5)
+    _cmp(oracle, case, dict(case.params, disp_max=5, ndof=6, obj=obj), nproc=5)
+
+
+@pytest.mark.parametrize("n", [3, 6, 12])
+def test_gn_solve_matches_oracle_bitwise(oracle, n):
+    """The wave-parallel column-pivoted QR of the search kernel against the oracle's
+    serial one (Eigen 3.4 ColPivHouseholderQR, Search.cpp:234): identical bits on
+    well-conditioned, symmetric positive definite, rank-deficient (duplicate and zero
+    columns), pivot-tie and all-zero systems."""
+    rng = np.random.default_rng(40 + n)
+    cases = []
+    A = rng.normal(size=(n, n))
+    cases.append(A)
+    J = rng.normal(size=(50, n)) * np.logspace(0, 4, n)
+    cases.append(J.T @ J)
+    B = rng.normal(size=(n, n))
+    B[:, n // 2] = B[:, 0]
+    cases.append(B)
+    Z = rng.normal(size=(n, n))
+    Z[:, -1] = 0.0
+    Z[-1, :] = 0.0
+    cases.append(Z)
+    cases.append(np.eye(n) * 2.0)                   # every column norm ties
+    cases.append(np.zeros((n, n)))
+    for A in cases:
+        b = rng.normal(size=n)
+        want = oracle.colpiv_solve(A, b)
+        got = emu.gn_solve(A, b)
+        assert np.array_equal(np.isnan(want), np.isnan(got))
+        ok = ~np.isnan(want)
+        assert np.array_equal(got[ok], want[ok]), (A, got, want)
 
 
 def test_strain_kernel(oracle):
