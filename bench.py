@@ -43,6 +43,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--bytes-sample", type=int, default=400, help="points sampled for the algorithmic byte count")
     ap.add_argument("--cache", default="", help="directory to keep the generated volumes in between runs")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="only run a plain device-to-device copy of this many bytes (3 times), the known-size "
+                         "kernel that scales the HBM byte counters of a profiler run")
     return ap.parse_args()
 
 
@@ -72,6 +75,16 @@ def main():
         print(f"bench.py measures one GPU only: the search runs on a single device (asked for {a.gpus}); "
               "run it without --gpus or with --gpus 1", file=sys.stderr)
         sys.exit(2)
+
+    if a.copy_bytes:
+        import torch
+        src = torch.ones(a.copy_bytes, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": a.copy_bytes, "launches": 3}), flush=True)
+        return
 
     import digitalvolumecorrelation_amd as D
     from digitalvolumecorrelation_amd import synth, workload

@@ -6,11 +6,14 @@ device.  There is no CPU fallback -- a missing library or device raises.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from pathlib import Path
 
 import numpy as np
 
-LIB_PATH = Path(__file__).resolve().parent / "libdvc_amd.so"
+# DVC_AMD_LIB selects another build of the library (same ABI), e.g. one compiled with
+# different tuning macros.
+LIB_PATH = Path(os.environ.get("DVC_AMD_LIB", Path(__file__).resolve().parent / "libdvc_amd.so"))
 
 POINT_GOOD, RANGE_FAIL, CONVG_FAIL, NOT_SEARCH = 0, -1, -2, -3
 OBJ = {"sad": 0, "ssd": 1, "zssd": 2, "nssd": 3, "znssd": 4}
@@ -63,6 +66,7 @@ def load() -> C.CDLL:
         "dvc_volume_upload": (I, [VP, C.c_int, VP, C.c_int64, C.c_int, C.c_int, C.c_int64, C.c_int64, C.c_int64]),
         "dvc_volume_attach": (I, [VP, C.c_int, VP, C.c_int, C.c_int64, C.c_int64, C.c_int64]),
         "dvc_interpolate": (I, [VP, C.c_int, P(RunParams), P(D), P(D), I, P(D), C.c_int, P(D), P(D)]),
+        "dvc_interpolate_box": (I, [VP, C.c_int, P(D), P(D), P(D), P(D), I, P(D), C.c_int, P(D), P(D)]),
         "dvc_objective": (I, [VP, C.c_int, P(D), P(D), I, P(D), P(D)]),
         "dvc_search_points": (I, [VP, P(RunParams), P(D), I, P(I), I, P(I), I, VP]),
         "dvc_search_prepare": (I, [VP, P(RunParams), P(D), I, P(I), I, P(I), I]),

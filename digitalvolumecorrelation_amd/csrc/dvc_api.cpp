@@ -237,20 +237,18 @@ int dvc_volume_attach(dvc_ctx *ctx, int slot, const void *dptr, int bpv, int64_t
     return DVC_OK;
 }
 
-int dvc_interpolate(dvc_ctx *ctx, int slot, const dvc_run_params *rc, const double center[3], const double *xyz,
-                    int32_t n, const double bbox[6], int method, double *out, double *grad) {
-    if (!ctx) return DVC_ERR_ARG;
-    if (int e = check_params(ctx, rc)) return e;
+// Interpolate::center_on + kernels + tri_cub_Lek / tri_lin / nearest on the boxes of an
+// Interpolate object (est box, act box = est box less one voxel, kern_4d extent box[3]).
+static int interp_boxes(dvc_ctx *ctx, int slot, HBox est, HBox act, const int box[3], const double center[3],
+                        const double *xyz, int32_t n, const double bbox[6], int method, double *out, double *grad) {
     if (slot != 0 && slot != 1) return fail(ctx, DVC_ERR_ARG, "bad slot");
     const Volume &v = ctx->vol[slot];
     if (!v.d) return fail(ctx, DVC_ERR_STATE, "volume not uploaded");
     if (n < 0 || (n > 0 && (!xyz || !out || !bbox || !center))) return fail(ctx, DVC_ERR_ARG, "bad arrays");
     if (method < 0 || method > 2) return fail(ctx, DVC_ERR_ARG, "bad interpolation method");
     if (grad && method != DVC_INTERP_TRICUBIC) return fail(ctx, DVC_ERR_ARG, "gradients are tricubic only");
-    Geometry G = make_geometry(*rc);
-    HBox est(G.est_mn, G.est_mx);
-    HBox act(G.act_mn, G.act_mx);
-    for (int q = 0; q < 3; q++) { act.cn[q] = G.act_cn[q]; est.cn[q] = G.est_cn[q]; }
+    for (int q = 0; q < 3; q++)
+        if (box[q] < 4) return fail(ctx, DVC_ERR_ARG, "interpolation box smaller than 4 voxels");
     est.center_on(center);
     act.center_on(center);
     int b0[3];
@@ -258,7 +256,7 @@ int dvc_interpolate(dvc_ctx *ctx, int slot, const dvc_run_params *rc, const doub
     int64_t dims[3] = {v.W, v.H, v.T};
     for (int q = 0; q < 3; q++) {
         b0[q] = trunc_i(est.mn[q]);
-        if (!(est.mn[q] == est.mn[q]) || b0[q] < 0 || (int64_t)b0[q] + G.box[q] > dims[q]) ok = false;
+        if (!(est.mn[q] == est.mn[q]) || b0[q] < 0 || (int64_t)b0[q] + box[q] > dims[q]) ok = false;
         if (!(bbox[q] >= act.mn[q]) || !(bbox[3 + q] <= act.mx[q])) ok = false;   // BoundBox::contains
     }
     int lim = method == DVC_INTERP_NEAREST ? 1 : 2;
@@ -268,7 +266,7 @@ int dvc_interpolate(dvc_ctx *ctx, int slot, const dvc_run_params *rc, const doub
                 double c = xyz[3 * i + q];
                 if (!(c == c) || std::fabs(c) > 1e15) { ok = false; break; }
                 int cell = trunc_i(c) - b0[q];
-                if (cell < 0 || cell > G.box[q] - lim) { ok = false; break; }
+                if (cell < 0 || cell > box[q] - lim) { ok = false; break; }
             }
     if (!ok) return fail(ctx, DVC_ERR_RANGE, "points outside the active interpolation box");
     if (n == 0) return DVC_OK;
@@ -278,7 +276,8 @@ int dvc_interpolate(dvc_ctx *ctx, int slot, const dvc_run_params *rc, const doub
     HIPCHK(hipMalloc((void **)&d_out, sizeof(double) * n));
     if (grad) HIPCHK(hipMalloc((void **)&d_grad, sizeof(double) * 3 * n));
     HIPCHK(hipMemcpyAsync(d_xyz, xyz, sizeof(double) * 3 * n, hipMemcpyHostToDevice, ctx->stream));
-    int r = launch_interp(v.d, v.bpv, (int)v.W, (int)v.H, (int)v.T, b0, G.box, d_xyz, n, method, d_out, d_grad, ctx->stream);
+    int r = launch_interp(v.d, v.bpv, (int)v.W, (int)v.H, (int)v.T, b0, box, d_xyz, n, method, d_out, d_grad,
+                          ctx->stream);
     if (r == 0) {
         hipMemcpyAsync(out, d_out, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream);
         if (grad) hipMemcpyAsync(grad, d_grad, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, ctx->stream);
@@ -288,6 +287,35 @@ int dvc_interpolate(dvc_ctx *ctx, int slot, const dvc_run_params *rc, const doub
     if (r != 0) return fail(ctx, DVC_ERR_HIP, "interpolation launch failed");
     if (e != hipSuccess) return hip_fail(ctx, e, "interpolation");
     return DVC_OK;
+}
+
+int dvc_interpolate(dvc_ctx *ctx, int slot, const dvc_run_params *rc, const double center[3], const double *xyz,
+                    int32_t n, const double bbox[6], int method, double *out, double *grad) {
+    if (!ctx) return DVC_ERR_ARG;
+    if (int e = check_params(ctx, rc)) return e;
+    Geometry G = make_geometry(*rc);
+    HBox est(G.est_mn, G.est_mx);
+    HBox act(G.act_mn, G.act_mx);
+    for (int q = 0; q < 3; q++) { act.cn[q] = G.act_cn[q]; est.cn[q] = G.est_cn[q]; }
+    return interp_boxes(ctx, slot, est, act, G.box, center, xyz, n, bbox, method, out, grad);
+}
+
+int dvc_interpolate_box(dvc_ctx *ctx, int slot, const double region_min[3], const double region_max[3],
+                        const double center[3], const double *xyz, int32_t n, const double bbox[6], int method,
+                        double *out, double *grad) {
+    if (!ctx) return DVC_ERR_ARG;
+    if (!region_min || !region_max) return fail(ctx, DVC_ERR_ARG, "bad region");
+    for (int q = 0; q < 3; q++)
+        if (!std::isfinite(region_min[q]) || !std::isfinite(region_max[q]) || region_max[q] < region_min[q] ||
+            std::fabs(region_min[q]) > 1e9 || std::fabs(region_max[q]) > 1e9)
+            return fail(ctx, DVC_ERR_ARG, "bad region");
+    // Interpolate::Interpolate (Interpolate.cpp:34-58)
+    HBox est(region_min, region_max);
+    HBox act(region_min, region_max);
+    act.grow_by(-1.0);
+    int box[3];
+    for (int q = 0; q < 3; q++) box[q] = trunc_i(est.mx[q]) - trunc_i(est.mn[q]);
+    return interp_boxes(ctx, slot, est, act, box, center, xyz, n, bbox, method, out, grad);
 }
 
 int dvc_objective(dvc_ctx *ctx, int kind, const double *ref, const double *tar, int32_t n, double *obj, double *res) {
@@ -331,15 +359,28 @@ int dvc_search_prepare(dvc_ctx *ctx, const dvc_run_params *rc, const double *poi
     for (int i = 0; i < npts * 3; i++)
         if (!std::isfinite(points[i])) return fail(ctx, DVC_ERR_ARG, "non-finite point coordinate");
     Geometry G = make_geometry(*rc);
-    int bt = ctx->block_threads ? ctx->block_threads : 256;
-    int nsamp_pad = (G.subv_num + 3) & ~3;
-    size_t lds = search_lds_bytes(nsamp_pad, bt);
-    if (lds + 8192 > ctx->lds_max) return fail(ctx, DVC_ERR_ARG, "too many samples per point for one workgroup's LDS");
     HIPCHK(hipSetDevice(ctx->device));
+    // threads per point: the requested count, else the largest of 512/256/128/64 whose
+    // LDS (sample values, MFMA staging, point state) fits one workgroup
+    const int bpv = ctx->vol[0].bpv;
+    int nsamp_pad = (G.subv_num + 3) & ~3;
+    const int auto_cands[4] = {512, 256, 128, 64};
+    const int *cands = ctx->block_threads ? &ctx->block_threads : auto_cands;
+    const int ncand = ctx->block_threads ? 1 : 4;
+    int bt = 0;
+    for (int i = 0; i < ncand && bt == 0; i++) {
+        size_t st = 0;
+        if (search_static_lds(bpv, cands[i], &st) != 0) return fail(ctx, DVC_ERR_HIP, "kernel attributes");
+        if (search_lds_bytes(nsamp_pad, cands[i]) + st <= ctx->lds_max) bt = cands[i];
+    }
+    if (bt == 0) return fail(ctx, DVC_ERR_ARG, "too many samples per point for one workgroup's LDS");
     // sphere template: identical offsets for every point (one seed, Search.cpp:102-105)
     std::vector<double> sph;
     double omn[3] = {0, 0, 0}, omx[3] = {0, 0, 0};
-    if (rc->geom == DVC_GEOM_SPHERE) sphere_template(*rc, G, sph, omn, omx);
+    if (rc->geom == DVC_GEOM_SPHERE) {
+        sphere_template(*rc, G, sph, omn, omx);
+        locality_order(sph);
+    }
     if (int e = dev_alloc(ctx, &ctx->d_pts, (size_t)3 * npts)) return e;
     if (int e = dev_alloc(ctx, &ctx->d_sph, sph.size())) return e;
     if (int e = dev_alloc(ctx, &ctx->d_order, (size_t)npts)) return e;
@@ -378,6 +419,7 @@ static int enqueue_search(dvc_ctx *ctx, hipStream_t st) {
     fill_search_args(a, rc, G, ctx->vol[0].bpv, (int)ctx->vol[0].W, (int)ctx->vol[0].H, (int)ctx->vol[0].T,
                      ctx->npts, ctx->nproc, ctx->k, ctx->off_min, ctx->off_max);
     a.vol[0] = ctx->vol[0].d; a.vol[1] = ctx->vol[1].d;
+    if (((uintptr_t)a.vol[0] | (uintptr_t)a.vol[1]) & 3) a.gather32 = 0;   // attached, unaligned
     a.sph = ctx->d_sph; a.pts = ctx->d_pts; a.order = ctx->d_order; a.pos = ctx->d_pos; a.neigh = ctx->d_neigh;
     a.res = ctx->d_res;
     int bt = ctx->chosen_threads;

@@ -8,8 +8,10 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <random>
+#include <utility>
 #include <vector>
 
 #include "dvc_internal.h"
@@ -94,6 +96,37 @@ inline void sphere_template(const dvc_run_params &rc, const Geometry &G, std::ve
         }
 }
 
+// Reorder sample offsets by their unit cells, z slowest and x fastest.  A sample's
+// value does not depend on its position in the cloud, only the order of the objective's
+// sums does.  In cell order the 64 lanes of a wavefront fall on a few rows (y, z) of
+// the volume, so each of a sample's 16 row loads touches a few cache lines for the
+// whole wavefront instead of one line per lane; the L1 tag lookups of these gathers are
+// what bounds the sample pass.  Stable: samples of one cell keep their relative order.
+inline void locality_order(std::vector<double> &sph) {
+    const size_t n = sph.size() / 3;
+    if (n == 0) return;
+    double lo[3] = {1e300, 1e300, 1e300};
+    for (size_t i = 0; i < n; i++)
+        for (int q = 0; q < 3; q++) lo[q] = std::min(lo[q], sph[3 * i + q]);
+    std::vector<std::pair<uint64_t, size_t>> key(n);
+    for (size_t i = 0; i < n; i++) {
+        uint64_t c[3];
+        for (int q = 0; q < 3; q++) {
+            double f = std::floor(sph[3 * i + q] - lo[q]);
+            c[q] = (uint64_t)std::min(std::max(f, 0.0), 2097151.0);
+        }
+        key[i] = {(c[2] << 42) | (c[1] << 21) | c[0], i};
+    }
+    std::stable_sort(key.begin(), key.end(),
+                     [](const std::pair<uint64_t, size_t> &a, const std::pair<uint64_t, size_t> &b) {
+                         return a.first < b.first;
+                     });
+    std::vector<double> out(sph.size());
+    for (size_t i = 0; i < n; i++)
+        for (int q = 0; q < 3; q++) out[3 * i + q] = sph[3 * key[i].second + q];
+    sph.swap(out);
+}
+
 // Kernel arguments of one search (pointers filled by the caller).
 inline void fill_search_args(SearchArgs &a, const dvc_run_params &rc, const Geometry &G, int bpv, int W, int H,
                              int T, int npts, int nproc, int k, const double omn[3], const double omx[3]) {
@@ -111,6 +144,9 @@ inline void fill_search_args(SearchArgs &a, const dvc_run_params &rc, const Geom
         a.off_min[q] = omn[q]; a.off_max[q] = omx[q];
     }
     a.subv_rad = G.subv_rad; a.basin = rc.basin_radius; a.disp_max = (double)rc.disp_max;
+    // 32-bit row offsets: whole-dword row strides and the box's slices within 2 GiB
+    const long long wb = (long long)W * bpv, whb = wb * (long long)H;
+    a.gather32 = (wb % 4 == 0) && ((long long)(G.box[2] + 1) * whb + 64 < (1ll << 31)) ? 1 : 0;
 }
 
 // Position of every point in order[] (npts when it is never searched, dvc.cpp:229-232).

@@ -18,6 +18,7 @@ struct SearchArgs {
     int nsamp, g;                    // samples per point; cube grid side
     int npts, nproc, k, nsamp_pad;
     int box[3];                      // interpolation box size in voxels (kern_4d extent)
+    int gather32;                    // box rows addressable by 32-bit offsets (see Gather)
     double est_mn[3], est_mx[3], est_cn[3];   // est_box_nom after the two grow_by calls
     double act_mn[3], act_mx[3], act_cn[3];   // act_box = est_box shrunk by one voxel
     double subv_rad, aspect[3], rigid[3], basin, disp_max;
@@ -44,6 +45,7 @@ struct StrainArgs {
 int launch_search(const SearchArgs &a, const int *level_pts, int block_threads, int grid_blocks, size_t lds_bytes,
                   void *stream);
 int search_set_lds_limit(size_t lds_total);
+int search_static_lds(int bpv, int block_threads, size_t *bytes);
 int search_max_blocks(int bpv, int block_threads, size_t lds_bytes, int *blocks_per_cu);
 size_t search_lds_bytes(int nsamp_pad, int block_threads);
 int launch_interp(const uint8_t *vol, int bpv, int W, int H, int T, const int b0[3], const int box[3],

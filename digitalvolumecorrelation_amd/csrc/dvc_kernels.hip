@@ -47,8 +47,18 @@ namespace dvc {
 
 #ifdef DVC_EMU
 typedef emu::f4 floatx4;
+#define DVC_GLOBAL
+__device__ __forceinline__ uint64_t uniform_u64(uint64_t v) { return v; }
 #else
 typedef float floatx4 __attribute__((ext_vector_type(4)));
+// global (not flat) address space: loads become global_load with an SGPR base
+#define DVC_GLOBAL __attribute__((address_space(1)))
+// a wave-uniform 64-bit value moved to scalar registers
+__device__ __forceinline__ uint64_t uniform_u64(uint64_t v) {
+    uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
 #endif
 
 // ------------------------------------------------------------------ helpers
@@ -148,6 +158,27 @@ struct BoxView {
 };
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// Row gathers of one point's box.  fast: the rows of the box lie within 2 GiB of `base`
+// (the first voxel of the box's first row and slice, dword aligned) and row strides are
+// whole dwords, so a sample's 16 rows share one 32-bit offset and byte shift and differ
+// only by wave-uniform row bases.  Otherwise rows are addressed with 64-bit arithmetic.
+struct Gather {
+    const DVC_GLOBAL uint8_t *base;
+    uint32_t Wb, WHb;                // row and slice strides in bytes
+    bool fast;
+};
+
+template <int BPV>
+__device__ __forceinline__ Gather make_gather(const VolView &v, const BoxView &bx, bool fast) {
+    Gather g;
+    g.fast = fast;
+    g.Wb = (uint32_t)(v.W * BPV);
+    g.WHb = (uint32_t)(v.WH * BPV);
+    uint64_t b = (uint64_t)(uintptr_t)(v.p + BPV * ((long long)bx.b[2] * v.WH + (long long)bx.b[1] * v.W));
+    g.base = (const DVC_GLOBAL uint8_t *)(uintptr_t)uniform_u64(b);
+    return g;
+}
 
 // Cubic Hermite basis on [0,1]: h[a][q] is the weight of the value (q=0) or slope (q=1)
 // at node a; dh its derivative.
@@ -295,6 +326,135 @@ __device__ __forceinline__ float interp_at(const VolView &v, const BoxView &bx, 
     return val;
 }
 
+// A tricubic sample split into address, load and arithmetic stages, so that a thread
+// can have the 16 row loads of several samples in flight before it needs any of them.
+// The arithmetic is the interior path of interp_at; cells on the box faces take
+// tricubic_faces at evaluation time (their rows are loaded at the nearest interior
+// cell, always inside the box, and not used).
+// Samples per thread whose volume loads are in flight together in the sample passes.
+#ifndef DVC_INFLIGHT
+#define DVC_INFLIGHT 1
+#endif
+constexpr int kInflight = DVC_INFLIGHT;
+
+template <int BPV>
+struct Tap {
+    static constexpr int K = BPV == 1 ? 2 : 3;   // dwords per row window
+    int ix, iy, iz;
+    float tx, ty, tz;
+    bool face;
+    uint32_t shift;                              // 2-bit byte shift of each row window
+    uint32_t raw[16][K];
+};
+
+template <int BPV>
+__device__ __forceinline__ void tap_locate(const BoxView &bx, double x, double y, double z, Tap<BPV> &t) {
+    int ix = (int)x, iy = (int)y, iz = (int)z;
+    float tx = (float)(x - (double)ix), ty = (float)(y - (double)iy), tz = (float)(z - (double)iz);
+    int cx = ix - bx.b[0], cy = iy - bx.b[1], cz = iz - bx.b[2];
+    if (cx < 0 || cx > bx.n[0] - 2 || cy < 0 || cy > bx.n[1] - 2 || cz < 0 || cz > bx.n[2] - 2 ||
+        !(tx >= 0.0f && tx <= 1.0f && ty >= 0.0f && ty <= 1.0f && tz >= 0.0f && tz <= 1.0f)) {
+        cx = clampi(cx, 0, bx.n[0] - 2);
+        cy = clampi(cy, 0, bx.n[1] - 2);
+        cz = clampi(cz, 0, bx.n[2] - 2);
+        tx = fminf(fmaxf(tx, 0.0f), 1.0f);
+        ty = fminf(fmaxf(ty, 0.0f), 1.0f);
+        tz = fminf(fmaxf(tz, 0.0f), 1.0f);
+        if (!(tx == tx)) tx = 0.0f;
+        if (!(ty == ty)) ty = 0.0f;
+        if (!(tz == tz)) tz = 0.0f;
+    }
+    t.ix = bx.b[0] + cx; t.iy = bx.b[1] + cy; t.iz = bx.b[2] + cz;
+    t.tx = tx; t.ty = ty; t.tz = tz;
+    t.face = cx < 1 || cx > bx.n[0] - 3 || cy < 1 || cy > bx.n[1] - 3 || cz < 1 || cz > bx.n[2] - 3;
+}
+
+template <int BPV>
+__device__ __forceinline__ void tap_load(const VolView &v, const Gather &g, const BoxView &bx, Tap<BPV> &t) {
+    // rows of the nearest interior cell: inside the box for every cell of the box
+    const int lx = clampi(t.ix, bx.b[0] + 1, bx.b[0] + bx.n[0] - 3);
+    const int ly = clampi(t.iy, bx.b[1] + 1, bx.b[1] + bx.n[1] - 3);
+    const int lz = clampi(t.iz, bx.b[2] + 1, bx.b[2] + bx.n[2] - 3);
+    if (g.fast) {
+        const uint32_t o = (uint32_t)(lz - 1 - bx.b[2]) * g.WHb + (uint32_t)(ly - 1 - bx.b[1]) * g.Wb +
+                           (uint32_t)(lx - 1) * (uint32_t)BPV;
+        const uint32_t sh = o & 3u, vo = o & ~3u;
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const int ky = r & 3, kz = r >> 2;
+            const DVC_GLOBAL uint8_t *row = g.base + (kz * g.WHb + ky * g.Wb);
+            const DVC_GLOBAL uint32_t *a4 = (const DVC_GLOBAL uint32_t *)(row + vo);
+#pragma unroll
+            for (int q = 0; q < Tap<BPV>::K; q++) t.raw[r][q] = a4[q];
+        }
+        t.shift = sh * 0x55555555u;
+        return;
+    }
+    uint32_t sh = 0;
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const int ky = r & 3, kz = r >> 2;
+        long long i = (long long)(lz - 1 + kz) * v.WH + (long long)(ly - 1 + ky) * v.W + (lx - 1);
+        uintptr_t ad = (uintptr_t)(v.p + BPV * i);
+        const uint32_t *a4 = (const uint32_t *)(ad & ~(uintptr_t)3);
+        sh |= (uint32_t)(ad & 3) << (2 * r);
+#pragma unroll
+        for (int q = 0; q < Tap<BPV>::K; q++) t.raw[r][q] = a4[q];
+    }
+    t.shift = sh;
+}
+
+template <int BPV>
+__device__ __forceinline__ float tap_eval(const VolView &v, const BoxView &bx, const Tap<BPV> &t, float bias,
+                                          bool want_grad, float &gx, float &gy, float &gz) {
+    gx = gy = gz = 0.0f;
+    if (t.face)
+        return tricubic_faces<BPV>(v, bx, t.ix, t.iy, t.iz, t.tx, t.ty, t.tz, bias, want_grad, gx, gy, gz);
+    float wx[4], dx[4], wy[4], dy[4], wz[4], dz[4];
+    cr_weights(t.tx, wx, dx);
+    cr_weights(t.ty, wy, dy);
+    cr_weights(t.tz, wz, dz);
+    float val = 0.0f;
+#pragma unroll
+    for (int kz = 0; kz < 4; kz++) {
+        float pv = 0.0f, px = 0.0f, py = 0.0f;
+#pragma unroll
+        for (int ky = 0; ky < 4; ky++) {
+            const int rr = kz * 4 + ky;
+            const uint32_t s = (t.shift >> (2 * rr)) & 3u;
+            float r[4];
+            if (BPV == 1) {
+                uint32_t w = __builtin_amdgcn_alignbyte(t.raw[rr][1], t.raw[rr][0], s);
+                r[0] = (float)(w & 0xffu) - bias;
+                r[1] = (float)((w >> 8) & 0xffu) - bias;
+                r[2] = (float)((w >> 16) & 0xffu) - bias;
+                r[3] = (float)(w >> 24) - bias;
+            } else {
+                uint32_t w0 = __builtin_amdgcn_alignbyte(t.raw[rr][1], t.raw[rr][0], s);
+                uint32_t w1 = __builtin_amdgcn_alignbyte(t.raw[rr][Tap<BPV>::K - 1], t.raw[rr][1], s);
+                r[0] = (float)(w0 & 0xffffu) - bias;
+                r[1] = (float)(w0 >> 16) - bias;
+                r[2] = (float)(w1 & 0xffffu) - bias;
+                r[3] = (float)(w1 >> 16) - bias;
+            }
+            float rv = r[0] * wx[0] + r[1] * wx[1] + r[2] * wx[2] + r[3] * wx[3];
+            pv += wy[ky] * rv;
+            if (want_grad) {
+                float rd = r[0] * dx[0] + r[1] * dx[1] + r[2] * dx[2] + r[3] * dx[3];
+                px += wy[ky] * rd;
+                py += dy[ky] * rv;
+            }
+        }
+        val += wz[kz] * pv;
+        if (want_grad) {
+            gx += wz[kz] * px;
+            gy += wz[kz] * py;
+            gz += dz[kz] * pv;
+        }
+    }
+    return val;
+}
+
 // ------------------------------------------------------------- point state
 
 struct PtShared {
@@ -332,6 +492,39 @@ __device__ __forceinline__ void stable_pt(const SearchArgs &a, const PtShared &S
         x = S.sp[0] + a.sph[3 * j + 0];
         y = S.sp[1] + a.sph[3 * j + 1];
         z = S.sp[2] + a.sph[3 * j + 2];
+    }
+}
+
+// Sphere offsets of sample j, loaded one loop iteration ahead of their use in the sample
+// passes (zero for cubes, whose samples are computed from j).
+struct SphOff {
+    double x, y, z;
+};
+
+__device__ __forceinline__ SphOff sph_off(const SearchArgs &a, int j, bool valid) {
+    SphOff o = {0.0, 0.0, 0.0};
+    if (a.geom == DVC_GEOM_SPHERE && valid) {
+        o.x = a.sph[3 * j + 0];
+        o.y = a.sph[3 * j + 1];
+        o.z = a.sph[3 * j + 2];
+    }
+    return o;
+}
+
+// stable_pt with the sphere offsets already loaded (same arithmetic)
+__device__ __forceinline__ void stable_pt_off(const SearchArgs &a, const PtShared &S, int j, const SphOff &o,
+                                              double &x, double &y, double &z) {
+#pragma clang fp contract(off)
+    if (a.geom == DVC_GEOM_CUBE) {
+        int g = a.g;
+        int kx = j % g, ky = (j / g) % g, kz = j / (g * g);
+        x = S.smin[0] + (double)kx * S.inc[0];
+        y = S.smin[1] + (double)ky * S.inc[1];
+        z = S.smin[2] + (double)kz * S.inc[2];
+    } else {
+        x = S.sp[0] + o.x;
+        y = S.sp[1] + o.y;
+        z = S.sp[2] + o.z;
     }
 }
 
@@ -471,60 +664,51 @@ __device__ bool evaluate(const SearchArgs &a, PtShared &S, int ndof, bool need_j
     }
     __syncthreads();
     if (S.pfail) return false;
-    // moving bounding box
-    if (ndof > 3) {
-        double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
-        for (int j = tid; j < n; j += B) {
-            double xs, ys, zs, x, y, z;
-            stable_pt(a, S, j, xs, ys, zs);
-            warp_pt(S, ndof, xs, ys, zs, x, y, z);
-            mn[0] = fmin(mn[0], x); mn[1] = fmin(mn[1], y); mn[2] = fmin(mn[2], z);
-            mx[0] = fmax(mx[0], x); mx[1] = fmax(mx[1], y); mx[2] = fmax(mx[2], z);
-        }
-        for (int q = 0; q < 3; q++) { mn[q] = wave_min(mn[q]); mx[q] = wave_max(mx[q]); }
-        if (lane == 0)
-            for (int q = 0; q < 3; q++) { red[wid * 6 + q] = mn[q]; red[wid * 6 + 3 + q] = mx[q]; }
-        __syncthreads();
-        if (tid == 0) {
-            for (int q = 0; q < 3; q++) {
-                double lo = red[q], hi = red[3 + q];
-                for (int w = 1; w < NW; w++) { lo = fmin(lo, red[w * 6 + q]); hi = fmax(hi, red[w * 6 + 3 + q]); }
-                // resize_bbox sentinels (FloatingCloud.cpp:208-214)
-                S.bb[q] = fmin(lo, 9999.9);
-                S.bb[3 + q] = fmax(hi, -9999.9);
-            }
-            S.fail = range_ok(a, S, S.bb) ? 0 : 1;
-        }
-    } else if (tid == 0) {
+    // Range rule.  With 3 DOF the moving bounding box is the stable one shifted; with
+    // more it is accumulated in the sample pass below, which then runs speculatively:
+    // every sample is clamped into the interpolation box (inside the volume), and a
+    // failed range check discards the sums.
+    const bool merged = ndof > 3;
+    if (tid == 0) {
 #pragma clang fp contract(off)
-        for (int q = 0; q < 3; q++) { S.bb[q] = S.sbb_mn[q] + S.u[q]; S.bb[3 + q] = S.sbb_mx[q] + S.u[q]; }
-        S.fail = range_ok(a, S, S.bb) ? 0 : 1;
+        if (!merged) {
+            for (int q = 0; q < 3; q++) { S.bb[q] = S.sbb_mn[q] + S.u[q]; S.bb[3 + q] = S.sbb_mx[q] + S.u[q]; }
+            S.fail = range_ok(a, S, S.bb) ? 0 : 1;
+        } else {
+            S.fail = S.box_ok ? 0 : 1;
+        }
     }
     __syncthreads();
     if (S.fail) return false;
 
     const VolView vv = {a.vol[1], a.W, (long long)a.W * a.H};
     const BoxView bx = {{S.b0[0], S.b0[1], S.b0[2]}, {a.box[0], a.box[1], a.box[2]}};
+    const Gather gv = make_gather<BPV>(vv, bx, a.gather32 != 0);
     const double rbmean = S.rbmean;
     const float fbias = S.fbias;
     const int method = a.interp;
     const bool jrow = need_j;
-    floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
     double acc64[4] = {0.0, 0.0, 0.0, 0.0};
     float *wst = stage + wid * 64 * 17;
     double s_tc = 0.0, s_d2 = 0.0, s_ad = 0.0;
-    const int steps = (n + B - 1) / B;
-    for (int it = 0; it < steps; it++) {
-        int j = tid + it * B;
+    double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
+
+    // moving-cloud position of sample j (and its stable position)
+    auto position = [&](int j, const SphOff &o, double &xs, double &ys, double &zs, double &x, double &y,
+                        double &z) {
+        stable_pt_off(a, S, j, o, xs, ys, zs);
+        warp_pt(S, ndof, xs, ys, zs, x, y, z);
+        if (merged) {
+            mn[0] = fmin(mn[0], x); mn[1] = fmin(mn[1], y); mn[2] = fmin(mn[2], z);
+            mx[0] = fmax(mx[0], x); mx[1] = fmax(mx[1], y); mx[2] = fmax(mx[2], z);
+        }
+    };
+    // accumulate one sample (every lane of the wave calls it; ok = false adds nothing)
+    auto accumulate = [&](bool ok, int j, float t, float gx, float gy, float gz, double xs, double ys, double zs) {
         float v[16];
 #pragma unroll
         for (int q = 0; q < 16; q++) v[q] = 0.0f;
-        if (j < n) {
-            double xs, ys, zs, x, y, z;
-            stable_pt(a, S, j, xs, ys, zs);
-            warp_pt(S, ndof, xs, ys, zs, x, y, z);
-            float gx, gy, gz;
-            float t = interp_at<BPV>(vv, bx, x, y, z, method, fbias, jrow, gx, gy, gz);
+        if (ok) {
             float tc = (float)((double)t - rbmean);
             float rc = rcb[j];
             tcb[j] = tc;
@@ -568,22 +752,102 @@ __device__ bool evaluate(const SearchArgs &a, PtShared &S, int ndof, bool need_j
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            // two accumulators: independent MFMA chains of 8
+            floatx4 acc0 = {0.0f, 0.0f, 0.0f, 0.0f}, acc1 = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-            for (int s = 0; s < 16; s++) {
-                float xv = wst[(4 * s + (lane >> 4)) * 17 + (lane & 15)];
-                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(xv, xv, acc, 0, 0, 0);
+            for (int s = 0; s < 16; s += 2) {
+                float xv0 = wst[(4 * s + (lane >> 4)) * 17 + (lane & 15)];
+                float xv1 = wst[(4 * (s + 1) + (lane >> 4)) * 17 + (lane & 15)];
+                acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(xv0, xv0, acc0, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(xv1, xv1, acc1, 0, 0, 0);
             }
 #pragma unroll
-            for (int r = 0; r < 4; r++) { acc64[r] += (double)acc[r]; acc[r] = 0.0f; }
+            for (int r = 0; r < 4; r++) acc64[r] += (double)acc0[r] + (double)acc1[r];
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
+    };
+
+    const int steps = (n + B - 1) / B;
+    if (method == DVC_INTERP_TRICUBIC) {
+        // kInflight samples per thread: all their row loads are issued before any is used
+        constexpr int U = kInflight;
+        SphOff pre[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) pre[u] = sph_off(a, tid + u * B, tid + u * B < n);
+        for (int it = 0; it < steps; it += U) {
+            double xs[U], ys[U], zs[U];
+            Tap<BPV> t[U];
+            SphOff cur[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                cur[u] = pre[u];
+                const int jn = tid + (it + U + u) * B;
+                pre[u] = sph_off(a, jn, jn < n);
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int j = tid + (it + u) * B;
+                double x = 0.0, y = 0.0, z = 0.0;
+                xs[u] = ys[u] = zs[u] = 0.0;
+                if (j < n) position(j, cur[u], xs[u], ys[u], zs[u], x, y, z);
+                tap_locate<BPV>(bx, x, y, z, t[u]);
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) tap_load<BPV>(vv, gv, bx, t[u]);
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                if (it + u < steps) {
+                    const int j = tid + (it + u) * B;
+                    float gx, gy, gz;
+                    float tv = tap_eval<BPV>(vv, bx, t[u], fbias, jrow, gx, gy, gz);
+                    accumulate(j < n, j, tv, gx, gy, gz, xs[u], ys[u], zs[u]);
+                }
+            }
+        }
+    } else {
+        for (int it = 0; it < steps; it++) {
+            const int j = tid + it * B;
+            const bool ok = j < n;
+            double xs = 0.0, ys = 0.0, zs = 0.0, x = 0.0, y = 0.0, z = 0.0;
+            float t = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f;
+            if (ok) {
+                position(j, sph_off(a, j, true), xs, ys, zs, x, y, z);
+                t = interp_at<BPV>(vv, bx, x, y, z, method, fbias, jrow, gx, gy, gz);
+            }
+            accumulate(ok, j, t, gx, gy, gz, xs, ys, zs);
+        }
     }
+    // block reduction of the sums and (merged) of the moving bounding box, then the
+    // range rule (Interpolate.cpp:474-481 on the box of FloatingCloud::resize_bbox)
     {
-        double v3[3] = {s_tc, s_d2, s_ad};
-        block_sum<B, 3>(v3, red, S.sums);
+        double v3[3] = {wave_sum(s_tc), wave_sum(s_d2), wave_sum(s_ad)};
+        if (merged)
+            for (int q = 0; q < 3; q++) { mn[q] = wave_min(mn[q]); mx[q] = wave_max(mx[q]); }
+        if (lane == 0) {
+            for (int q = 0; q < 3; q++) red[wid * 9 + q] = v3[q];
+            for (int q = 0; q < 3; q++) { red[wid * 9 + 3 + q] = mn[q]; red[wid * 9 + 6 + q] = mx[q]; }
+        }
+        __syncthreads();
+        if (tid < 3) {
+            double sacc = 0.0;
+            for (int w = 0; w < NW; w++) sacc += red[w * 9 + tid];
+            S.sums[tid] = sacc;
+        }
+        if (merged && tid == 0) {
+            for (int q = 0; q < 3; q++) {
+                double lo = red[3 + q], hi = red[6 + q];
+                for (int w = 1; w < NW; w++) { lo = fmin(lo, red[w * 9 + 3 + q]); hi = fmax(hi, red[w * 9 + 6 + q]); }
+                // resize_bbox sentinels (FloatingCloud.cpp:208-214)
+                S.bb[q] = fmin(lo, 9999.9);
+                S.bb[3 + q] = fmax(hi, -9999.9);
+            }
+            S.fail = range_ok(a, S, S.bb) ? 0 : 1;
+        }
     }
+    __syncthreads();
+    if (S.fail) return false;
     if (jrow) {
         double *st64 = (double *)stage;
         // D row = 4*(lane/16) + r, column = lane%16 (symmetric, so either reading works)
@@ -905,15 +1169,53 @@ __device__ void point_search(const SearchArgs &a, int p, int n, PtShared &S, flo
     {
         const VolView vr = {a.vol[0], a.W, (long long)a.W * a.H};
         const BoxView bx = {{S.b0[0], S.b0[1], S.b0[2]}, {a.box[0], a.box[1], a.box[2]}};
+        const Gather gr = make_gather<BPV>(vr, bx, a.gather32 != 0);
         const float fbias = S.fbias;
         double s[1] = {0.0};
-        for (int j = tid; j < ns; j += B) {
-            double x, y, z;
-            stable_pt(a, S, j, x, y, z);
-            float gx, gy, gz;
-            float r = interp_at<BPV>(vr, bx, x, y, z, a.interp, fbias, false, gx, gy, gz);
-            rcb[j] = r;
-            s[0] += (double)r;
+        if (a.interp == DVC_INTERP_TRICUBIC) {
+            // kInflight samples per thread in flight (see evaluate)
+            constexpr int U = kInflight;
+            SphOff pre[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) pre[u] = sph_off(a, tid + u * B, tid + u * B < ns);
+            for (int j0 = tid; j0 < ns; j0 += U * B) {
+                Tap<BPV> t[U];
+                SphOff cur[U];
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    cur[u] = pre[u];
+                    const int jn = j0 + (U + u) * B;
+                    pre[u] = sph_off(a, jn, jn < ns);
+                }
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    const int j = j0 + u * B;
+                    double x = 0.0, y = 0.0, z = 0.0;
+                    if (j < ns) stable_pt_off(a, S, j, cur[u], x, y, z);
+                    tap_locate<BPV>(bx, x, y, z, t[u]);
+                }
+#pragma unroll
+                for (int u = 0; u < U; u++) tap_load<BPV>(vr, gr, bx, t[u]);
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    const int j = j0 + u * B;
+                    if (j < ns) {
+                        float gx, gy, gz;
+                        float r = tap_eval<BPV>(vr, bx, t[u], fbias, false, gx, gy, gz);
+                        rcb[j] = r;
+                        s[0] += (double)r;
+                    }
+                }
+            }
+        } else {
+            for (int j = tid; j < ns; j += B) {
+                double x, y, z;
+                stable_pt(a, S, j, x, y, z);
+                float gx, gy, gz;
+                float r = interp_at<BPV>(vr, bx, x, y, z, a.interp, fbias, false, gx, gy, gz);
+                rcb[j] = r;
+                s[0] += (double)r;
+            }
         }
         block_sum<B, 1>(s, red, S.sums);
         const double rbmean = S.sums[0] / (double)ns;
@@ -1049,7 +1351,7 @@ __global__ void __launch_bounds__(B) search_kernel(SearchArgs a, const int *leve
     float *tcb = dyn + a.nsamp_pad;
     float *stage = dyn + 2 * a.nsamp_pad;
     DVC_SHARED(PtShared, S);
-    DVC_SHARED(double[(B / 64) * 8], red);
+    DVC_SHARED(double[(B / 64) * 9], red);
     DVC_SHARED(double[256], V);
     const int tid = threadIdx.x;
     const int n = level_pts[blockIdx.x];
@@ -1118,6 +1420,28 @@ int launch_search(const SearchArgs &a, const int *level_pts, int block_threads, 
     DVC_DISPATCH(2, 256, launch_search_t)(a, level_pts, grid, lds, st);
     DVC_DISPATCH(2, 512, launch_search_t)(a, level_pts, grid, lds, st);
     DVC_DISPATCH(2, 1024, launch_search_t)(a, level_pts, grid, lds, st);
+    return -1;
+}
+
+template <int BPV, int B>
+static int static_lds_t(size_t *bytes, int *) {
+    hipFuncAttributes fa;
+    if (hipFuncGetAttributes(&fa, (const void *)search_kernel<BPV, B>) != hipSuccess) return -1;
+    *bytes = fa.sharedSizeBytes;
+    return 0;
+}
+
+int search_static_lds(int bpv, int block_threads, size_t *bytes) {
+    DVC_DISPATCH(1, 64, static_lds_t)(bytes, nullptr);
+    DVC_DISPATCH(1, 128, static_lds_t)(bytes, nullptr);
+    DVC_DISPATCH(1, 256, static_lds_t)(bytes, nullptr);
+    DVC_DISPATCH(1, 512, static_lds_t)(bytes, nullptr);
+    DVC_DISPATCH(1, 1024, static_lds_t)(bytes, nullptr);
+    DVC_DISPATCH(2, 64, static_lds_t)(bytes, nullptr);
+    DVC_DISPATCH(2, 128, static_lds_t)(bytes, nullptr);
+    DVC_DISPATCH(2, 256, static_lds_t)(bytes, nullptr);
+    DVC_DISPATCH(2, 512, static_lds_t)(bytes, nullptr);
+    DVC_DISPATCH(2, 1024, static_lds_t)(bytes, nullptr);
     return -1;
 }
 

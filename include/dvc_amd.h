@@ -116,6 +116,13 @@ int dvc_interpolate(dvc_ctx *ctx, int slot, const dvc_run_params *rc, const doub
                     const double *xyz, int32_t n, const double bbox[6], int method,
                     double *out, double *grad);
 
+/* The same on the boxes of Interpolate(const BoundBox *region) (Interpolate.cpp:34-58):
+ * est box = region [min, max], active box = region less one voxel on every side,
+ * coefficient extent = trunc(max) - trunc(min) voxels (at least 4 per axis). */
+int dvc_interpolate_box(dvc_ctx *ctx, int slot, const double region_min[3], const double region_max[3],
+                        const double center[3], const double *xyz, int32_t n, const double bbox[6], int method,
+                        double *out, double *grad);
+
 /* obj_SAD / obj_SSD / obj_ZSSD / obj_NSSD / obj_ZNSSD with residual
  * (ObjectiveFunctions.h:31-45, ObjectiveFunctions.cpp:24-263), on the device in FP64.
  * residual[n] may be NULL. */

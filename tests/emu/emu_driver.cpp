@@ -23,7 +23,7 @@ static void run_search(const SearchArgs &a, size_t lds, const std::vector<int> &
 
 extern "C" int emu_search(const dvc_run_params *rc, const void *ref, const void *tar, int bpv, int W, int H, int T,
                           const double *pts, int npts, const int *order, int nproc, const int *neigh, int k,
-                          int block_threads, dvc_point_result *out) {
+                          int block_threads, int gather_mode, dvc_point_result *out) {
     Geometry G = make_geometry(*rc);
     int N = npts;
     if (nproc > 1 && nproc < npts) N = nproc;
@@ -31,7 +31,10 @@ extern "C" int emu_search(const dvc_run_params *rc, const void *ref, const void 
     if (!order_positions(order, npts, N, pos)) return -1;
     std::vector<double> sph;
     double omn[3] = {0, 0, 0}, omx[3] = {0, 0, 0};
-    if (rc->geom == DVC_GEOM_SPHERE) sphere_template(*rc, G, sph, omn, omx);
+    if (rc->geom == DVC_GEOM_SPHERE) {
+        sphere_template(*rc, G, sph, omn, omx);
+        locality_order(sph);
+    }
     size_t vbytes = (size_t)W * H * T * bpv;
     std::vector<uint32_t> v0(vbytes / 4 + 32), v1(vbytes / 4 + 32);
     std::memcpy(v0.data(), ref, vbytes);
@@ -44,6 +47,7 @@ extern "C" int emu_search(const dvc_run_params *rc, const void *ref, const void 
     fill_search_args(a, *rc, G, bpv, W, H, T, npts, N, k, omn, omx);
     a.vol[0] = (const uint8_t *)v0.data();
     a.vol[1] = (const uint8_t *)v1.data();
+    if (gather_mode == 0) a.gather32 = 0;          // force the 64-bit row addressing
     a.sph = sph.data(); a.pts = pts; a.order = order; a.pos = pos.data(); a.neigh = neigh;
     a.res = res.data();
     size_t lds = (size_t)a.nsamp_pad * 8 + (size_t)(block_threads / 64) * 64 * 17 * 4;

@@ -20,11 +20,12 @@ def built():
     emu.build()
 
 
-def _cmp(oracle, case, params, threads=64, nproc=0):
+def _cmp(oracle, case, params, threads=64, nproc=0, gather32=True):
     order, neigh = D.sort_order_neighbors(case.points, start=case.start)
     run = oracle.make_run(shape=case.ref.shape, bytes_per=case.ref.dtype.itemsize, **params)
     o, _, _ = oracle.dvc_run(run, case.ref, case.tar, case.points, order, neigh, nproc=nproc, nthreads=4)
-    g = emu.search(D.make_params(**params), case.ref, case.tar, case.points, order, neigh, nproc=nproc, threads=threads)
+    g = emu.search(D.make_params(**params), case.ref, case.tar, case.points, order, neigh, nproc=nproc, threads=threads,
+                   gather32=gather32)
     assert np.array_equal(g["status"], o["status"])
     assert np.array_equal(g["iters"], o["iters"])
     good = o["status"] == 0
@@ -34,8 +35,17 @@ def _cmp(oracle, case, params, threads=64, nproc=0):
     return g, o
 
 
-def test_translation_cube_u8(oracle):
+@pytest.mark.parametrize("gather32", [True, False])
+def test_translation_cube_u8(oracle, gather32):
     case = synth.config1(size=56, n=2, spacing=8, npts=1000)
+    _cmp(oracle, case, dict(case.params, subvol_size=21, disp_max=4), gather32=gather32)
+
+
+def test_odd_row_width_u8(oracle):
+    """A row width that is not a whole number of dwords takes the 64-bit row addressing."""
+    case = synth.config1(size=58, n=2, spacing=8, npts=1000)
+    ref, tar = case.ref[:, :, :57].copy(), case.tar[:, :, :57].copy()
+    case = synth.Case(case.name, ref, tar, case.points, case.truth, case.params)
     _cmp(oracle, case, dict(case.params, subvol_size=21, disp_max=4))
 
 
@@ -94,3 +104,14 @@ def test_strain_kernel(oracle):
         assert np.array_equal(o["npts"], g["npts"])
         for k in ("DG", "E", "L", "disp", "swr"):
             assert np.abs(o[k] - g[k]).max() <= 1e-9, k
+
+
+def test_12dof_range_failures(oracle):
+    """Range_Fail at set-up (boxes leaving the volume) and inside the Gauss-Newton loop
+    (the moving cloud leaving the active box with disp_max 1), where the bounding box
+    comes from the speculative sample pass."""
+    case = synth.config2(size=72, n=3, spacing=20, npts=600, diameter=21)
+    g, o = _cmp(oracle, case, dict(case.params, disp_max=5), threads=128)
+    assert (o["status"] == D.RANGE_FAIL).any() and (o["status"] == D.POINT_GOOD).any()
+    g, o = _cmp(oracle, case, dict(case.params, disp_max=1, rigid_trans=(0.0, 0.0, 0.0)), threads=128)
+    assert np.all(o["status"] == D.RANGE_FAIL) and np.all(o["iters"] >= 3)
