@@ -145,6 +145,8 @@ def main():
         traffic = json.loads(tf.read_text()).get("hbm_bytes_per_step")
     roof = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "traffic_per": "step (HBM bytes of one step, FETCH_SIZE + WRITE_SIZE scaled by a 1 GiB copy, from "
+                           "profiles/search_kernel_affine_u16_256_traffic.json)",
             "kernel": "search_kernel<2,B>", "launches_per_step": ctx.levels(),
             "algorithmic_bytes_per_point": round(per_pt, 1), "bytes_points_sampled": nsamp}
 

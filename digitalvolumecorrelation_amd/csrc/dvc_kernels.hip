@@ -925,7 +925,9 @@ __device__ bool evaluate(const SearchArgs &a, PtShared &S, int ndof, bool need_j
 __device__ __forceinline__ double rdlane(double v, int l) { return emu::shfl_idx(v, l); }
 __device__ __forceinline__ double shfl_from(double v, int src) { return emu::shfl_idx(v, src); }
 __device__ __forceinline__ int shfl_from(int v, int src) { return (int)emu::shfl_idx((double)v, src); }
+__device__ __forceinline__ int rdlane_i(int v, int l) { return (int)emu::shfl_idx((double)v, l); }
 #else
+__device__ __forceinline__ int rdlane_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
 __device__ __forceinline__ double rdlane(double v, int l) {
     long long b = __double_as_longlong(v);
     int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
@@ -1689,6 +1691,156 @@ __device__ void fullpiv_apply(const double *Q, int n, const double *hc, const in
     for (int i = 0; i < rank; i++) x[perm[i]] = c[i];
 }
 
+// The same factorisation and solve on one wavefront, for the 10x10 strain fit with its
+// three right-hand sides: lane j < 10 holds column j of the normal matrix, lanes 10..12
+// the right-hand sides X^T d.  The pivot search takes each column's first largest
+// entry and then the first column holding the largest (the serial column-major scan);
+// row and column swaps, reflectors and the rank rule follow fullpiv_factor and
+// fullpiv_apply operation by operation.  The reflectors are applied to the right-hand
+// sides as they are formed: those past the rank only change entries the back
+// substitution does not read.  Solutions land in par[3][10].
+__device__ void fullpiv_wave10(const double *M, double (*par)[10]) {
+#pragma clang fp contract(off)
+    constexpr int N = 10;
+    const int lane = threadIdx.x & 63;
+    const bool is_col = lane < N, is_rhs = lane >= N && lane < N + 3;
+    double col[N];
+#pragma unroll
+    for (int i = 0; i < N; i++)
+        col[i] = is_col ? M[lane * N + i] : (is_rhs ? M[100 + (lane - N) * N + i] : 0.0);
+    const double prec = 2.220446049250313e-16 * (double)N;
+    double biggest = 0.0, maxpivot = 0.0;
+    int nonzero = N;
+    int rt[N], ct[N];
+    double hc[N];
+#pragma unroll
+    for (int k = 0; k < N; k++) { rt[k] = k; ct[k] = k; hc[k] = 0.0; }
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+        if (nonzero != N) break;
+        // first largest |entry| of this lane's column among rows k..N-1
+        double cm = -1.0;
+        int ci = k;
+#pragma unroll
+        for (int i = k; i < N; i++) {
+            double av = fabs(col[i]);
+            if (av > cm) { cm = av; ci = i; }
+        }
+        double sc = -1.0;
+        int rb = k, cb = k;
+#pragma unroll
+        for (int j = k; j < N; j++) {
+            double v = rdlane(cm, j);
+            int r = rdlane_i(ci, j);
+            if (v > sc) { sc = v; rb = r; cb = j; }
+        }
+        if (k == 0) biggest = sc;
+        if (sc <= biggest * prec) {
+            nonzero = k;
+            break;
+        }
+        rt[k] = rb; ct[k] = cb;
+        // row swap k <-> rb in columns k.. (and in the right-hand sides)
+        if (k != rb && ((lane >= k && lane < N) || is_rhs)) {
+            double tk = col[k], tr = 0.0;
+#pragma unroll
+            for (int i = k + 1; i < N; i++)
+                if (i == rb) tr = col[i];
+            col[k] = tr;
+#pragma unroll
+            for (int i = k + 1; i < N; i++)
+                if (i == rb) col[i] = tk;
+        }
+        // column swap k <-> cb
+        if (k != cb) {
+            const int src = (lane == k) ? cb : ((lane == cb) ? k : lane);
+#pragma unroll
+            for (int i = 0; i < N; i++) col[i] = shfl_from(col[i], src);
+        }
+        // makeHouseholder of column k, rows k..N-1 (uniform)
+        const int m = N - k;
+        double ess[N];
+        double tail = 0.0;
+#pragma unroll
+        for (int i = 1; i < m; i++) { ess[i] = rdlane(col[k + i], k); tail += ess[i] * ess[i]; }
+        const double c0 = rdlane(col[k], k);
+        double tau, beta;
+        if (tail <= 2.2250738585072014e-308) {
+            tau = 0.0; beta = c0;
+#pragma unroll
+            for (int i = 1; i < m; i++) ess[i] = 0.0;
+        } else {
+            beta = sqrt(c0 * c0 + tail);
+            if (c0 >= 0.0) beta = -beta;
+#pragma unroll
+            for (int i = 1; i < m; i++) ess[i] = ess[i] / (c0 - beta);
+            tau = (beta - c0) / beta;
+        }
+        hc[k] = tau;
+        if (lane == k) {
+            col[k] = beta;
+#pragma unroll
+            for (int i = 1; i < m; i++) col[k + i] = ess[i];
+        }
+        if (fabs(beta) > maxpivot) maxpivot = fabs(beta);
+        if ((lane > k && lane < N) || is_rhs) {
+            if (m == 1) {
+                col[k] *= (1.0 - tau);
+            } else if (tau != 0.0) {
+                double t = 0.0;
+#pragma unroll
+                for (int i = 1; i < m; i++) t += ess[i] * col[k + i];
+                t += col[k];
+                col[k] -= tau * t;
+#pragma unroll
+                for (int i = 1; i < m; i++) col[k + i] -= tau * ess[i] * t;
+            }
+        }
+    }
+    const double pthr = fabs(maxpivot) * (2.220446049250313e-16 * (double)N);
+    int rank = 0;
+#pragma unroll
+    for (int i = 0; i < N; i++)
+        if (i < nonzero) rank += (fabs(rdlane(col[i], i)) > pthr);
+    // back substitution in each right-hand-side lane
+    double c[N];
+#pragma unroll
+    for (int i = 0; i < N; i++) c[i] = col[i];
+#pragma unroll
+    for (int i = N - 1; i >= 0; i--) {
+        if (i < rank) {
+            double sacc = c[i];
+#pragma unroll
+            for (int j = i + 1; j < N; j++)
+                if (j < rank) sacc -= rdlane(col[i], j) * c[j];
+            c[i] = sacc / rdlane(col[i], i);
+        }
+    }
+    // column permutation from the transpositions; x[perm[i]] = c[i]
+    int perm[N];
+#pragma unroll
+    for (int i = 0; i < N; i++) perm[i] = i;
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+        int t = perm[k], pk = 0;
+#pragma unroll
+        for (int q = 0; q < N; q++)
+            if (q == ct[k]) pk = perm[q];
+        perm[k] = pk;
+#pragma unroll
+        for (int q = 0; q < N; q++)
+            if (q == ct[k]) perm[q] = t;
+    }
+    if (is_rhs) {
+        double *x = par[lane - N];
+#pragma unroll
+        for (int i = 0; i < N; i++) x[i] = 0.0;
+#pragma unroll
+        for (int i = 0; i < N; i++)
+            if (i < rank) x[perm[i]] = c[i];
+    }
+}
+
 // Eigenvalues of a symmetric 3x3 (cyclic Jacobi), sorted descending
 // (strain.cpp:553-556 takes EigenSolver real parts and sorts them).
 __device__ void sym3_eigen(const double *S, double *lam) {
@@ -1738,20 +1890,33 @@ __global__ void __launch_bounds__(64 * kStrainWaves) strain_kernel(StrainArgs a)
     DVC_SHARED(double[kStrainWaves][75][3], Ds);
     DVC_SHARED(double[kStrainWaves][100 + 30], Ms);
     DVC_SHARED(int[kStrainWaves][76], sel);
+    DVC_SHARED(double[kStrainWaves][3][10], Ps);
     if (i >= a.n) return;      // whole wave leaves together; no block barrier below
     const int *nb = a.neigh + (long long)i * a.k;
     double px = a.pts[3 * i], py = a.pts[3 * i + 1], pz = a.pts[3 * i + 2];
     double distance = sqrt(px * px + py * py + pz * pz);
     double scale = 1.0 / distance;
     double bx = scale * px, by = scale * py, bz = scale * pz;
-    if (lane == 0) {
-        int lim = a.refill ? (a.k < 75 ? a.k : 75) : (a.ndp < a.k ? a.ndp : a.k);
+    {
+        // candidates in neighbour order; the first ndp that pass form the window.  Every
+        // lane tests its own candidates, a ballot keeps the order.
+        const int lim = a.refill ? (a.k < 75 ? a.k : 75) : (a.ndp < a.k ? a.ndp : a.k);
         int np = 0;
-        for (int q = 0; q < lim && np < a.ndp; q++) {
-            int m = nb[q];
-            if (m >= 0 && m < a.n && a.status[m] == 0 && a.objmin[m] <= a.objt) sel[w][np++] = m;
+        for (int q0 = 0; q0 < lim && np < a.ndp; q0 += 64) {
+            const int q = q0 + lane;
+            bool ok = false;
+            int m = -1;
+            if (q < lim) {
+                m = nb[q];
+                ok = m >= 0 && m < a.n && a.status[m] == 0 && a.objmin[m] <= a.objt;
+            }
+            const unsigned long long mask = __ballot(ok);
+            const int before = __popcll(mask & ((1ull << lane) - 1ull));
+            if (ok && np + before < a.ndp) sel[w][np + before] = m;
+            const int cnt = __popcll(mask);
+            np = (np + cnt < a.ndp) ? np + cnt : a.ndp;
         }
-        sel[w][75] = np;
+        if (lane == 0) sel[w][75] = np;
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -1803,15 +1968,14 @@ __global__ void __launch_bounds__(64 * kStrainWaves) strain_kernel(StrainArgs a)
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    fullpiv_wave10(Ms[w], Ps[w]);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     if (lane != 0) return;
-    double Q[100], hc[10];
-    int rt[10], ct[10], rank;
-    for (int e = 0; e < 100; e++) Q[e] = Ms[w][e];
-    fullpiv_factor(Q, 10, hc, rt, ct, &rank);
     double DG[9], displ[3];
     for (int c = 0; c < 3; c++) {
-        double par[10];
-        fullpiv_apply(Q, 10, hc, rt, ct, rank, &Ms[w][100 + 10 * c], par);
+        const double *par = Ps[w][c];
         DG[c * 3 + 0] = par[1] + 2 * par[4] * bx + par[7] * by + par[9] * bz;
         DG[c * 3 + 1] = par[2] + 2 * par[5] * by + par[7] * bx + par[8] * bz;
         DG[c * 3 + 2] = par[3] + 2 * par[6] * bz + par[8] * by + par[9] * bx;

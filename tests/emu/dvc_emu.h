@@ -77,6 +77,18 @@ inline double shfl_idx(double v, int src) {
     return r;
 }
 
+// __ballot: bit i set when lane i of this wave passes true
+inline unsigned long long ballot(bool b) {
+    unsigned t = thread_idx.x, w = t / 64, l = t % 64;
+    cur->xd[w * 64 + l] = b ? 1.0 : 0.0;
+    wave_sync();
+    unsigned long long m = 0;
+    for (unsigned i = 0; i < 64 && w * 64 + i < cur->threads; i++)
+        if (cur->xd[w * 64 + i] != 0.0) m |= 1ull << i;
+    wave_sync();
+    return m;
+}
+
 typedef float f4 __attribute__((vector_size(16)));
 inline f4 mfma16x16x4(float a, float b, f4 c) {
     unsigned t = thread_idx.x, w = t / 64, l = t % 64;
@@ -143,6 +155,8 @@ void run_grid(unsigned first, unsigned blocks, unsigned threads, size_t dyn_byte
 inline void __syncthreads() { emu::cur->bar->arrive_and_wait(); }
 inline double __shfl_xor(double v, int mask, int) { return emu::shfl_xor(v, mask); }
 #define __builtin_amdgcn_wave_barrier() emu::wave_sync()
+#define __ballot(x) emu::ballot(x)
+inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
 #define __builtin_amdgcn_fence(order, scope) std::atomic_thread_fence(std::memory_order_seq_cst)
 #define __builtin_amdgcn_s_sleep(n) std::this_thread::yield()
 #define __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, x, y, z) emu::mfma16x16x4((a), (b), (c))
