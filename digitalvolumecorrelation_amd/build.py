@@ -58,7 +58,7 @@ def build_host() -> list[Path]:
     if not HOST.exists():
         return out
     BIN.mkdir(exist_ok=True)
-    common = [str(HOST / "dvc_host.cpp")]
+    common = [str(HOST / "dvc_io.cpp"), str(HOST / "dvc_search.cpp")]
     for name, main in (("dvc", "dvc_main.cpp"), ("strain", "strain_main.cpp")):
         src = HOST / main
         if not src.exists():

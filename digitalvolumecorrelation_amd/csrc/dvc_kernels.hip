@@ -1717,7 +1717,7 @@ __device__ void fullpiv_wave10(const double *M, double (*par)[10]) {
     for (int k = 0; k < N; k++) { rt[k] = k; ct[k] = k; hc[k] = 0.0; }
 #pragma unroll
     for (int k = 0; k < N; k++) {
-        if (nonzero != N) break;
+        if (nonzero != N) continue;      // (no break: keeps the loop fully unrolled)
         // first largest |entry| of this lane's column among rows k..N-1
         double cm = -1.0;
         int ci = k;
@@ -1737,7 +1737,7 @@ __device__ void fullpiv_wave10(const double *M, double (*par)[10]) {
         if (k == 0) biggest = sc;
         if (sc <= biggest * prec) {
             nonzero = k;
-            break;
+            continue;
         }
         rt[k] = rb; ct[k] = cb;
         // row swap k <-> rb in columns k.. (and in the right-hand sides)

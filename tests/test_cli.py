@@ -1,0 +1,120 @@
+"""The dvc and strain programs on the host side (no GPU needed): keyword input file,
+ROI reading, the .sort.csv and .stat files, and the reference's error messages
+(Core/InputRead.cpp:477-716, 1033-1106; Core/DataCloud.cpp:43-67, 146-158;
+Core/strain.cpp:136-330).  Without a GPU the programs stop with a device error after
+writing these files; on a GPU box they run to the end."""
+import math
+
+import numpy as np
+import pytest
+
+import cli_util as U
+from digitalvolumecorrelation_amd import synth
+
+
+@pytest.fixture(scope="module")
+def bins():
+    return U.ensure_built()
+
+
+@pytest.fixture(scope="module")
+def case():
+    return synth.config1(size=48, n=2, spacing=8, npts=500)
+
+
+def _params(case):
+    return dict(case.params, subvol_size=15, disp_max=3)
+
+
+def test_input_echo_and_sort_file(tmp_path, bins, case, oracle):
+    start = (20.0, 21.0, 22.0)
+    d = U.write_run(tmp_path, case.ref, case.tar, case.points, _params(case),
+                    extra={"starting_point": " ".join(map(str, start))})
+    r = U.run([bins / "dvc", "dvc_in"], tmp_path)
+    assert r.returncode in (0, 1), r.stdout + r.stderr
+    assert "Max points to process is 0" in r.stdout
+    order, neigh = oracle.sort_order_neighbors(case.points, start=np.array(start))
+    assert np.array_equal(U.read_sort(d / "roi.txt.sort.csv"), neigh)
+    sta = (d / "result.stat").read_text()
+    assert sta.startswith("\n### echo of the input file for this run\n\n")
+    for line in ("reference_filename\tref.raw", "vol_bit_depth\t8", "subvol_geom\tcube", "disp_max\t3",
+                 "num_srch_dof\t3", "obj_function\tznssd", "interp_type\ttricubic", "rigid_trans\t1\t-1\t2",
+                 "basin_radius\t0", "subvol_aspect\t1\t1\t1", f"Point Cloud contains {len(case.points)} points"):
+        assert line in sta, line
+    mn, mx = case.points.min(0), case.points.max(0)
+    assert f"bounding box min = [{U.cfmt(mn[0])} {U.cfmt(mn[1])} {U.cfmt(mn[2])}]" in sta
+    assert f"bounding box max = [{U.cfmt(mx[0])} {U.cfmt(mx[1])} {U.cfmt(mx[2])}]" in sta
+    if r.returncode == 1:
+        assert "device failure" in r.stdout
+
+
+def test_absent_starting_point_sorts_from_nan(tmp_path, bins, case, oracle):
+    """DataCloud.cpp:149-155 compares a vector of NaNs with !=, which is always true, so
+    without the keyword the sort starts from (NaN, NaN, NaN)."""
+    d = U.write_run(tmp_path, case.ref, case.tar, case.points, _params(case))
+    r = U.run([bins / "dvc", "dvc_in"], tmp_path)
+    assert r.returncode in (0, 1)
+    _, neigh = oracle.sort_order_neighbors(case.points, start=np.full(3, math.nan))
+    assert np.array_equal(U.read_sort(d / "roi.txt.sort.csv"), neigh)
+
+
+@pytest.mark.parametrize("omit, extra, message", [
+    (("disp_max",), {}, "Input Error: keyword 'disp_max' not found in input file."),
+    ((), {"num_srch_dof": 5}, "Input Error: num_srch_dof contains an invalid parameter."),
+    ((), {"obj_function": "ncc"}, "Input Error: obj_function contains an invalid parameter."),
+    ((), {"subvol_aspect": "1.0 20.0 1.0"}, "Input Error: subvol_aspect contains an invalid parameter."),
+    ((), {"vol_wide": 47}, "Reference volume file size does not match the volume description."),
+    ((), {"reference_filename": "missing.raw"}, "Input Error: reference_filename cannot find file: missing.raw"),
+    (("num_points_to_process",), {}, "keyword 'num_points_to_process' not found"),
+])
+def test_input_errors(tmp_path, bins, case, omit, extra, message):
+    U.write_run(tmp_path, case.ref, case.tar, case.points, _params(case), extra=extra, omit=omit)
+    r = U.run([bins / "dvc", "dvc_in"], tmp_path)
+    assert r.returncode == 0
+    assert message in r.stdout, r.stdout
+    assert not (tmp_path / "roi.txt.sort.csv").exists()
+
+
+def test_roi_file_rules(tmp_path, bins, case):
+    """Header and malformed lines are skipped; a label that reads as a real number is
+    rejected (InputRead.cpp:661-686)."""
+    d = U.write_run(tmp_path, case.ref, case.tar, case.points, _params(case))
+    good = [(7, 20.5, 21.0, 22.0), (9, 24.0, 22.5, 23.0), (3, 22.0, 25.0, 24.5)]
+    text = "n x y z\n# comment\n" + "".join(f"{a}\t{b}\t{c}\t{e}\n" for a, b, c, e in good[:2])
+    text += "1.5 20 21 22\n12 20 21\n" + "%d %r %r %r\r\n" % good[2]
+    (d / "roi.txt").write_text(text)
+    r = U.run([bins / "dvc", "dvc_in"], tmp_path)
+    assert "Point Cloud contains 3 points" in (d / "result.stat").read_text(), r.stdout
+    assert U.read_sort(d / "roi.txt.sort.csv").shape == (3, 3)
+
+
+def test_header_and_big_endian_volumes_accepted(tmp_path, bins):
+    case = synth.config2(size=48, n=2, spacing=8, npts=400, diameter=13)
+    p = dict(case.params, disp_max=3)
+    d = U.write_run(tmp_path, case.ref, case.tar, case.points, p, header=b"\0" * 24, big_endian=True)
+    r = U.run([bins / "dvc", "dvc_in"], tmp_path)
+    assert r.returncode in (0, 1), r.stdout
+    sta = (d / "result.stat").read_text()
+    assert "vol_endian\tbig" in sta and "vol_hdr_lngth\t24" in sta
+
+
+def test_strain_program_sort_and_flags(tmp_path, bins, oracle):
+    pts = synth.grid_points(3, 6.0, (30.0, 31.0, 32.0))
+    rng = np.random.default_rng(4)
+    disp = rng.normal(0, 0.1, pts.shape)
+    lines = ["n\tx\ty\tz\tstatus\tobjmin\tu\tv\tw"]
+    for i, (p, u) in enumerate(zip(pts, disp)):
+        lines.append(f"{i + 1}\t{U.cfmt(p[0])}\t{U.cfmt(p[1])}\t{U.cfmt(p[2])}\t0\t0.001\t"
+                     f"{u[0]:.6f}\t{u[1]:.6f}\t{u[2]:.6f}")
+    (tmp_path / "run.disp").write_text("\n".join(lines) + "\n")
+    r = U.run([bins / "strain", "-sw", "12", "-Q", "run.disp"], tmp_path)
+    assert r.returncode in (0, 1), r.stdout
+    assert "-> unused flag -Q found, ignored" in r.stdout
+    assert "using strain window: 12" in r.stdout
+    _, neigh = oracle.sort_order_neighbors(pts, start=pts[0])
+    assert np.array_equal(U.read_sort(tmp_path / "run.sort.csv"), neigh)
+    r = U.run([bins / "strain", "run.disp"], tmp_path.parent)
+    assert "cannot find .disp file" in r.stdout
+    (tmp_path / "bad.disp").write_text("x\ty\n1\t2\n")
+    r = U.run([bins / "strain", "bad.disp"], tmp_path)
+    assert ".disp file lacks proper header" in r.stdout
