@@ -35,12 +35,15 @@ std::vector<std::string> split_lines(const std::string &text, char eol) {
     return out;
 }
 
-// i-th tab-separated non-empty field (the file-name keywords, InputRead.cpp:1295-1299).
+// i-th tab-separated non-empty field (the file-name keywords, InputRead.cpp:1295-1299);
+// unlike the reference, a name that ends the line is accepted.
 bool tab_field(const std::string &line, int want, std::string &out) {
     std::istringstream in(line);
     std::string f;
     int got = 0;
     while (std::getline(in, f, '\t')) {
+        // the line terminator (and stray spaces) end the last field
+        while (!f.empty() && (f.back() == '\n' || f.back() == '\r' || f.back() == ' ')) f.pop_back();
         if (f.empty()) continue;
         if (got == want) { out = f; return true; }
         got += 1;

@@ -118,3 +118,11 @@ def test_strain_program_sort_and_flags(tmp_path, bins, oracle):
     (tmp_path / "bad.disp").write_text("x\ty\n1\t2\n")
     r = U.run([bins / "strain", "bad.disp"], tmp_path)
     assert ".disp file lacks proper header" in r.stdout
+
+
+def test_file_name_at_end_of_line(tmp_path, bins, case):
+    d = U.write_run(tmp_path, case.ref, case.tar, case.points, _params(case))
+    text = (d / "dvc_in").read_text().replace("ref.raw\t\t### comment", "ref.raw")
+    (d / "dvc_in").write_text(text)
+    r = U.run([bins / "dvc", "dvc_in"], tmp_path)
+    assert "reference_filename\tref.raw\n" in (d / "result.stat").read_text(), r.stdout
