@@ -78,6 +78,7 @@ def load() -> C.CDLL:
         "dvc_strain_fit": (I, [VP, P(D), P(D), P(I), P(D), P(I), I, I, I, D, I, I,
                                P(D), P(I), P(D), P(D), P(D), P(D)]),
         "dvc_sort_order_neighbors": (I, [P(D), I, P(D), P(I), P(I), I]),
+        "dvc_knn_order_neighbors": (I, [P(D), I, P(D), I, P(I), P(I), I]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -257,6 +258,21 @@ class Context:
                                             float(objt), int(refill), int(planar), _dp(o["disp"]), _ip(o["npts"]),
                                             _dp(o["swr"]), _dp(o["E"]), _dp(o["L"]), _dp(o["DG"])))
         return o
+
+
+def knn_order_neighbors(points: np.ndarray, start=None, k: int = 75, nthreads: int = 0):
+    """Order and k nearest neighbours through a grid (ties in index order)."""
+    lib = load()
+    p = np.ascontiguousarray(points, dtype=np.float64)
+    n = p.shape[0]
+    s = np.ascontiguousarray(p[0] if start is None else start, dtype=np.float64)
+    k = min(k, n)
+    order = np.zeros(n, np.int32)
+    neigh = np.zeros((n, k), np.int32)
+    rc = lib.dvc_knn_order_neighbors(_dp(p), n, _dp(s), k, _ip(order), _ip(neigh), nthreads)
+    if rc != k:
+        raise DvcError(rc, "knn_order_neighbors failed")
+    return order, neigh
 
 
 def sort_order_neighbors(points: np.ndarray, start=None, nthreads: int = 0):

@@ -597,4 +597,111 @@ int dvc_sort_order_neighbors(const double *pts, int32_t n, const double start[3]
     return k;
 }
 
+// k nearest neighbours through a uniform grid of cells: exact k-NN with ties broken by
+// point index (the order std::stable_sort gives), O(n k) instead of the full O(n^2 log n)
+// sort per point.  Identical to dvc_sort_order_neighbors whenever no two candidate
+// distances of a point are equal; on lattices, equal distances appear in index order.
+// The processing order is the same full sort of distances from `start`.
+int dvc_knn_order_neighbors(const double *pts, int32_t n, const double start[3], int32_t k, int32_t *order,
+                            int32_t *neigh, int32_t nthreads) {
+    if (n < 1 || !pts || !start || !order || !neigh || k < 1) return DVC_ERR_ARG;
+    if (k > n) k = n;
+    for (int i = 0; i < 3 * n; i++)
+        if (!std::isfinite(pts[i])) return DVC_ERR_ARG;
+    {
+        std::vector<DualSort> idx(n);
+        for (int i = 0; i < n; i++) { idx[i].index = i; idx[i].value = pdist(start, &pts[3 * i]); }
+        std::sort(idx.begin(), idx.end(), sortByValue);
+        for (int i = 0; i < n; i++) order[i] = idx[i].index;
+    }
+    // grid: about 8 points per cell, at most 2^21 cells
+    double lo[3], hi[3];
+    for (int q = 0; q < 3; q++) { lo[q] = pts[q]; hi[q] = pts[q]; }
+    for (int i = 1; i < n; i++)
+        for (int q = 0; q < 3; q++) { lo[q] = std::min(lo[q], pts[3 * i + q]); hi[q] = std::max(hi[q], pts[3 * i + q]); }
+    double ext[3], vol = 1.0;
+    int flat_axes = 0;
+    for (int q = 0; q < 3; q++) {
+        ext[q] = hi[q] - lo[q];
+        if (ext[q] > 0.0) vol *= ext[q]; else flat_axes += 1;
+    }
+    double h = std::pow(vol * 8.0 / (double)n, 1.0 / (double)(3 - std::min(flat_axes, 2)));
+    if (!(h > 0.0) || !std::isfinite(h)) h = 1.0;
+    long long dims[3];
+    for (;;) {
+        long long cells = 1;
+        for (int q = 0; q < 3; q++) { dims[q] = (long long)(ext[q] / h) + 1; cells *= dims[q]; }
+        if (cells <= (1ll << 21)) break;
+        h *= 1.25;
+    }
+    auto cell_of = [&](const double *p, long long c[3]) {
+        for (int q = 0; q < 3; q++) {
+            long long v = (long long)((p[q] - lo[q]) / h);
+            c[q] = v < 0 ? 0 : (v >= dims[q] ? dims[q] - 1 : v);
+        }
+    };
+    const long long ncell = dims[0] * dims[1] * dims[2];
+    std::vector<int32_t> cstart(ncell + 1, 0), cpts(n);
+    std::vector<long long> pc(n);
+    for (int i = 0; i < n; i++) {
+        long long c[3];
+        cell_of(&pts[3 * i], c);
+        pc[i] = (c[2] * dims[1] + c[1]) * dims[0] + c[0];
+        cstart[pc[i] + 1] += 1;
+    }
+    for (long long c = 0; c < ncell; c++) cstart[c + 1] += cstart[c];
+    {
+        std::vector<int32_t> fill(cstart.begin(), cstart.end() - 1);
+        for (int i = 0; i < n; i++) cpts[fill[pc[i]]++] = i;
+    }
+    int nt = nthreads > 0 ? nthreads : (int)std::thread::hardware_concurrency();
+    if (nt < 1) nt = 1;
+    if (nt > n) nt = n;
+    const long long maxdim = std::max(dims[0], std::max(dims[1], dims[2]));
+    auto work = [&](int t) {
+        std::vector<DualSort> cand;
+        auto better = [](const DualSort &a, const DualSort &b) {
+            return a.value < b.value || (a.value == b.value && a.index < b.index);
+        };
+        for (int i = t; i < n; i += nt) {
+            const double *pi = &pts[3 * i];
+            long long c[3];
+            cell_of(pi, c);
+            cand.clear();
+            for (long long r = 0;; r++) {
+                // shell of cells at Chebyshev distance r from the query's cell
+                for (long long z = c[2] - r; z <= c[2] + r; z++) {
+                    if (z < 0 || z >= dims[2]) continue;
+                    for (long long y = c[1] - r; y <= c[1] + r; y++) {
+                        if (y < 0 || y >= dims[1]) continue;
+                        const bool yz_edge = (z == c[2] - r || z == c[2] + r || y == c[1] - r || y == c[1] + r);
+                        for (long long x = c[0] - r; x <= c[0] + r; x += (yz_edge || r == 0) ? 1 : 2 * r) {
+                            if (x < 0 || x >= dims[0]) continue;
+                            const long long id = (z * dims[1] + y) * dims[0] + x;
+                            for (int32_t s = cstart[id]; s < cstart[id + 1]; s++) {
+                                const int j = cpts[s];
+                                cand.push_back({j, pdist(pi, &pts[3 * j])});
+                            }
+                        }
+                    }
+                }
+                if ((int)cand.size() >= k) {
+                    std::nth_element(cand.begin(), cand.begin() + (k - 1), cand.end(), better);
+                    // every point outside the searched cells is at least r*h away
+                    if (cand[k - 1].value < (double)r * h || r > maxdim) break;
+                } else if (r > maxdim) {
+                    break;
+                }
+            }
+            std::partial_sort(cand.begin(), cand.begin() + k, cand.end(), better);
+            for (int j = 0; j < k; j++) neigh[(size_t)i * k + j] = cand[j].index;
+        }
+    };
+    std::vector<std::thread> th;
+    for (int t = 1; t < nt; t++) th.emplace_back(work, t);
+    work(0);
+    for (auto &x : th) x.join();
+    return k;
+}
+
 }  // extern "C"

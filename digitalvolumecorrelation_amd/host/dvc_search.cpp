@@ -120,6 +120,13 @@ dvc_run_params run_params(const RunControl *run) {
 
 // ------------------------------------------------------------------ DataCloud
 
+bool grid_neighbours(long long n) {
+    const char *e = std::getenv("DVC_NEIGHBOURS");
+    if (e && std::string(e) == "exact") return false;
+    if (e && std::string(e) == "grid") return true;
+    return n > kExactNeighbourLimit;
+}
+
 void DataCloud::sort_order_neighbors(Point start) {
     const int n = (int)points.size();
     std::vector<double> p((size_t)3 * n);
@@ -127,8 +134,9 @@ void DataCloud::sort_order_neighbors(Point start) {
     const int k = n < 75 ? n : 75;
     std::vector<int32_t> ord(n), nb((size_t)n * k);
     const double s[3] = {start.x(), start.y(), start.z()};
-    int rc = dvc_sort_order_neighbors(p.data(), n, s, ord.data(), nb.data(), 0);
-    if (rc != k) throw Device_Fail{rc, "dvc_sort_order_neighbors"};
+    int rc = grid_neighbours(n) ? dvc_knn_order_neighbors(p.data(), n, s, k, ord.data(), nb.data(), 0)
+                                : dvc_sort_order_neighbors(p.data(), n, s, ord.data(), nb.data(), 0);
+    if (rc != k) throw Device_Fail{rc, "neighbour tables"};
     order.assign(ord.begin(), ord.end());
     neigh.assign(n, std::vector<int>(k));
     for (int i = 0; i < n; i++)

@@ -93,6 +93,12 @@ public:
     std::vector<std::vector<ResultRecord>> results;
 };
 
+// Neighbour tables: the reference's full sort per point (exact tie order, O(n^2 log n))
+// up to kExactNeighbourLimit points, the grid k-NN (ties in index order, O(n k)) above;
+// DVC_NEIGHBOURS=exact or =grid chooses explicitly.
+constexpr long long kExactNeighbourLimit = 20000;
+bool grid_neighbours(long long n);
+
 // A process-wide device context for the objects below (device from DVC_DEVICE, else 0).
 dvc_ctx *shared_context();
 

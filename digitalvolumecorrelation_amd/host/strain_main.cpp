@@ -18,6 +18,7 @@
 
 #include "../../include/dvc_amd.h"
 #include "dvc_io.h"
+#include "dvc_search.h"
 
 namespace {
 
@@ -137,7 +138,11 @@ int main(int argc, char *argv[]) {
         k = n < kNdpMax ? n : kNdpMax;
         std::vector<int32_t> order(n);
         nb.resize((size_t)n * k);
-        if (dvc_sort_order_neighbors(pts.data(), n, pts.data(), order.data(), nb.data(), 0) != k) return 1;
+        const int rc = dvc::grid_neighbours(n) ? dvc_knn_order_neighbors(pts.data(), n, pts.data(), k, order.data(),
+                                                                         nb.data(), 0)
+                                               : dvc_sort_order_neighbors(pts.data(), n, pts.data(), order.data(),
+                                                                          nb.data(), 0);
+        if (rc != k) return 1;
         write_sort_file(base, nb.data(), n, k);
     }
     if (yz) std::cout << "using 2d yz coordinate plane analsis" << std::endl;

@@ -184,6 +184,14 @@ int dvc_strain_fit(dvc_ctx *ctx, const double *pts, const double *disp, const in
 int dvc_sort_order_neighbors(const double *pts, int32_t n, const double start[3],
                              int32_t *order, int32_t *neigh, int32_t nthreads);
 
+/* The same tables for large clouds through a uniform grid: the order as above, and the
+ * k (<= n) nearest neighbours of every point with equal distances in point-index order
+ * (std::stable_sort order).  Identical to dvc_sort_order_neighbors when a point's
+ * candidate distances are distinct; O(n k) rather than the reference's O(n^2 log n)
+ * (DataCloud.cpp:95-140).  Returns k. */
+int dvc_knn_order_neighbors(const double *pts, int32_t n, const double start[3], int32_t k,
+                            int32_t *order, int32_t *neigh, int32_t nthreads);
+
 #ifdef __cplusplus
 }
 #endif
