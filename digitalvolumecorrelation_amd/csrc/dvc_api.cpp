@@ -55,6 +55,8 @@ struct dvc_ctx {
     int chosen_threads = 256;
     std::vector<char> searched;
     std::vector<int> level_off;      // dependency levels (dvc_geometry.h)
+    float *d_rvals = nullptr;        // reference samples of every searched point (or null)
+    RefStats *d_rstat = nullptr;
     // the level launches of a prepared search, captured once and replayed
     int use_graph = 1;
     hipGraph_t graph = nullptr;
@@ -93,10 +95,16 @@ void drop_graph(dvc_ctx *c) {
     c->graph = nullptr;
 }
 
+// HBM for the reference sub-volumes computed ahead of the level chain
+constexpr size_t kRefBudgetBytes = (size_t)8 << 30;
+
 void free_plan(dvc_ctx *c) {
     drop_graph(c);
     hipFree(c->d_pts); hipFree(c->d_sph); hipFree(c->d_order); hipFree(c->d_pos);
     hipFree(c->d_neigh); hipFree(c->d_levels); hipFree(c->d_res);
+    hipFree(c->d_rvals); hipFree(c->d_rstat);
+    c->d_rvals = nullptr;
+    c->d_rstat = nullptr;
     c->d_pts = c->d_sph = nullptr;
     c->d_order = c->d_pos = c->d_neigh = c->d_levels = nullptr;
     c->d_res = nullptr;
@@ -390,6 +398,12 @@ int dvc_search_prepare(dvc_ctx *ctx, const dvc_run_params *rc, const double *poi
     dependency_levels(order, pos, neigh, npts, k, N, level_pts, level_off);
     if (int e = dev_alloc(ctx, &ctx->d_levels, level_pts.size())) return e;
     if (int e = dev_alloc(ctx, &ctx->d_res, (size_t)npts)) return e;
+    // reference sub-volumes of all searched points, computed ahead of the levels when
+    // they fit a budget of HBM; otherwise each point computes its own in the level chain
+    if ((size_t)N * (size_t)nsamp_pad * sizeof(float) <= kRefBudgetBytes) {
+        if (int e = dev_alloc(ctx, &ctx->d_rvals, (size_t)N * nsamp_pad)) return e;
+        if (int e = dev_alloc(ctx, &ctx->d_rstat, (size_t)N)) return e;
+    }
     HIPCHK(hipMemcpyAsync(ctx->d_pts, points, sizeof(double) * 3 * npts, hipMemcpyHostToDevice, ctx->stream));
     if (!sph.empty()) HIPCHK(hipMemcpyAsync(ctx->d_sph, sph.data(), sizeof(double) * sph.size(), hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(hipMemcpyAsync(ctx->d_order, order, sizeof(int) * npts, hipMemcpyHostToDevice, ctx->stream));
@@ -422,10 +436,14 @@ static int enqueue_search(dvc_ctx *ctx, hipStream_t st) {
     if (((uintptr_t)a.vol[0] | (uintptr_t)a.vol[1]) & 3) a.gather32 = 0;   // attached, unaligned
     a.sph = ctx->d_sph; a.pts = ctx->d_pts; a.order = ctx->d_order; a.pos = ctx->d_pos; a.neigh = ctx->d_neigh;
     a.res = ctx->d_res;
+    a.rvals = ctx->d_rvals;
+    a.rstat = ctx->d_rstat;
     int bt = ctx->chosen_threads;
     size_t lds = search_lds_bytes(a.nsamp_pad, bt);
     if (launch_zero_int((int *)ctx->d_res, (long long)ctx->npts * (long long)(sizeof(dvc_point_result) / 4), st) != 0)
         return fail(ctx, DVC_ERR_HIP, "zero launch");
+    if (a.rvals && launch_reference(a, bt, (size_t)a.nsamp_pad * sizeof(float), st) != 0)
+        return fail(ctx, DVC_ERR_HIP, "reference kernel launch failed");
     const int depth = (int)ctx->level_off.size() - 1;
     for (int L = 0; L < depth; L++) {
         int cnt = ctx->level_off[L + 1] - ctx->level_off[L];

@@ -9,6 +9,11 @@ namespace dvc {
 
 constexpr int kMaxNeigh = 128;       // neighbour slots read per point (reference: 75)
 
+// Sums of one point's reference sub-volume (reference_kernel -> search_kernel).
+struct RefStats {
+    double rmean, rbmean, R2, Src, Rn2;
+};
+
 // Arguments of the search kernel.  Geometry mirrors Search::Search
 // (Search.cpp:24-82) and Interpolate::Interpolate (Interpolate.cpp:34-56).
 struct SearchArgs {
@@ -29,6 +34,8 @@ struct SearchArgs {
     const int *pos;                  // position of each point in order[] (npts if unsearched)
     const int *neigh;                // [npts*k]
     dvc_point_result *res;           // [npts]
+    float *rvals;                    // [nproc * nsamp_pad] reference samples by position, or null
+    RefStats *rstat;                 // [nproc]
 };
 
 struct StrainArgs {
@@ -53,6 +60,7 @@ int launch_interp(const uint8_t *vol, int bpv, int W, int H, int T, const int b0
 int launch_objective(int kind, const double *ref, const double *tar, int n, double *obj_out,
                      double *residual, void *stream);
 int launch_strain(const StrainArgs &a, void *stream);
+int launch_reference(const SearchArgs &a, int block_threads, size_t lds_bytes, void *stream);
 int launch_zero_int(int *p, long long n, void *stream);
 int launch_swap16(uint16_t *p, long long n, void *stream);
 

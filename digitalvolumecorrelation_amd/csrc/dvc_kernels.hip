@@ -1120,12 +1120,12 @@ __device__ void gn_step(const SearchArgs &a, PtShared &S, const double *V) {
 
 // ------------------------------------------------------------- the search
 
+// Point set-up (Search.cpp:98-119, 252-258): cloud geometry, the reference box on the
+// point, its range rule and the integer bias.  Returns false (uniformly) on Range_Fail.
 template <int BPV, int B>
-__device__ void point_search(const SearchArgs &a, int p, int n, PtShared &S, float *rcb, float *tcb,
-                             float *stage, double *red, double *V) {
+__device__ bool point_setup(const SearchArgs &a, int n, PtShared &S) {
 #pragma clang fp contract(off)
-    const int tid = threadIdx.x, lane = tid & 63;
-    const int ns = a.nsamp;
+    const int tid = threadIdx.x;
     // ---- cloud geometry and reference box (Search.cpp:98-119, 252-258)
     if (tid == 0) {
 #pragma clang fp contract(off)
@@ -1163,11 +1163,16 @@ __device__ void point_search(const SearchArgs &a, int p, int n, PtShared &S, flo
         }
     }
     __syncthreads();
-    if (S.fail) {
-        if (tid == 0) S.status = DVC_RANGE_FAIL;
-        return;
-    }
-    // ---- reference sub-volume
+    return S.fail == 0;
+}
+
+// The reference sub-volume (Search.cpp:252-258): sample values at the stable cloud,
+// centred, left in rcb[], and their sums in S.
+template <int BPV, int B>
+__device__ void reference_pass(const SearchArgs &a, PtShared &S, float *rcb, double *red) {
+#pragma clang fp contract(off)
+    const int tid = threadIdx.x;
+    const int ns = a.nsamp;
     {
         const VolView vr = {a.vol[0], a.W, (long long)a.W * a.H};
         const BoxView bx = {{S.b0[0], S.b0[1], S.b0[2]}, {a.box[0], a.box[1], a.box[2]}};
@@ -1236,6 +1241,30 @@ __device__ void point_search(const SearchArgs &a, int p, int n, PtShared &S, flo
             S.rmean = rmean; S.rbmean = rbmean;
             S.R2 = S.sums[1]; S.Src = S.sums[2]; S.Rn2 = S.sums[3];
         }
+    }
+}
+
+template <int BPV, int B>
+__device__ void point_search(const SearchArgs &a, int p, int n, PtShared &S, float *rcb, float *tcb,
+                             float *stage, double *red, double *V) {
+#pragma clang fp contract(off)
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int ns = a.nsamp;
+    if (!point_setup<BPV, B>(a, n, S)) {
+        if (tid == 0) S.status = DVC_RANGE_FAIL;
+        return;
+    }
+    if (a.rvals) {
+        // computed for every point before the levels run (reference_kernel)
+        const float *rv = a.rvals + (size_t)p * a.nsamp_pad;
+        for (int j = tid; j < ns; j += B) rcb[j] = rv[j];
+        if (tid == 0) {
+            const RefStats &st = a.rstat[p];
+            S.rmean = st.rmean; S.rbmean = st.rbmean;
+            S.R2 = st.R2; S.Src = st.Src; S.Rn2 = st.Rn2;
+        }
+    } else {
+        reference_pass<BPV, B>(a, S, rcb, red);
     }
     // ---- starting_param: neighbours searched earlier.  Launches run the dependency
     // levels in order, so every such neighbour's result is already in memory.
@@ -1344,6 +1373,30 @@ __device__ void point_search(const SearchArgs &a, int p, int n, PtShared &S, flo
     }
 }
 
+// The reference sub-volumes of all searched points at once: they depend only on the
+// point, not on its neighbours, so they are taken out of the level chain.  Workgroup p
+// handles the point at position p of the order.
+template <int BPV, int B>
+__global__ void __launch_bounds__(B) reference_kernel(SearchArgs a) {
+    DVC_DYN_LDS(float, dyn);
+    float *rcb = dyn;
+    DVC_SHARED(PtShared, S);
+    DVC_SHARED(double[(B / 64) * 9], red);
+    const int tid = threadIdx.x;
+    const int p = blockIdx.x;
+    const int n = a.order[p];
+    if (!point_setup<BPV, B>(a, n, S)) return;
+    reference_pass<BPV, B>(a, S, rcb, red);
+    __syncthreads();
+    float *rv = a.rvals + (size_t)p * a.nsamp_pad;
+    for (int j = tid; j < a.nsamp; j += B) rv[j] = rcb[j];
+    if (tid == 0) {
+        RefStats &st = a.rstat[p];
+        st.rmean = S.rmean; st.rbmean = S.rbmean;
+        st.R2 = S.R2; st.Src = S.Src; st.Rn2 = S.Rn2;
+    }
+}
+
 // One dependency level of the point loop: workgroup b searches point level_pts[b].
 // All neighbours a point depends on belong to earlier levels (earlier launches).
 template <int BPV, int B>
@@ -1392,12 +1445,21 @@ static int launch_search_t(const SearchArgs &a, const int *level_pts, int grid, 
 // (direct or replayed from a graph) never change function attributes.
 template <int BPV, int B>
 static int lds_limit_t(size_t lds_total, int *) {
-    auto k = search_kernel<BPV, B>;
-    hipFuncAttributes fa;
-    if (hipFuncGetAttributes(&fa, (const void *)k) != hipSuccess) return -1;
-    if (fa.sharedSizeBytes >= lds_total) return -1;
-    int dyn = (int)(lds_total - fa.sharedSizeBytes);
-    return hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, dyn) == hipSuccess ? 0 : -1;
+    const void *ks[2] = {(const void *)search_kernel<BPV, B>, (const void *)reference_kernel<BPV, B>};
+    for (const void *k : ks) {
+        hipFuncAttributes fa;
+        if (hipFuncGetAttributes(&fa, k) != hipSuccess) return -1;
+        if (fa.sharedSizeBytes >= lds_total) return -1;
+        int dyn = (int)(lds_total - fa.sharedSizeBytes);
+        if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, dyn) != hipSuccess) return -1;
+    }
+    return 0;
+}
+
+template <int BPV, int B>
+static int launch_reference_t(const SearchArgs &a, size_t lds, hipStream_t st) {
+    hipLaunchKernelGGL((reference_kernel<BPV, B>), dim3(a.nproc), dim3(B), lds, st, a);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 template <int BPV, int B>
@@ -1444,6 +1506,23 @@ int search_static_lds(int bpv, int block_threads, size_t *bytes) {
     DVC_DISPATCH(2, 256, static_lds_t)(bytes, nullptr);
     DVC_DISPATCH(2, 512, static_lds_t)(bytes, nullptr);
     DVC_DISPATCH(2, 1024, static_lds_t)(bytes, nullptr);
+    return -1;
+}
+
+int launch_reference(const SearchArgs &a, int block_threads, size_t lds, void *stream) {
+    if (a.nproc <= 0) return 0;
+    hipStream_t st = (hipStream_t)stream;
+    int bpv = a.bpv;
+    DVC_DISPATCH(1, 64, launch_reference_t)(a, lds, st);
+    DVC_DISPATCH(1, 128, launch_reference_t)(a, lds, st);
+    DVC_DISPATCH(1, 256, launch_reference_t)(a, lds, st);
+    DVC_DISPATCH(1, 512, launch_reference_t)(a, lds, st);
+    DVC_DISPATCH(1, 1024, launch_reference_t)(a, lds, st);
+    DVC_DISPATCH(2, 64, launch_reference_t)(a, lds, st);
+    DVC_DISPATCH(2, 128, launch_reference_t)(a, lds, st);
+    DVC_DISPATCH(2, 256, launch_reference_t)(a, lds, st);
+    DVC_DISPATCH(2, 512, launch_reference_t)(a, lds, st);
+    DVC_DISPATCH(2, 1024, launch_reference_t)(a, lds, st);
     return -1;
 }
 

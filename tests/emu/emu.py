@@ -34,7 +34,7 @@ def load():
     return _lib
 
 
-def search(params, ref, tar, points, order, neigh, nproc=0, threads=64, gather32=True):
+def search(params, ref, tar, points, order, neigh, nproc=0, threads=64, gather32=True, precompute=False):
     from digitalvolumecorrelation_amd._lib import RESULT_DTYPE
     lib = load()
     ref = np.ascontiguousarray(ref)
@@ -46,7 +46,7 @@ def search(params, ref, tar, points, order, neigh, nproc=0, threads=64, gather32
     out = np.zeros(len(p), dtype=RESULT_DTYPE)
     rc = lib.emu_search(C.byref(params), ref.ctypes.data_as(C.c_void_p), tar.ctypes.data_as(C.c_void_p),
                         ref.dtype.itemsize, W, H, T, p.ctypes.data_as(C.c_void_p), len(p), o.ctypes.data_as(C.c_void_p),
-                        nproc, nb.ctypes.data_as(C.c_void_p), nb.shape[1], threads, -1 if gather32 else 0,
+                        nproc, nb.ctypes.data_as(C.c_void_p), nb.shape[1], threads, -1 if gather32 else 0, int(precompute),
                         out.ctypes.data_as(C.c_void_p))
     if rc != 0:
         raise RuntimeError(f"emulated search error flags {rc}")

@@ -23,7 +23,7 @@ static void run_search(const SearchArgs &a, size_t lds, const std::vector<int> &
 
 extern "C" int emu_search(const dvc_run_params *rc, const void *ref, const void *tar, int bpv, int W, int H, int T,
                           const double *pts, int npts, const int *order, int nproc, const int *neigh, int k,
-                          int block_threads, int gather_mode, dvc_point_result *out) {
+                          int block_threads, int gather_mode, int precompute, dvc_point_result *out) {
     Geometry G = make_geometry(*rc);
     int N = npts;
     if (nproc > 1 && nproc < npts) N = nproc;
@@ -51,6 +51,24 @@ extern "C" int emu_search(const dvc_run_params *rc, const void *ref, const void 
     a.sph = sph.data(); a.pts = pts; a.order = order; a.pos = pos.data(); a.neigh = neigh;
     a.res = res.data();
     size_t lds = (size_t)a.nsamp_pad * 8 + (size_t)(block_threads / 64) * 64 * 17 * 4;
+    std::vector<float> rvals;
+    std::vector<RefStats> rstat;
+    if (precompute) {
+        // reference sub-volumes of all searched points first (reference_kernel)
+        rvals.assign((size_t)N * a.nsamp_pad, 0.0f);
+        rstat.assign(N, RefStats());
+        a.rvals = rvals.data();
+        a.rstat = rstat.data();
+        emu::grid_dim.x = N;
+        for (int b0 = 0; b0 < N; b0 += 8) {
+            int nb = N - b0 < 8 ? N - b0 : 8;
+            if (bpv == 1 && block_threads == 64) emu::run_grid(b0, nb, 64, (size_t)a.nsamp_pad * 4, [&] { reference_kernel<1, 64>(a); });
+            else if (bpv == 1 && block_threads == 128) emu::run_grid(b0, nb, 128, (size_t)a.nsamp_pad * 4, [&] { reference_kernel<1, 128>(a); });
+            else if (bpv == 2 && block_threads == 64) emu::run_grid(b0, nb, 64, (size_t)a.nsamp_pad * 4, [&] { reference_kernel<2, 64>(a); });
+            else if (bpv == 2 && block_threads == 128) emu::run_grid(b0, nb, 128, (size_t)a.nsamp_pad * 4, [&] { reference_kernel<2, 128>(a); });
+            else return -1;
+        }
+    }
     if (bpv == 1 && block_threads == 64) run_search<1, 64>(a, lds, lp, lo);
     else if (bpv == 1 && block_threads == 256) run_search<1, 256>(a, lds, lp, lo);
     else if (bpv == 1 && block_threads == 128) run_search<1, 128>(a, lds, lp, lo);

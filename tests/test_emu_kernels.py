@@ -20,12 +20,12 @@ def built():
     emu.build()
 
 
-def _cmp(oracle, case, params, threads=64, nproc=0, gather32=True):
+def _cmp(oracle, case, params, threads=64, nproc=0, gather32=True, precompute=False):
     order, neigh = D.sort_order_neighbors(case.points, start=case.start)
     run = oracle.make_run(shape=case.ref.shape, bytes_per=case.ref.dtype.itemsize, **params)
     o, _, _ = oracle.dvc_run(run, case.ref, case.tar, case.points, order, neigh, nproc=nproc, nthreads=4)
     g = emu.search(D.make_params(**params), case.ref, case.tar, case.points, order, neigh, nproc=nproc, threads=threads,
-                   gather32=gather32)
+                   gather32=gather32, precompute=precompute)
     assert np.array_equal(g["status"], o["status"])
     assert np.array_equal(g["iters"], o["iters"])
     good = o["status"] == 0
@@ -49,9 +49,10 @@ def test_odd_row_width_u8(oracle):
     _cmp(oracle, case, dict(case.params, subvol_size=21, disp_max=4))
 
 
-def test_affine_sphere_u16_12dof(oracle):
+@pytest.mark.parametrize("precompute", [False, True])
+def test_affine_sphere_u16_12dof(oracle, precompute):
     case = synth.config2(size=72, n=2, spacing=10, npts=600, diameter=21)
-    g, o = _cmp(oracle, case, dict(case.params, disp_max=5), threads=128)
+    g, o = _cmp(oracle, case, dict(case.params, disp_max=5), threads=128, precompute=precompute)
     assert np.all(o["status"] == 0)
 
 
@@ -111,7 +112,7 @@ def test_12dof_range_failures(oracle):
     (the moving cloud leaving the active box with disp_max 1), where the bounding box
     comes from the speculative sample pass."""
     case = synth.config2(size=72, n=3, spacing=20, npts=600, diameter=21)
-    g, o = _cmp(oracle, case, dict(case.params, disp_max=5), threads=128)
+    g, o = _cmp(oracle, case, dict(case.params, disp_max=5), threads=128, precompute=True)
     assert (o["status"] == D.RANGE_FAIL).any() and (o["status"] == D.POINT_GOOD).any()
     g, o = _cmp(oracle, case, dict(case.params, disp_max=1, rigid_trans=(0.0, 0.0, 0.0)), threads=128)
     assert np.all(o["status"] == D.RANGE_FAIL) and np.all(o["iters"] >= 3)
