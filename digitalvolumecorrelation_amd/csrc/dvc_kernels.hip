@@ -471,8 +471,8 @@ struct PtShared {
     double rmean, R2, Rn2, Src;
     float fbias;                     // integer intensity subtracted before interpolation sums
     double rbmean;                   // mean of the biased reference samples
-    double tmean, N2, Str, T2, Strr, Szssd;
-    double obj, csum, omin;
+    double tmean, N2, Str, T2, Strr;
+    double obj, omin;
     int gmin[3];
     double sums[8];
 };
@@ -650,7 +650,7 @@ __device__ __forceinline__ void set_warp(PtShared &S, int ndof) {
 // are left in V (16x16, FP64).  Returns false on Range_Fail.
 template <int BPV, int B>
 __device__ bool evaluate(const SearchArgs &a, PtShared &S, int ndof, bool need_j,
-                         float *rcb, float *tcb, float *stage, double *red, double *V) {
+                         float *rcb, float *stage, double *red, double *V) {
     constexpr int NW = B / 64;
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int n = a.nsamp;
@@ -690,7 +690,10 @@ __device__ bool evaluate(const SearchArgs &a, PtShared &S, int ndof, bool need_j
     const bool jrow = need_j;
     double acc64[4] = {0.0, 0.0, 0.0, 0.0};
     float *wst = stage + wid * 64 * 17;
-    double s_tc = 0.0, s_d2 = 0.0, s_ad = 0.0;
+    // one pass of sums: t, then t^2 and t r (zero-mean and normalised objectives) or
+    // (t - r)^2 and |t - r| (SSD, SAD); products of FP32 values are exact in FP64
+    const bool centred = a.obj == DVC_OBJ_ZSSD || a.obj == DVC_OBJ_NSSD || a.obj == DVC_OBJ_ZNSSD;
+    double s_tc = 0.0, s_x1 = 0.0, s_x2 = 0.0;
     double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
 
     // moving-cloud position of sample j (and its stable position)
@@ -711,11 +714,16 @@ __device__ bool evaluate(const SearchArgs &a, PtShared &S, int ndof, bool need_j
         if (ok) {
             float tc = (float)((double)t - rbmean);
             float rc = rcb[j];
-            tcb[j] = tc;
-            double d = (double)tc - (double)rc;
-            s_tc += (double)tc;
-            s_d2 += d * d;
-            s_ad += fabs(d);
+            const double tcd = (double)tc, rcd = (double)rc;
+            s_tc += tcd;
+            if (centred) {
+                s_x1 += tcd * tcd;
+                s_x2 += tcd * rcd;
+            } else {
+                const double d = tcd - rcd;
+                s_x1 += d * d;
+                s_x2 += fabs(d);
+            }
             if (jrow) {
                 v[0] = gx; v[1] = gy; v[2] = gz;
                 if (ndof > 3) {
@@ -822,7 +830,7 @@ __device__ bool evaluate(const SearchArgs &a, PtShared &S, int ndof, bool need_j
     // block reduction of the sums and (merged) of the moving bounding box, then the
     // range rule (Interpolate.cpp:474-481 on the box of FloatingCloud::resize_bbox)
     {
-        double v3[3] = {wave_sum(s_tc), wave_sum(s_d2), wave_sum(s_ad)};
+        double v3[3] = {wave_sum(s_tc), wave_sum(s_x1), wave_sum(s_x2)};
         if (merged)
             for (int q = 0; q < 3; q++) { mn[q] = wave_min(mn[q]); mx[q] = wave_max(mx[q]); }
         if (lane == 0) {
@@ -861,59 +869,37 @@ __device__ bool evaluate(const SearchArgs &a, PtShared &S, int ndof, bool need_j
         }
         __syncthreads();
     }
-    const int kind = a.obj;
-    const double nn = (double)n;
-    if (kind == DVC_OBJ_SAD || kind == DVC_OBJ_SSD) {
-        if (tid == 0) {
-            S.tmean = S.sums[0] / nn;
-            S.obj = (kind == DVC_OBJ_SAD) ? S.sums[2] : S.sums[1];
+    // objective value and the moments the step needs (ObjectiveFunctions.cpp:24-263):
+    // the centred sums of the reference's second and third passes follow from the one
+    // pass above, e.g. sum (t - tm)^2 = sum t^2 - tm sum t, and the normalised
+    // objectives are (2 - 2 rho)/4 and (2 - 2 rho)/2 with rho the normalised correlation
+    if (tid == 0) {
+        const int kind = a.obj;
+        const double nn = (double)n;
+        const double st = S.sums[0], sx1 = S.sums[1], sx2 = S.sums[2];
+        const double tm = st / nn;
+        S.tmean = tm;
+        if (kind == DVC_OBJ_SAD) {
+            S.obj = sx2;
+        } else if (kind == DVC_OBJ_SSD) {
+            S.obj = sx1;
+        } else {
+            const double rm = S.rmean;
+            const double N2 = sx1 - tm * st;
+            const double Str = sx2 - tm * S.Src;
+            const double T2 = sx1 + 2.0 * rm * st + nn * rm * rm;
+            const double Strr = sx2 + rm * st + rm * S.Src + nn * rm * rm;
+            S.N2 = N2; S.Str = Str; S.T2 = T2; S.Strr = Strr;
+            if (kind == DVC_OBJ_ZSSD) {
+                S.obj = N2 - 2.0 * Str + S.R2;
+            } else if (kind == DVC_OBJ_ZNSSD) {
+                S.obj = (2.0 - 2.0 * Str / (sqrt(N2) * sqrt(S.R2))) / 4.0;
+            } else {
+                S.obj = (2.0 - 2.0 * Strr / (sqrt(T2) * sqrt(S.Rn2))) / 2.0;
+            }
         }
-        __syncthreads();
-        return true;
     }
-    const double tm = S.sums[0] / nn;
-    // pass B: centred sums
-    {
-        double b[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-        for (int j = tid; j < n; j += B) {
-            double tc = (double)tcb[j], rc = (double)rcb[j];
-            double tt = tc - tm;
-            double t = tc + S.rmean, r = rc + S.rmean;
-            double dz = tt - rc;
-            b[0] += tt * tt;
-            b[1] += tt * rc;
-            b[2] += t * t;
-            b[3] += t * r;
-            b[4] += dz * dz;
-        }
-        block_sum<B, 5>(b, red, &S.sums[3]);
-    }
-    if (kind == DVC_OBJ_ZSSD) {
-        if (tid == 0) { S.tmean = tm; S.N2 = S.sums[3]; S.obj = S.sums[7]; }
-        __syncthreads();
-        return true;
-    }
-    // pass C: normalised residual sum of squares
-    {
-        double N = sqrt(S.sums[3]), T = sqrt(S.sums[5]);
-        double R = sqrt(S.R2), Rn = sqrt(S.Rn2);
-        double c[1] = {0.0};
-        for (int j = tid; j < n; j += B) {
-            double tc = (double)tcb[j], rc = (double)rcb[j];
-            double d;
-            if (kind == DVC_OBJ_ZNSSD) d = ((tc - tm) / N) - (rc / R);
-            else d = (tc + S.rmean) / T - (rc + S.rmean) / Rn;
-            c[0] += d * d;
-        }
-        block_sum<B, 1>(c, red, &S.csum);
-        double out[1] = {S.csum};
-        if (tid == 0) {
-            S.tmean = tm;
-            S.N2 = S.sums[3]; S.Str = S.sums[4]; S.T2 = S.sums[5]; S.Strr = S.sums[6];
-            S.obj = (kind == DVC_OBJ_ZNSSD) ? out[0] / 4.0 : out[0] / 2.0;
-        }
-        __syncthreads();
-    }
+    __syncthreads();
     return true;
 }
 
@@ -1245,8 +1231,8 @@ __device__ void reference_pass(const SearchArgs &a, PtShared &S, float *rcb, dou
 }
 
 template <int BPV, int B>
-__device__ void point_search(const SearchArgs &a, int p, int n, PtShared &S, float *rcb, float *tcb,
-                             float *stage, double *red, double *V) {
+__device__ void point_search(const SearchArgs &a, int p, int n, PtShared &S, float *rcb, float *stage,
+                             double *red, double *V) {
 #pragma clang fp contract(off)
     const int tid = threadIdx.x, lane = tid & 63;
     const int ns = a.nsamp;
@@ -1313,7 +1299,7 @@ __device__ void point_search(const SearchArgs &a, int p, int n, PtShared &S, flo
                         S.evalpar[2] = S.par[2] + (double)z * inc;
                     }
                     __syncthreads();
-                    if (!evaluate<BPV, B>(a, S, 3, false, rcb, tcb, stage, red, V)) {
+                    if (!evaluate<BPV, B>(a, S, 3, false, rcb, stage, red, V)) {
                         if (tid == 0) S.status = DVC_RANGE_FAIL;
                         return;
                     }
@@ -1338,7 +1324,7 @@ __device__ void point_search(const SearchArgs &a, int p, int n, PtShared &S, flo
             S.iters += 1;                 // LM_prep_at calls, counted when attempted
         }
         __syncthreads();
-        if (!evaluate<BPV, B>(a, S, nd, true, rcb, tcb, stage, red, V)) {
+        if (!evaluate<BPV, B>(a, S, nd, true, rcb, stage, red, V)) {
             if (tid == 0) S.status = DVC_RANGE_FAIL;
             return;
         }
@@ -1366,7 +1352,7 @@ __device__ void point_search(const SearchArgs &a, int p, int n, PtShared &S, flo
         if (tid == 0)
             for (int q = 0; q < 12; q++) S.evalpar[q] = S.par[q];
         __syncthreads();
-        if (!evaluate<BPV, B>(a, S, nd, false, rcb, tcb, stage, red, V)) {
+        if (!evaluate<BPV, B>(a, S, nd, false, rcb, stage, red, V)) {
             if (tid == 0) S.status = DVC_RANGE_FAIL;
             return;
         }
@@ -1403,15 +1389,14 @@ template <int BPV, int B>
 __global__ void __launch_bounds__(B) search_kernel(SearchArgs a, const int *level_pts) {
     DVC_DYN_LDS(float, dyn);
     float *rcb = dyn;
-    float *tcb = dyn + a.nsamp_pad;
-    float *stage = dyn + 2 * a.nsamp_pad;
+    float *stage = dyn + a.nsamp_pad;
     DVC_SHARED(PtShared, S);
     DVC_SHARED(double[(B / 64) * 9], red);
     DVC_SHARED(double[256], V);
     const int tid = threadIdx.x;
     const int n = level_pts[blockIdx.x];
     const int p = a.pos[n];
-    point_search<BPV, B>(a, p, n, S, rcb, tcb, stage, red, V);
+    point_search<BPV, B>(a, p, n, S, rcb, stage, red, V);
     __syncthreads();
     if (tid == 0) {
         dvc_point_result *r = &a.res[n];
@@ -1430,7 +1415,7 @@ __global__ void __launch_bounds__(B) search_kernel(SearchArgs a, const int *leve
 
 #ifndef DVC_EMU
 size_t search_lds_bytes(int nsamp_pad, int block_threads) {
-    return (size_t)nsamp_pad * 2 * sizeof(float) + (size_t)(block_threads / 64) * 64 * 17 * sizeof(float);
+    return (size_t)nsamp_pad * sizeof(float) + (size_t)(block_threads / 64) * 64 * 17 * sizeof(float);
 }
 
 template <int BPV, int B>
