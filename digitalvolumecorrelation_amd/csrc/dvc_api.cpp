@@ -96,7 +96,7 @@ void drop_graph(dvc_ctx *c) {
 }
 
 // HBM for the reference sub-volumes computed ahead of the level chain
-constexpr size_t kRefBudgetBytes = (size_t)8 << 30;
+constexpr size_t kRefBudgetBytes = (size_t)48 << 30;
 
 void free_plan(dvc_ctx *c) {
     drop_graph(c);
@@ -368,10 +368,13 @@ int dvc_search_prepare(dvc_ctx *ctx, const dvc_run_params *rc, const double *poi
         if (!std::isfinite(points[i])) return fail(ctx, DVC_ERR_ARG, "non-finite point coordinate");
     Geometry G = make_geometry(*rc);
     HIPCHK(hipSetDevice(ctx->device));
-    // threads per point: the requested count, else the largest of 512/256/128/64 whose
-    // LDS (sample values, MFMA staging, point state) fits one workgroup
+    // reference samples of all searched points go to an HBM table ahead of the level
+    // chain when it fits the budget; otherwise each point computes its own into LDS
     const int bpv = ctx->vol[0].bpv;
     int nsamp_pad = (G.subv_num + 3) & ~3;
+    const bool table = (size_t)N * (size_t)nsamp_pad * sizeof(float) <= kRefBudgetBytes;
+    // threads per point: the requested count, else the largest of 512/256/128/64 whose
+    // LDS (MFMA staging, point state, and the samples without the table) fits
     const int auto_cands[4] = {512, 256, 128, 64};
     const int *cands = ctx->block_threads ? &ctx->block_threads : auto_cands;
     const int ncand = ctx->block_threads ? 1 : 4;
@@ -379,7 +382,7 @@ int dvc_search_prepare(dvc_ctx *ctx, const dvc_run_params *rc, const double *poi
     for (int i = 0; i < ncand && bt == 0; i++) {
         size_t st = 0;
         if (search_static_lds(bpv, cands[i], &st) != 0) return fail(ctx, DVC_ERR_HIP, "kernel attributes");
-        if (search_lds_bytes(nsamp_pad, cands[i]) + st <= ctx->lds_max) bt = cands[i];
+        if (search_lds_bytes(nsamp_pad, cands[i], !table) + st <= ctx->lds_max) bt = cands[i];
     }
     if (bt == 0) return fail(ctx, DVC_ERR_ARG, "too many samples per point for one workgroup's LDS");
     // sphere template: identical offsets for every point (one seed, Search.cpp:102-105)
@@ -398,9 +401,7 @@ int dvc_search_prepare(dvc_ctx *ctx, const dvc_run_params *rc, const double *poi
     dependency_levels(order, pos, neigh, npts, k, N, level_pts, level_off);
     if (int e = dev_alloc(ctx, &ctx->d_levels, level_pts.size())) return e;
     if (int e = dev_alloc(ctx, &ctx->d_res, (size_t)npts)) return e;
-    // reference sub-volumes of all searched points, computed ahead of the levels when
-    // they fit a budget of HBM; otherwise each point computes its own in the level chain
-    if ((size_t)N * (size_t)nsamp_pad * sizeof(float) <= kRefBudgetBytes) {
+    if (table) {
         if (int e = dev_alloc(ctx, &ctx->d_rvals, (size_t)N * nsamp_pad)) return e;
         if (int e = dev_alloc(ctx, &ctx->d_rstat, (size_t)N)) return e;
     }
@@ -439,7 +440,7 @@ static int enqueue_search(dvc_ctx *ctx, hipStream_t st) {
     a.rvals = ctx->d_rvals;
     a.rstat = ctx->d_rstat;
     int bt = ctx->chosen_threads;
-    size_t lds = search_lds_bytes(a.nsamp_pad, bt);
+    size_t lds = search_lds_bytes(a.nsamp_pad, bt, a.rvals == nullptr);
     if (launch_zero_int((int *)ctx->d_res, (long long)ctx->npts * (long long)(sizeof(dvc_point_result) / 4), st) != 0)
         return fail(ctx, DVC_ERR_HIP, "zero launch");
     if (a.rvals && launch_reference(a, bt, (size_t)a.nsamp_pad * sizeof(float), st) != 0)

@@ -54,7 +54,7 @@ int launch_search(const SearchArgs &a, const int *level_pts, int block_threads, 
 int search_set_lds_limit(size_t lds_total);
 int search_static_lds(int bpv, int block_threads, size_t *bytes);
 int search_max_blocks(int bpv, int block_threads, size_t lds_bytes, int *blocks_per_cu);
-size_t search_lds_bytes(int nsamp_pad, int block_threads);
+size_t search_lds_bytes(int nsamp_pad, int block_threads, bool samples_in_lds);
 int launch_interp(const uint8_t *vol, int bpv, int W, int H, int T, const int b0[3], const int box[3],
                   const double *xyz, int n, int method, double *out, double *grad, void *stream);
 int launch_objective(int kind, const double *ref, const double *tar, int n, double *obj_out,

@@ -650,7 +650,7 @@ __device__ __forceinline__ void set_warp(PtShared &S, int ndof) {
 // are left in V (16x16, FP64).  Returns false on Range_Fail.
 template <int BPV, int B>
 __device__ bool evaluate(const SearchArgs &a, PtShared &S, int ndof, bool need_j,
-                         float *rcb, float *stage, double *red, double *V) {
+                         const float *rsrc, float *stage, double *red, double *V) {
     constexpr int NW = B / 64;
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int n = a.nsamp;
@@ -713,7 +713,7 @@ __device__ bool evaluate(const SearchArgs &a, PtShared &S, int ndof, bool need_j
         for (int q = 0; q < 16; q++) v[q] = 0.0f;
         if (ok) {
             float tc = (float)((double)t - rbmean);
-            float rc = rcb[j];
+            float rc = rsrc[j];
             const double tcd = (double)tc, rcd = (double)rc;
             s_tc += tcd;
             if (centred) {
@@ -1240,10 +1240,11 @@ __device__ void point_search(const SearchArgs &a, int p, int n, PtShared &S, flo
         if (tid == 0) S.status = DVC_RANGE_FAIL;
         return;
     }
+    // reference samples: computed for every point before the levels run
+    // (reference_kernel) and read from HBM, or computed here into LDS
+    const float *rsrc = rcb;
     if (a.rvals) {
-        // computed for every point before the levels run (reference_kernel)
-        const float *rv = a.rvals + (size_t)p * a.nsamp_pad;
-        for (int j = tid; j < ns; j += B) rcb[j] = rv[j];
+        rsrc = a.rvals + (size_t)p * a.nsamp_pad;
         if (tid == 0) {
             const RefStats &st = a.rstat[p];
             S.rmean = st.rmean; S.rbmean = st.rbmean;
@@ -1299,7 +1300,7 @@ __device__ void point_search(const SearchArgs &a, int p, int n, PtShared &S, flo
                         S.evalpar[2] = S.par[2] + (double)z * inc;
                     }
                     __syncthreads();
-                    if (!evaluate<BPV, B>(a, S, 3, false, rcb, stage, red, V)) {
+                    if (!evaluate<BPV, B>(a, S, 3, false, rsrc, stage, red, V)) {
                         if (tid == 0) S.status = DVC_RANGE_FAIL;
                         return;
                     }
@@ -1324,7 +1325,7 @@ __device__ void point_search(const SearchArgs &a, int p, int n, PtShared &S, flo
             S.iters += 1;                 // LM_prep_at calls, counted when attempted
         }
         __syncthreads();
-        if (!evaluate<BPV, B>(a, S, nd, true, rcb, stage, red, V)) {
+        if (!evaluate<BPV, B>(a, S, nd, true, rsrc, stage, red, V)) {
             if (tid == 0) S.status = DVC_RANGE_FAIL;
             return;
         }
@@ -1352,7 +1353,7 @@ __device__ void point_search(const SearchArgs &a, int p, int n, PtShared &S, flo
         if (tid == 0)
             for (int q = 0; q < 12; q++) S.evalpar[q] = S.par[q];
         __syncthreads();
-        if (!evaluate<BPV, B>(a, S, nd, false, rcb, stage, red, V)) {
+        if (!evaluate<BPV, B>(a, S, nd, false, rsrc, stage, red, V)) {
             if (tid == 0) S.status = DVC_RANGE_FAIL;
             return;
         }
@@ -1364,18 +1365,15 @@ __device__ void point_search(const SearchArgs &a, int p, int n, PtShared &S, flo
 // handles the point at position p of the order.
 template <int BPV, int B>
 __global__ void __launch_bounds__(B) reference_kernel(SearchArgs a) {
-    DVC_DYN_LDS(float, dyn);
-    float *rcb = dyn;
     DVC_SHARED(PtShared, S);
     DVC_SHARED(double[(B / 64) * 9], red);
     const int tid = threadIdx.x;
     const int p = blockIdx.x;
     const int n = a.order[p];
     if (!point_setup<BPV, B>(a, n, S)) return;
-    reference_pass<BPV, B>(a, S, rcb, red);
+    // each thread writes and re-reads only its own samples
+    reference_pass<BPV, B>(a, S, a.rvals + (size_t)p * a.nsamp_pad, red);
     __syncthreads();
-    float *rv = a.rvals + (size_t)p * a.nsamp_pad;
-    for (int j = tid; j < a.nsamp; j += B) rv[j] = rcb[j];
     if (tid == 0) {
         RefStats &st = a.rstat[p];
         st.rmean = S.rmean; st.rbmean = S.rbmean;
@@ -1388,8 +1386,9 @@ __global__ void __launch_bounds__(B) reference_kernel(SearchArgs a) {
 template <int BPV, int B>
 __global__ void __launch_bounds__(B) search_kernel(SearchArgs a, const int *level_pts) {
     DVC_DYN_LDS(float, dyn);
-    float *rcb = dyn;
-    float *stage = dyn + a.nsamp_pad;
+    // [MFMA staging tiles][reference samples, when not read from HBM]
+    float *stage = dyn;
+    float *rcb = dyn + (B / 64) * 64 * 17;
     DVC_SHARED(PtShared, S);
     DVC_SHARED(double[(B / 64) * 9], red);
     DVC_SHARED(double[256], V);
@@ -1414,8 +1413,9 @@ __global__ void __launch_bounds__(B) search_kernel(SearchArgs a, const int *leve
 }
 
 #ifndef DVC_EMU
-size_t search_lds_bytes(int nsamp_pad, int block_threads) {
-    return (size_t)nsamp_pad * sizeof(float) + (size_t)(block_threads / 64) * 64 * 17 * sizeof(float);
+size_t search_lds_bytes(int nsamp_pad, int block_threads, bool samples_in_lds) {
+    return (samples_in_lds ? (size_t)nsamp_pad * sizeof(float) : 0) +
+           (size_t)(block_threads / 64) * 64 * 17 * sizeof(float);
 }
 
 template <int BPV, int B>
@@ -1443,7 +1443,8 @@ static int lds_limit_t(size_t lds_total, int *) {
 
 template <int BPV, int B>
 static int launch_reference_t(const SearchArgs &a, size_t lds, hipStream_t st) {
-    hipLaunchKernelGGL((reference_kernel<BPV, B>), dim3(a.nproc), dim3(B), lds, st, a);
+    (void)lds;
+    hipLaunchKernelGGL((reference_kernel<BPV, B>), dim3(a.nproc), dim3(B), 0, st, a);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

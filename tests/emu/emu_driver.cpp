@@ -50,7 +50,7 @@ extern "C" int emu_search(const dvc_run_params *rc, const void *ref, const void 
     if (gather_mode == 0) a.gather32 = 0;          // force the 64-bit row addressing
     a.sph = sph.data(); a.pts = pts; a.order = order; a.pos = pos.data(); a.neigh = neigh;
     a.res = res.data();
-    size_t lds = (size_t)a.nsamp_pad * 4 + (size_t)(block_threads / 64) * 64 * 17 * 4;
+    size_t lds = (size_t)a.nsamp_pad * 4 + (size_t)(block_threads / 64) * 64 * 17 * 4;   // samples + staging
     std::vector<float> rvals;
     std::vector<RefStats> rstat;
     if (precompute) {
@@ -62,10 +62,10 @@ extern "C" int emu_search(const dvc_run_params *rc, const void *ref, const void 
         emu::grid_dim.x = N;
         for (int b0 = 0; b0 < N; b0 += 8) {
             int nb = N - b0 < 8 ? N - b0 : 8;
-            if (bpv == 1 && block_threads == 64) emu::run_grid(b0, nb, 64, (size_t)a.nsamp_pad * 4, [&] { reference_kernel<1, 64>(a); });
-            else if (bpv == 1 && block_threads == 128) emu::run_grid(b0, nb, 128, (size_t)a.nsamp_pad * 4, [&] { reference_kernel<1, 128>(a); });
-            else if (bpv == 2 && block_threads == 64) emu::run_grid(b0, nb, 64, (size_t)a.nsamp_pad * 4, [&] { reference_kernel<2, 64>(a); });
-            else if (bpv == 2 && block_threads == 128) emu::run_grid(b0, nb, 128, (size_t)a.nsamp_pad * 4, [&] { reference_kernel<2, 128>(a); });
+            if (bpv == 1 && block_threads == 64) emu::run_grid(b0, nb, 64, 0, [&] { reference_kernel<1, 64>(a); });
+            else if (bpv == 1 && block_threads == 128) emu::run_grid(b0, nb, 128, 0, [&] { reference_kernel<1, 128>(a); });
+            else if (bpv == 2 && block_threads == 64) emu::run_grid(b0, nb, 64, 0, [&] { reference_kernel<2, 64>(a); });
+            else if (bpv == 2 && block_threads == 128) emu::run_grid(b0, nb, 128, 0, [&] { reference_kernel<2, 128>(a); });
             else return -1;
         }
     }

@@ -95,3 +95,12 @@ def test_points_near_volume_edge(ctx, oracle):
     case = synth.config1(size=64, n=4, spacing=14)   # outer points' boxes leave the volume
     g, o = _compare(ctx, oracle, case, case.params)
     assert (g["status"] == D.RANGE_FAIL).any() and (g["status"] == D.POINT_GOOD).any()
+
+
+def test_cloud_larger_than_lds(ctx, oracle):
+    """A cube cloud of 35^3 = 42,875 samples (subvol_npts 40,000; the reference accepts up
+    to 50,000): the reference samples live in the HBM table, not in LDS."""
+    case = synth.config1(size=96, n=2, spacing=10, npts=40000)
+    p = dict(case.params, subvol_size=41, disp_max=4)
+    g, o = _compare(ctx, oracle, case, p, truth_tol=0.05)
+    assert np.all(g["status"] == 0)
