@@ -1325,7 +1325,11 @@ __device__ void point_search(const SearchArgs &a, int p, int n, PtShared &S, flo
             S.iters += 1;                 // LM_prep_at calls, counted when attempted
         }
         __syncthreads();
-        if (!evaluate<BPV, B>(a, S, nd, true, rsrc, stage, red, V)) {
+        // the step-size test needs only the previous step: when it already stops the
+        // loop, this evaluation only supplies the final objective and needs no Jacobian
+        const double del_mag = sqrt(S.upd[0] * S.upd[0] + S.upd[1] * S.upd[1] + S.upd[2] * S.upd[2]);
+        const bool small_step = i > 0 && del_mag < 0.01;
+        if (!evaluate<BPV, B>(a, S, nd, !small_step, rsrc, stage, red, V)) {
             if (tid == 0) S.status = DVC_RANGE_FAIL;
             return;
         }
@@ -1334,8 +1338,7 @@ __device__ void point_search(const SearchArgs &a, int p, int n, PtShared &S, flo
             bool stop = false;
             if (i > 0) {
                 double del_obj = fabs(S.obj - obj_old);
-                double del_mag = sqrt(S.upd[0] * S.upd[0] + S.upd[1] * S.upd[1] + S.upd[2] * S.upd[2]);
-                if ((del_obj < 0.000001) || (del_mag < 0.01)) stop = true;
+                if ((del_obj < 0.000001) || small_step) stop = true;
             }
             if (!stop) {
                 if (nd == 3) gn_step<3>(a, S, V);
