@@ -43,6 +43,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--bytes-sample", type=int, default=400, help="points sampled for the algorithmic byte count")
     ap.add_argument("--cache", default="", help="directory to keep the generated volumes in between runs")
+    ap.add_argument("--no-proposal", action="store_true", help="skip the learned-proposal leg")
+    ap.add_argument("--weights", default="", help="proposal network parameters (.npz); default: the package's")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a plain device-to-device copy of this many bytes (3 times), the known-size "
                          "kernel that scales the HBM byte counters of a profiler run")
@@ -67,6 +69,50 @@ def load_case(cache: str, synth):
         np.save(r, case.ref)
         np.save(t, case.tar)
     return case
+
+
+def proposal_leg(ctx, a, case, params, order, neigh, plain, D):
+    """The same workload with learned start warps: every step proposes a warp for each
+    point (crops + network forward), searches all points from it in one launch, then
+    runs the judge pass from the reference's own start guess on those results and keeps
+    the proposal's result only where it is no worse (DVC_START_JUDGED)."""
+    from digitalvolumecorrelation_amd import proposal as P
+    wpath = Path(a.weights) if a.weights else P.WEIGHTS
+    if not wpath.exists():
+        log(f"no proposal weights at {wpath}: NN-proposal leg skipped")
+        return None
+    net = P.ProposalNet(ctx, max_batch=256, params=P.ProposalNet.load_params(wpath))
+    ctx.set_start_mode(D.START_JUDGED)
+    ctx.prepare(params, case.points, order, neigh)
+    ctx.set_start_mode(D.START_CHAIN)
+    for _ in range(a.warmup):
+        net.propose()
+        ctx.execute()
+        ctx.kernel_ms()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        net.propose()
+        ctx.execute()
+    ctx.kernel_ms()                       # the last step's closing event
+    wall_ms = (time.perf_counter() - t0) * 1e3 / a.steps
+    res = ctx.fetch()
+    prop, judge, kept = ctx.fetch_passes()
+    n = len(case.points)
+    log(f"NN proposal: {wall_ms:.2f} ms/step")
+    both = (res["status"] == 0) & (plain["status"] == 0)
+    du = np.abs(res["par"][both, :3] - plain["par"][both, :3]).max(axis=1) if both.any() else np.zeros(0)
+    pg = prop["status"] == 0
+    rel = np.abs(res["obj"][both] - plain["obj"][both]) / np.abs(plain["obj"][both])
+    return {"value": n / (wall_ms / 1e3), "unit": "points/s", "ms_per_step": wall_ms,
+            "speedup_vs_plain": None, "weights": wpath.name,
+            "mode": "judged: proposal pass + judge pass from the reference start guess, keep-if-no-worse",
+            "kept_proposal": int((kept == 1).sum()), "fallback_to_judge": int((kept == 0).sum()),
+            "status_equal_plain": bool(np.array_equal(res["status"], plain["status"])),
+            "mean_iters_proposal_pass": float(prop["iters"][pg].mean()) if pg.any() else None,
+            "mean_iters_plain": float(plain["iters"][plain["status"] == 0].mean()),
+            "max_du_vs_plain_voxel": float(du.max()) if du.size else None,
+            "points_within_1e-3_of_plain": int((du <= 1e-3).sum()),
+            "max_obj_rel_vs_plain": float(rel.max()) if rel.size else None}
 
 
 def main():
@@ -150,6 +196,10 @@ def main():
             "kernel": "search_kernel<2,B>", "launches_per_step": ctx.levels(),
             "algorithmic_bytes_per_point": round(per_pt, 1), "bytes_points_sampled": nsamp}
 
+    nn = None
+    if not a.no_proposal:
+        nn = proposal_leg(ctx, a, case, params, order, neigh, res, D)
+
     cpu = None
     agree = None
     if not a.no_cpu:
@@ -173,6 +223,8 @@ def main():
                  "iters_equal": int((g["iters"] == o["iters"]).sum()), "max_du_voxel": du, "max_obj_rel": rel,
                  "within_1e-3": bool(du <= 1e-3)}
 
+    if nn:
+        nn["speedup_vs_plain"] = nn["value"] / value
     out = {
         "metric": METRIC,
         "value": value,
@@ -193,6 +245,7 @@ def main():
                    "strain_fit_device_ms": float(np.min(strain_ms)),
                    "arith": "f32 interpolation and sums, f64 warp, normal equations and solve"},
         "roofline": roof,
+        "nn_proposal": nn,
         "cpu_baseline": cpu,
         "agreement_vs_cpu": agree,
     }

@@ -16,6 +16,8 @@ import numpy as np
 LIB_PATH = Path(os.environ.get("DVC_AMD_LIB", Path(__file__).resolve().parent / "libdvc_amd.so"))
 
 POINT_GOOD, RANGE_FAIL, CONVG_FAIL, NOT_SEARCH = 0, -1, -2, -3
+# start modes of a prepared search (dvc_set_start_mode)
+START_CHAIN, START_GIVEN, START_JUDGED = 0, 1, 2
 OBJ = {"sad": 0, "ssd": 1, "zssd": 2, "nssd": 3, "znssd": 4}
 INTERP = {"nearest": 0, "trilinear": 1, "tricubic": 2}
 GEOM = {"cube": 0, "sphere": 1}
@@ -75,8 +77,25 @@ def load() -> C.CDLL:
         "dvc_set_block_threads": (I, [VP, C.c_int]),
         "dvc_set_launch_mode": (I, [VP, C.c_int]),
         "dvc_search_info": (I, [VP, C.c_void_p, C.c_void_p, C.c_void_p]),
+        "dvc_set_start_mode": (I, [VP, C.c_int]),
+        "dvc_search_set_starts": (I, [VP, VP, C.c_int]),
+        "dvc_search_fetch_passes": (I, [VP, VP, VP, VP]),
         "dvc_strain_fit": (I, [VP, P(D), P(D), P(I), P(D), P(I), I, I, I, D, I, I,
                                P(D), P(I), P(D), P(D), P(D), P(D)]),
+        "dvc_net_create": (I, [VP, C.c_int, C.c_int, P(VP)]),
+        "dvc_net_destroy": (None, [VP]),
+        "dvc_net_last_error": (C.c_char_p, [VP]),
+        "dvc_net_num_params": (C.c_int64, []),
+        "dvc_net_params": (I, [VP, P(C.c_float), C.c_int64, C.c_int]),
+        "dvc_net_crop": (I, [VP, VP, VP, I, VP, VP]),
+        "dvc_device_alloc": (I, [VP, P(VP), C.c_int64]),
+        "dvc_device_free": (I, [VP, VP]),
+        "dvc_device_copy": (I, [VP, VP, VP, C.c_int64, C.c_int]),
+        "dvc_synchronize": (I, [VP]),
+        "dvc_net_forward": (I, [VP, VP, I, VP, VP]),
+        "dvc_net_gradients": (I, [VP, VP, VP, I, P(C.c_float), P(C.c_float)]),
+        "dvc_net_train_step": (I, [VP, VP, VP, I, C.c_float, P(C.c_float)]),
+        "dvc_search_propose": (I, [VP, VP]),
         "dvc_sort_order_neighbors": (I, [P(D), I, P(D), P(I), P(I), I]),
         "dvc_knn_order_neighbors": (I, [P(D), I, P(D), I, P(I), P(I), I]),
     }
@@ -92,7 +111,7 @@ def exported_symbols() -> list[str]:
     """Entry points declared in include/dvc_amd.h."""
     hdr = Path(__file__).resolve().parent.parent / "include" / "dvc_amd.h"
     import re
-    return sorted(set(re.findall(r"^\s*(?:int|void|const char)\s*\*?\s*(dvc_\w+)\s*\(", hdr.read_text(), re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int64_t|int|void|const char)\s*\*?\s*(dvc_\w+)\s*\(", hdr.read_text(), re.M)))
 
 
 def _dp(a: np.ndarray):
@@ -161,6 +180,13 @@ class Context:
             raise DvcError(rc, self.lib.dvc_last_error(self.h).decode())
         return rc
 
+    def set_stream(self, stream_ptr: int | None):
+        """Run the context's work on this hipStream_t (None: its own stream)."""
+        self._check(self.lib.dvc_set_stream(self.h, C.c_void_p(stream_ptr) if stream_ptr else None))
+
+    def synchronize(self):
+        self._check(self.lib.dvc_synchronize(self.h))
+
     # ---- volumes
     def upload(self, slot: int, vol: np.ndarray, big_endian: bool = False, header: bytes = b""):
         vol = np.ascontiguousarray(vol)
@@ -214,6 +240,29 @@ class Context:
         self._check(self.lib.dvc_search_prepare(self.h, C.byref(params), _dp(self._pts), self._npts, _ip(self._ord),
                                                 nproc, _ip(self._nb), self._nb.shape[1]))
 
+    def set_start_mode(self, mode: int):
+        """START_CHAIN (the reference's serial start guess), START_GIVEN or START_JUDGED;
+        applied by the next prepare."""
+        self._check(self.lib.dvc_set_start_mode(self.h, int(mode)))
+
+    def set_starts(self, start):
+        """Start warps [npts, 12] of a prepared given-start search: a host array, or a
+        device pointer (int) to npts*12 float64 values."""
+        if isinstance(start, int):
+            self._check(self.lib.dvc_search_set_starts(self.h, C.c_void_p(start), 1))
+            return
+        st = np.ascontiguousarray(start, dtype=np.float64).reshape(self._npts, 12)
+        self._check(self.lib.dvc_search_set_starts(self.h, st.ctypes.data_as(C.c_void_p), 0))
+
+    def fetch_passes(self):
+        """(proposal-pass results, judge-pass results, kept) of a START_JUDGED execute."""
+        a = np.zeros(self._npts, dtype=RESULT_DTYPE)
+        b = np.zeros(self._npts, dtype=RESULT_DTYPE)
+        kept = np.zeros(self._npts, dtype=np.int32)
+        self._check(self.lib.dvc_search_fetch_passes(self.h, a.ctypes.data_as(C.c_void_p),
+                                                     b.ctypes.data_as(C.c_void_p), kept.ctypes.data_as(C.c_void_p)))
+        return a, b, kept
+
     def _info(self):
         v = [C.c_int32(), C.c_int32(), C.c_int32()]
         self._check(self.lib.dvc_search_info(self.h, *[C.byref(x) for x in v]))
@@ -235,9 +284,16 @@ class Context:
         return res
 
     def search(self, params, points, order, neigh, nproc=0) -> np.ndarray:
-        self.prepare(params, points, order, neigh, nproc)
-        self.execute()
-        return self.fetch()
+        """The serial point loop (dvc_search_points), whatever the start mode."""
+        self._pts = np.ascontiguousarray(points, dtype=np.float64)
+        self._ord = np.ascontiguousarray(order, dtype=np.int32)
+        self._nb = np.ascontiguousarray(neigh, dtype=np.int32)
+        self._npts = self._pts.shape[0]
+        res = np.zeros(self._npts, dtype=RESULT_DTYPE)
+        self._check(self.lib.dvc_search_points(self.h, C.byref(params), _dp(self._pts), self._npts, _ip(self._ord),
+                                               nproc, _ip(self._nb), self._nb.shape[1],
+                                               res.ctypes.data_as(C.c_void_p)))
+        return res
 
     def kernel_ms(self) -> float:
         ms = C.c_float()
@@ -258,6 +314,52 @@ class Context:
                                             float(objt), int(refill), int(planar), _dp(o["disp"]), _ip(o["npts"]),
                                             _dp(o["swr"]), _dp(o["E"]), _dp(o["L"]), _dp(o["DG"])))
         return o
+
+
+class DeviceArray:
+    """A device allocation of the context (dvc_device_alloc), shaped like a numpy
+    array.  ``ptr`` is its address; ``row(i)`` the address of row i of the first axis."""
+
+    def __init__(self, ctx: Context, shape, dtype):
+        self.ctx = ctx
+        self.shape = tuple(int(v) for v in (shape if isinstance(shape, (tuple, list)) else (shape,)))
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = C.c_void_p()
+        ctx._check(ctx.lib.dvc_device_alloc(ctx.h, C.byref(p), self.nbytes))
+        self.ptr = int(p.value or 0)
+
+    @classmethod
+    def from_host(cls, ctx: Context, arr: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(arr)
+        d = cls(ctx, a.shape, a.dtype)
+        d.upload(a)
+        return d
+
+    def row(self, i: int) -> int:
+        return self.ptr + int(i) * (self.nbytes // self.shape[0])
+
+    def upload(self, arr: np.ndarray, row: int = 0):
+        a = np.ascontiguousarray(arr, dtype=self.dtype)
+        self.ctx._check(self.ctx.lib.dvc_device_copy(self.ctx.h, C.c_void_p(self.row(row)),
+                                                     a.ctypes.data_as(C.c_void_p), a.nbytes, 0))
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        self.ctx._check(self.ctx.lib.dvc_device_copy(self.ctx.h, out.ctypes.data_as(C.c_void_p),
+                                                     C.c_void_p(self.ptr), self.nbytes, 1))
+        return out
+
+    def free(self):
+        if getattr(self, "ptr", 0) and getattr(self.ctx, "h", None):
+            self.ctx.lib.dvc_device_free(self.ctx.h, C.c_void_p(self.ptr))
+        self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 def knn_order_neighbors(points: np.ndarray, start=None, k: int = 75, nthreads: int = 0):

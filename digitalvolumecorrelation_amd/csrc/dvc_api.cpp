@@ -57,6 +57,14 @@ struct dvc_ctx {
     std::vector<int> level_off;      // dependency levels (dvc_geometry.h)
     float *d_rvals = nullptr;        // reference samples of every searched point (or null)
     RefStats *d_rstat = nullptr;
+    // start modes (DVC_START_*): the mode set for the next prepare and the prepared one
+    int start_mode = DVC_START_CHAIN;
+    int plan_mode = DVC_START_CHAIN;
+    bool starts_set = false;
+    double *d_start = nullptr;                 // [npts*12] given start warps
+    dvc_point_result *d_res_prop = nullptr;    // proposal pass (DVC_START_JUDGED)
+    dvc_point_result *d_res_judge = nullptr;   // judge pass (DVC_START_JUDGED)
+    int *d_kept = nullptr;                     // 1 where the proposal's result was kept
     // the level launches of a prepared search, captured once and replayed
     int use_graph = 1;
     hipGraph_t graph = nullptr;
@@ -103,6 +111,9 @@ void free_plan(dvc_ctx *c) {
     hipFree(c->d_pts); hipFree(c->d_sph); hipFree(c->d_order); hipFree(c->d_pos);
     hipFree(c->d_neigh); hipFree(c->d_levels); hipFree(c->d_res);
     hipFree(c->d_rvals); hipFree(c->d_rstat);
+    hipFree(c->d_start); hipFree(c->d_res_prop); hipFree(c->d_res_judge); hipFree(c->d_kept);
+    c->d_start = nullptr; c->d_res_prop = c->d_res_judge = nullptr; c->d_kept = nullptr;
+    c->starts_set = false;
     c->d_rvals = nullptr;
     c->d_rstat = nullptr;
     c->d_pts = c->d_sph = nullptr;
@@ -139,6 +150,29 @@ int check_volumes(dvc_ctx *ctx) {
 }
 
 }  // namespace
+
+namespace dvc {
+
+int ctx_device(dvc_ctx *ctx) { return ctx->device; }
+void *ctx_stream(dvc_ctx *ctx) { return (void *)ctx->stream; }
+
+int ctx_volumes(dvc_ctx *ctx, const uint8_t **v0, const uint8_t **v1, int *bpv, int *W, int *H, int *T) {
+    if (check_volumes(ctx) != 0) return -1;
+    if (ctx->vol[0].W > INT32_MAX || ctx->vol[0].H > INT32_MAX || ctx->vol[0].T > INT32_MAX) return -1;
+    *v0 = ctx->vol[0].d; *v1 = ctx->vol[1].d; *bpv = ctx->vol[0].bpv;
+    *W = (int)ctx->vol[0].W; *H = (int)ctx->vol[0].H; *T = (int)ctx->vol[0].T;
+    return 0;
+}
+
+int ctx_prepared(dvc_ctx *ctx, const double **pts, const int **order, int *npts, int *nproc, double **start) {
+    if (!ctx->plan_ready || ctx->plan_mode == DVC_START_CHAIN) return -1;
+    *pts = ctx->d_pts; *order = ctx->d_order; *npts = ctx->npts; *nproc = ctx->nproc; *start = ctx->d_start;
+    return 0;
+}
+
+void ctx_mark_starts(dvc_ctx *ctx) { ctx->starts_set = true; }
+
+}  // namespace dvc
 
 extern "C" {
 
@@ -398,9 +432,27 @@ int dvc_search_prepare(dvc_ctx *ctx, const dvc_run_params *rc, const double *poi
     if (int e = dev_alloc(ctx, &ctx->d_pos, (size_t)npts)) return e;
     if (int e = dev_alloc(ctx, &ctx->d_neigh, (size_t)npts * k)) return e;
     std::vector<int> level_pts, level_off;
-    dependency_levels(order, pos, neigh, npts, k, N, level_pts, level_off);
+    const int mode = ctx->start_mode;
+    if (mode == DVC_START_CHAIN) {
+        dependency_levels(order, pos, neigh, npts, k, N, level_pts, level_off);
+    } else {
+        // no point reads a result of the same pass: one launch over the whole order
+        level_pts.assign(order, order + N);
+        level_off = {0, N};
+    }
     if (int e = dev_alloc(ctx, &ctx->d_levels, level_pts.size())) return e;
     if (int e = dev_alloc(ctx, &ctx->d_res, (size_t)npts)) return e;
+    ctx->starts_set = false;
+    if (mode != DVC_START_CHAIN) {
+        if (int e = dev_alloc(ctx, &ctx->d_start, (size_t)npts * 12)) return e;
+        HIPCHK(hipMemsetAsync(ctx->d_start, 0, sizeof(double) * 12 * (size_t)npts, ctx->stream));
+    }
+    if (mode == DVC_START_JUDGED) {
+        if (int e = dev_alloc(ctx, &ctx->d_res_prop, (size_t)npts)) return e;
+        if (int e = dev_alloc(ctx, &ctx->d_res_judge, (size_t)npts)) return e;
+        if (int e = dev_alloc(ctx, &ctx->d_kept, (size_t)npts)) return e;
+        HIPCHK(hipMemsetAsync(ctx->d_kept, 0, sizeof(int) * (size_t)npts, ctx->stream));
+    }
     if (table) {
         if (int e = dev_alloc(ctx, &ctx->d_rvals, (size_t)N * nsamp_pad)) return e;
         if (int e = dev_alloc(ctx, &ctx->d_rstat, (size_t)N)) return e;
@@ -419,6 +471,7 @@ int dvc_search_prepare(dvc_ctx *ctx, const dvc_run_params *rc, const double *poi
     for (int q = 0; q < 3; q++) { ctx->off_min[q] = omn[q]; ctx->off_max[q] = omx[q]; }
     ctx->chosen_threads = bt;
     ctx->level_off = level_off;
+    ctx->plan_mode = mode;
     ctx->searched.assign(npts, 0);
     for (int i = 0; i < npts; i++) ctx->searched[i] = pos[i] < N ? 1 : 0;
     ctx->plan_ready = true;
@@ -437,26 +490,131 @@ static int enqueue_search(dvc_ctx *ctx, hipStream_t st) {
     if (((uintptr_t)a.vol[0] | (uintptr_t)a.vol[1]) & 3) a.gather32 = 0;   // attached, unaligned
     a.sph = ctx->d_sph; a.pts = ctx->d_pts; a.order = ctx->d_order; a.pos = ctx->d_pos; a.neigh = ctx->d_neigh;
     a.res = ctx->d_res;
+    a.nbres = ctx->d_res;
+    a.start = nullptr;
     a.rvals = ctx->d_rvals;
     a.rstat = ctx->d_rstat;
     int bt = ctx->chosen_threads;
     size_t lds = search_lds_bytes(a.nsamp_pad, bt, a.rvals == nullptr);
-    if (launch_zero_int((int *)ctx->d_res, (long long)ctx->npts * (long long)(sizeof(dvc_point_result) / 4), st) != 0)
+    const long long res_words = (long long)ctx->npts * (long long)(sizeof(dvc_point_result) / 4);
+    if (launch_zero_int((int *)ctx->d_res, res_words, st) != 0)
         return fail(ctx, DVC_ERR_HIP, "zero launch");
     if (a.rvals && launch_reference(a, bt, (size_t)a.nsamp_pad * sizeof(float), st) != 0)
         return fail(ctx, DVC_ERR_HIP, "reference kernel launch failed");
-    const int depth = (int)ctx->level_off.size() - 1;
-    for (int L = 0; L < depth; L++) {
-        int cnt = ctx->level_off[L + 1] - ctx->level_off[L];
-        if (launch_search(a, ctx->d_levels + ctx->level_off[L], bt, cnt, lds, st) != 0)
-            return fail(ctx, DVC_ERR_HIP, "search kernel launch failed");
+    if (ctx->plan_mode == DVC_START_CHAIN) {
+        const int depth = (int)ctx->level_off.size() - 1;
+        for (int L = 0; L < depth; L++) {
+            int cnt = ctx->level_off[L + 1] - ctx->level_off[L];
+            if (launch_search(a, ctx->d_levels + ctx->level_off[L], bt, cnt, lds, st) != 0)
+                return fail(ctx, DVC_ERR_HIP, "search kernel launch failed");
+        }
+        return DVC_OK;
     }
+    // proposal pass: every point from its given start warp, one launch
+    a.start = ctx->d_start;
+    if (ctx->plan_mode == DVC_START_JUDGED) {
+        a.res = ctx->d_res_prop;
+        if (launch_zero_int((int *)ctx->d_res_prop, res_words, st) != 0 ||
+            launch_zero_int((int *)ctx->d_res_judge, res_words, st) != 0)
+            return fail(ctx, DVC_ERR_HIP, "zero launch");
+    }
+    if (launch_search(a, ctx->d_levels, bt, ctx->nproc, lds, st) != 0)
+        return fail(ctx, DVC_ERR_HIP, "search kernel launch failed");
+    if (ctx->plan_mode == DVC_START_JUDGED) {
+        // judge pass: the reference's start guess on the proposal pass's results
+        a.start = nullptr;
+        a.nbres = ctx->d_res_prop;
+        a.res = ctx->d_res_judge;
+        if (launch_search(a, ctx->d_levels, bt, ctx->nproc, lds, st) != 0)
+            return fail(ctx, DVC_ERR_HIP, "search kernel launch failed");
+        if (launch_judge(ctx->d_order, ctx->nproc, ctx->d_res_prop, ctx->d_res_judge, ctx->d_res, ctx->d_kept, st) != 0)
+            return fail(ctx, DVC_ERR_HIP, "judge launch failed");
+    }
+    return DVC_OK;
+}
+
+int dvc_device_alloc(dvc_ctx *ctx, void **ptr, int64_t bytes) {
+    if (!ctx || !ptr || bytes < 0) return DVC_ERR_ARG;
+    *ptr = nullptr;
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipMalloc(ptr, (size_t)std::max<int64_t>(bytes, 1)));
+    return DVC_OK;
+}
+
+int dvc_device_free(dvc_ctx *ctx, void *ptr) {
+    if (!ctx) return DVC_ERR_ARG;
+    if (!ptr) return DVC_OK;
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipFree(ptr));
+    return DVC_OK;
+}
+
+int dvc_device_copy(dvc_ctx *ctx, void *dst, const void *src, int64_t bytes, int kind) {
+    if (!ctx || bytes < 0 || (bytes > 0 && (!dst || !src)) || kind < 0 || kind > 2) return DVC_ERR_ARG;
+    if (bytes == 0) return DVC_OK;
+    const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : (kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice);
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipMemcpyAsync(dst, src, (size_t)bytes, k, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return DVC_OK;
+}
+
+int dvc_synchronize(dvc_ctx *ctx) {
+    if (!ctx) return DVC_ERR_ARG;
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return DVC_OK;
+}
+
+int dvc_set_start_mode(dvc_ctx *ctx, int mode) {
+    if (!ctx) return DVC_ERR_ARG;
+    if (mode != DVC_START_CHAIN && mode != DVC_START_GIVEN && mode != DVC_START_JUDGED)
+        return fail(ctx, DVC_ERR_ARG, "unknown start mode");
+    ctx->start_mode = mode;
+    return DVC_OK;
+}
+
+int dvc_search_set_starts(dvc_ctx *ctx, const double *start, int on_device) {
+    if (!ctx || !start) return DVC_ERR_ARG;
+    if (!ctx->plan_ready) return fail(ctx, DVC_ERR_STATE, "dvc_search_prepare first");
+    if (ctx->plan_mode == DVC_START_CHAIN)
+        return fail(ctx, DVC_ERR_STATE, "the prepared search takes its starts from the serial chain");
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipMemcpyAsync(ctx->d_start, start, sizeof(double) * 12 * (size_t)ctx->npts,
+                          on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, ctx->stream));
+    ctx->starts_set = true;
+    return DVC_OK;
+}
+
+int dvc_search_fetch_passes(dvc_ctx *ctx, dvc_point_result *proposal, dvc_point_result *judge, int32_t *kept) {
+    if (!ctx) return DVC_ERR_ARG;
+    if (!ctx->plan_ready) return fail(ctx, DVC_ERR_STATE, "dvc_search_prepare first");
+    if (ctx->plan_mode != DVC_START_JUDGED) return fail(ctx, DVC_ERR_STATE, "the prepared search has no judge pass");
+    HIPCHK(hipSetDevice(ctx->device));
+    const size_t n = (size_t)ctx->npts;
+    if (proposal) HIPCHK(hipMemcpyAsync(proposal, ctx->d_res_prop, sizeof(dvc_point_result) * n, hipMemcpyDeviceToHost, ctx->stream));
+    if (judge) HIPCHK(hipMemcpyAsync(judge, ctx->d_res_judge, sizeof(dvc_point_result) * n, hipMemcpyDeviceToHost, ctx->stream));
+    if (kept) HIPCHK(hipMemcpyAsync(kept, ctx->d_kept, sizeof(int) * n, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (size_t i = 0; i < n; i++)
+        if (!ctx->searched[i]) {
+            dvc_point_result *rs[2] = {proposal, judge};
+            for (dvc_point_result *r : rs)
+                if (r) {
+                    r[i].status = DVC_NOT_SEARCH; r[i].iters = 0; r[i].obj = 0.0;
+                    for (int q = 0; q < 12; q++) r[i].par[q] = 0.0;
+                }
+            if (kept) kept[i] = -1;
+        }
     return DVC_OK;
 }
 
 int dvc_search_execute(dvc_ctx *ctx) {
     if (!ctx) return DVC_ERR_ARG;
     if (!ctx->plan_ready) return fail(ctx, DVC_ERR_STATE, "dvc_search_prepare first");
+    if (ctx->plan_mode != DVC_START_CHAIN && !ctx->starts_set)
+        return fail(ctx, DVC_ERR_STATE, "dvc_search_set_starts first");
     if (int e = check_volumes(ctx)) return e;
     HIPCHK(hipSetDevice(ctx->device));
     if (ctx->use_graph && !ctx->gexec) {
@@ -521,8 +679,14 @@ int dvc_search_fetch(dvc_ctx *ctx, dvc_point_result *results) {
 int dvc_search_points(dvc_ctx *ctx, const dvc_run_params *rc, const double *points, int32_t npts,
                       const int32_t *order, int32_t nproc, const int32_t *neigh, int32_t k,
                       dvc_point_result *results) {
-    if (int e = dvc_search_prepare(ctx, rc, points, npts, order, nproc, neigh, k)) return e;
-    if (int e = dvc_search_execute(ctx)) return e;
+    if (!ctx) return DVC_ERR_ARG;
+    // always the serial loop's start guess, whatever mode later prepares use
+    const int mode = ctx->start_mode;
+    ctx->start_mode = DVC_START_CHAIN;
+    int e = dvc_search_prepare(ctx, rc, points, npts, order, nproc, neigh, k);
+    ctx->start_mode = mode;
+    if (e) return e;
+    if (int e2 = dvc_search_execute(ctx)) return e2;
     return dvc_search_fetch(ctx, results);
 }
 

@@ -33,7 +33,9 @@ struct SearchArgs {
     const int *order;                // [npts]
     const int *pos;                  // position of each point in order[] (npts if unsearched)
     const int *neigh;                // [npts*k]
-    dvc_point_result *res;           // [npts]
+    dvc_point_result *res;           // [npts] results written by the launch
+    const dvc_point_result *nbres;   // [npts] results the start guess averages (res in the serial chain)
+    const double *start;             // [npts*12] given start warps (a proposal pass), or null
     float *rvals;                    // [nproc * nsamp_pad] reference samples by position, or null
     RefStats *rstat;                 // [nproc]
 };
@@ -62,6 +64,17 @@ int launch_objective(int kind, const double *ref, const double *tar, int n, doub
 int launch_strain(const StrainArgs &a, void *stream);
 int launch_reference(const SearchArgs &a, int block_threads, size_t lds_bytes, void *stream);
 int launch_zero_int(int *p, long long n, void *stream);
+int launch_judge(const int *order, int n, const dvc_point_result *prop, const dvc_point_result *judge,
+                 dvc_point_result *out, int *kept, void *stream);
 int launch_swap16(uint16_t *p, long long n, void *stream);
+
+// context internals for the proposal network (dvc_net.hip), implemented in dvc_api.cpp
+int ctx_device(dvc_ctx *ctx);
+void *ctx_stream(dvc_ctx *ctx);
+// both volume slots filled with the same shape and type: 0, else -1
+int ctx_volumes(dvc_ctx *ctx, const uint8_t **v0, const uint8_t **v1, int *bpv, int *W, int *H, int *T);
+// a prepared given-start or judged search: its device points, order, sizes and start table
+int ctx_prepared(dvc_ctx *ctx, const double **pts, const int **order, int *npts, int *nproc, double **start);
+void ctx_mark_starts(dvc_ctx *ctx);
 
 }  // namespace dvc

@@ -1253,38 +1253,59 @@ __device__ void point_search(const SearchArgs &a, int p, int n, PtShared &S, flo
     } else {
         reference_pass<BPV, B>(a, S, rcb, red);
     }
-    // ---- starting_param: neighbours searched earlier.  Launches run the dependency
-    // levels in order, so every such neighbour's result is already in memory.
-    if (tid < 64) {
-        for (int q = lane; q < a.k; q += 64) {
-            int m = a.neigh[(long long)n * a.k + q];
-            int st = DVC_NOT_SEARCH;
-            double px = 0.0, py = 0.0, pz = 0.0;
-            if (m >= 0 && m < a.npts && a.pos[m] < p) {
-                st = a.res[m].status;
-                px = a.res[m].par[0]; py = a.res[m].par[1]; pz = a.res[m].par[2];
+    if (a.start) {
+        // ---- a given start warp (proposal pass): all ndof parameters, no neighbours
+        if (tid == 0) {
+            int bad = 0;
+            for (int q = 0; q < 12; q++) {
+                const double v = (q < a.ndof) ? a.start[(long long)n * 12 + q] : 0.0;
+                if (!isfinite(v)) bad = 1;
+                S.par[q] = v;
             }
-            S.nb_status[q] = st;
-            S.nb_par[q][0] = px; S.nb_par[q][1] = py; S.nb_par[q][2] = pz;
+            S.fail = bad;
+            if (!bad) place_box(a, S, S.sp[0] + S.par[0], S.sp[1] + S.par[1], S.sp[2] + S.par[2]);
         }
-    }
-    __syncthreads();
-    if (tid == 0) {
-#pragma clang fp contract(off)
-        double sx = 0.0, sy = 0.0, sz = 0.0;
-        int good = 0;
-        for (int q = 0; q < a.k; q++)
-            if (S.nb_status[q] == DVC_POINT_GOOD) {
-                sx += S.nb_par[q][0]; sy += S.nb_par[q][1]; sz += S.nb_par[q][2]; good += 1;
+        __syncthreads();
+        if (S.fail) {
+            if (tid == 0) S.status = DVC_RANGE_FAIL;
+            return;
+        }
+    } else {
+        // ---- starting_param: good neighbours at earlier positions of the order.  In
+        // the serial chain (nbres == res) launches run the dependency levels in order, so
+        // every such neighbour's result is already in memory; a judge pass reads them
+        // from the finished table of a proposal pass.
+        if (tid < 64) {
+            for (int q = lane; q < a.k; q += 64) {
+                int m = a.neigh[(long long)n * a.k + q];
+                int st = DVC_NOT_SEARCH;
+                double px = 0.0, py = 0.0, pz = 0.0;
+                if (m >= 0 && m < a.npts && a.pos[m] < p) {
+                    st = a.nbres[m].status;
+                    px = a.nbres[m].par[0]; py = a.nbres[m].par[1]; pz = a.nbres[m].par[2];
+                }
+                S.nb_status[q] = st;
+                S.nb_par[q][0] = px; S.nb_par[q][1] = py; S.nb_par[q][2] = pz;
             }
-        if (good != 0) { sx /= good; sy /= good; sz /= good; }
-        if (good != 0) { S.par[0] = sx; S.par[1] = sy; S.par[2] = sz; }
-        else { S.par[0] = a.rigid[0]; S.par[1] = a.rigid[1]; S.par[2] = a.rigid[2]; }
-        // target box on the offset point (Search.cpp:267-273)
-        double ox = S.sp[0] + S.par[0], oy = S.sp[1] + S.par[1], oz = S.sp[2] + S.par[2];
-        place_box(a, S, ox, oy, oz);
+        }
+        __syncthreads();
+        if (tid == 0) {
+#pragma clang fp contract(off)
+            double sx = 0.0, sy = 0.0, sz = 0.0;
+            int good = 0;
+            for (int q = 0; q < a.k; q++)
+                if (S.nb_status[q] == DVC_POINT_GOOD) {
+                    sx += S.nb_par[q][0]; sy += S.nb_par[q][1]; sz += S.nb_par[q][2]; good += 1;
+                }
+            if (good != 0) { sx /= good; sy /= good; sz /= good; }
+            if (good != 0) { S.par[0] = sx; S.par[1] = sy; S.par[2] = sz; }
+            else { S.par[0] = a.rigid[0]; S.par[1] = a.rigid[1]; S.par[2] = a.rigid[2]; }
+            // target box on the offset point (Search.cpp:267-273)
+            double ox = S.sp[0] + S.par[0], oy = S.sp[1] + S.par[1], oz = S.sp[2] + S.par[2];
+            place_box(a, S, ox, oy, oz);
+        }
+        __syncthreads();
     }
-    __syncthreads();
     // ---- translation grid (Search.cpp:934-1043)
     if (a.basin != 0.0) {
         const int ni = (int)(a.disp_max / a.basin);
@@ -1632,6 +1653,32 @@ int launch_objective(int kind, const double *ref, const double *tar, int n, doub
 #endif  // DVC_EMU
 
 // ------------------------------------------------------------- small kernels
+
+// Keep-if-not-worse selection of a proposal pass against its judge pass (the
+// reference's start guess applied to the proposal pass's results).  The proposal's
+// result is kept when it converged and its objective is no worse than the judge's by
+// more than the reference's own objective tolerance (1e-6, Search.cpp:222-229).
+__global__ void judge_kernel(const int *order, int n, const dvc_point_result *prop,
+                             const dvc_point_result *judge, dvc_point_result *out, int *kept) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int m = order[i];
+    const dvc_point_result &a = prop[m];
+    const dvc_point_result &b = judge[m];
+    const bool keep = a.status == DVC_POINT_GOOD && (b.status != DVC_POINT_GOOD || a.obj <= b.obj + 1e-6);
+    out[m] = keep ? a : b;
+    kept[m] = keep ? 1 : 0;
+}
+
+#ifndef DVC_EMU
+int launch_judge(const int *order, int n, const dvc_point_result *prop, const dvc_point_result *judge,
+                 dvc_point_result *out, int *kept, void *stream) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(judge_kernel, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, order, n, prop,
+                       judge, out, kept);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+#endif
 
 __global__ void zero_int_kernel(int *p, long long n) {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;

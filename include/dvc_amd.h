@@ -91,6 +91,16 @@ int  dvc_set_stream(dvc_ctx *ctx, void *hip_stream);
  * events on the stream the kernels ran on.  Waits for that work to finish. */
 int  dvc_last_kernel_ms(dvc_ctx *ctx, float *ms);
 
+/* Device memory on the context's device, for callers that keep their own inputs
+ * resident (no reference counterpart).  dvc_device_copy: kind 0 host to device,
+ * 1 device to host, 2 device to device; it waits for the copy and for all earlier work
+ * on the context stream.  dvc_device_free waits for that stream first.
+ * dvc_synchronize waits for all work queued on the context stream. */
+int dvc_device_alloc(dvc_ctx *ctx, void **ptr, int64_t bytes);
+int dvc_device_free(dvc_ctx *ctx, void *ptr);
+int dvc_device_copy(dvc_ctx *ctx, void *dst, const void *src, int64_t bytes, int kind);
+int dvc_synchronize(dvc_ctx *ctx);
+
 /* ---------------------------------------------------------------- volumes */
 
 /* Make an image volume resident in HBM in its native uint8 / uint16 form.
@@ -152,6 +162,35 @@ int dvc_search_prepare(dvc_ctx *ctx, const dvc_run_params *rc, const double *poi
 int dvc_search_execute(dvc_ctx *ctx);
 int dvc_search_fetch(dvc_ctx *ctx, dvc_point_result *results);
 
+/* Start modes of a prepared search.  The reference has one: the serial loop's start
+ * guess (Search.cpp:277-377), DVC_START_CHAIN.  The other two have no counterpart in
+ * the reference; they serve starting warps proposed for every point at once (e.g. by a
+ * learned proposal) and drop the dependency between points, so each pass is one launch:
+ *   DVC_START_GIVEN  : every point searched from its given start warp start[12n..12n+ndof).
+ *   DVC_START_JUDGED : the given-start pass, then a judge pass in which every point starts
+ *                      from the reference's own guess (mean translation of good neighbours
+ *                      earlier in order) computed on the first pass's results; a point keeps
+ *                      the given-start result only if it converged and its objective is no
+ *                      worse than the judge's by more than 1e-6 (the objective tolerance of
+ *                      Search.cpp:222-229), else the judge's.
+ * dvc_set_start_mode applies to the next dvc_search_prepare (default DVC_START_CHAIN;
+ * dvc_search_points always runs the chain). */
+#define DVC_START_CHAIN  0
+#define DVC_START_GIVEN  1
+#define DVC_START_JUDGED 2
+int dvc_set_start_mode(dvc_ctx *ctx, int mode);
+/* Start warps of a prepared DVC_START_GIVEN / DVC_START_JUDGED search: start[npts*12]
+ * (u v w phi the psi exx eyy ezz exy eyz exz per point; entries past ndof ignored), in
+ * host memory, or in device memory when on_device is 1 (copied on the context stream, so
+ * a producer on another stream must be finished).  Required before dvc_search_execute;
+ * later calls replace them for the next execute.  A non-finite start fails its point
+ * with DVC_RANGE_FAIL in the given-start pass. */
+int dvc_search_set_starts(dvc_ctx *ctx, const double *start, int on_device);
+/* After a DVC_START_JUDGED execute: the given-start pass's results, the judge pass's
+ * results (each [npts], either may be NULL) and kept[npts] (1: given-start result kept,
+ * 0: judge's, -1: not searched; may be NULL).  dvc_search_fetch returns the selection. */
+int dvc_search_fetch_passes(dvc_ctx *ctx, dvc_point_result *proposal, dvc_point_result *judge, int32_t *kept);
+
 /* Launch tuning: threads per point (64, 128, 256, 512 or 1024; 0 = automatic), applied
  * by the next dvc_search_prepare. */
 int dvc_set_block_threads(dvc_ctx *ctx, int threads);
@@ -161,6 +200,52 @@ int dvc_set_launch_mode(dvc_ctx *ctx, int graph);
 /* Shape of the prepared search: dependency levels (one kernel launch each), threads
  * per point, points searched.  Any pointer may be NULL. */
 int dvc_search_info(dvc_ctx *ctx, int32_t *levels, int32_t *threads_per_point, int32_t *points_searched);
+
+/* -------------------------------------------------------- learned proposal */
+
+/* A learned start-warp proposal.  The reference has no counterpart (its only start
+ * guess is the neighbours' mean translation, Search.cpp:277-377); this network proposes
+ * the 12 warp parameters of a point from the 32^3 reference/target crops around it, for
+ * the given-start and judged search modes above.  Network: four 3x3x3 stride-2
+ * convolutions with ReLU (2->16->32->64->128 channels, BF16 weights and activations, FP32
+ * accumulation on the matrix cores), the mean over the last 2^3 voxels and a 128->12
+ * linear head.  Parameters are one FP32 vector of dvc_net_num_params() values: for each
+ * convolution its weights [cout][27*cin] (tap-major, tap = (kz*3+ky)*3+kx, then input
+ * channel) and biases [cout], then the head's weights [12][128] and biases [12].  The
+ * head's output j is scaled by 8 (u v w, voxels), 0.05 (angles, radians) or 0.02
+ * (strains).  All device buffers are on the context's device and stream. */
+typedef struct dvc_net dvc_net;
+/* max_batch crops per forward (1..4096); training != 0 also allocates gradient and
+ * optimiser buffers (then max_batch must be a multiple of 4). */
+int  dvc_net_create(dvc_ctx *ctx, int max_batch, int training, dvc_net **net);
+void dvc_net_destroy(dvc_net *net);
+const char *dvc_net_last_error(const dvc_net *net);
+int64_t dvc_net_num_params(void);
+/* set != 0: upload count FP32 parameters from host (and reset the optimiser);
+ * set == 0: download them. */
+int  dvc_net_params(dvc_net *net, float *host, int64_t count, int set);
+/* Crops of the context's volume pair around points_dev[3*idx_dev[i]] (idx_dev NULL:
+ * point i), i < count, into slot slot_dev[i] (NULL: slot i) of out_dev (BF16
+ * [slots][32][32][32][2], channel 0 reference, 1 target, each normalised to zero mean
+ * and unit variance; out_dev NULL: the network's own input buffer, count <= max_batch).
+ * All pointers are device pointers; the caller sizes out_dev for every slot named. */
+int  dvc_net_crop(dvc_net *net, const double *points_dev, const int32_t *idx_dev, int32_t count, void *out_dev,
+                  const int32_t *slot_dev);
+/* Forward pass of B crops (in_dev: device BF16 crops, NULL: the network's input buffer)
+ * into par_dev[B*12] (FP64 warp parameters, may be NULL) and y_dev[B*12] (FP32 unscaled
+ * head output, may be NULL). */
+int  dvc_net_forward(dvc_net *net, const void *in_dev, int32_t B, double *par_dev, float *y_dev);
+/* Loss (mean over the batch of the squared error of the scaled outputs against
+ * label_dev[B*12] / scale) and its gradient with respect to every parameter, into
+ * grad_host[dvc_net_num_params()] and loss_host (either may be NULL). */
+int  dvc_net_gradients(dvc_net *net, const void *in_dev, const float *label_dev, int32_t B, float *grad_host,
+                       float *loss_host);
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8) on that gradient. */
+int  dvc_net_train_step(dvc_net *net, const void *in_dev, const float *label_dev, int32_t B, float lr,
+                        float *loss_host);
+/* Proposals for every point of a prepared DVC_START_GIVEN / DVC_START_JUDGED search,
+ * written straight into its start table (replaces dvc_search_set_starts). */
+int  dvc_search_propose(dvc_ctx *ctx, dvc_net *net);
 
 /* ----------------------------------------------------------------- strain */
 

@@ -976,7 +976,7 @@ struct Searcher {
     }
 
     /* process_point (Search.cpp:95-192) returning the status code (Utility.h:38-41). */
-    int process_point(const Pt &sp, const std::vector<const Result *> &nres) {
+    int process_point(const Pt &sp, const std::vector<const Result *> &nres, const double *start = nullptr) {
         delete fc;
         fc = new FCloud();
         if (rc.geom == 1)
@@ -992,7 +992,15 @@ struct Searcher {
         interp->center_on(sp);
         interp->load(ref);
         if (sample(fc->stable, ref_sv.data(), nullptr) != 0) return -1;
-        starting_param(nres);
+        if (start) {
+            /* a given start warp (no reference counterpart): all ndof parameters */
+            for (int i = 0; i < rc.ndof; i++) {
+                if (!std::isfinite(start[i])) return -1;
+                par_min[i] = start[i];
+            }
+        } else {
+            starting_param(nres);
+        }
         Pt off = sp; off.move_by(par_min[0], par_min[1], par_min[2]);
         interp->center_on(off);
         interp->load(tar);
@@ -1178,6 +1186,72 @@ int orc_dvc_run(const orc_run *rc, const void *refv, const void *tarv, const dou
             for (int q = 0; q < rc->ndof; q++) results[n].par[q] = S.par_min[q];
         } else {
             results[n].status = st;
+        }
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
+    return (int)N;
+}
+
+/* Given-start passes (no reference counterpart).  mode 1: every searched point from
+ * start[12n..]; mode 2: that pass into prop[], then the judge pass into judge[] with
+ * the reference's start guess (starting_param, Search.cpp:277-377) over the good
+ * neighbours at earlier positions of order[] taken from prop[], and results[] the
+ * proposal's result where it is good and its objective <= the judge's + 1e-6, else the
+ * judge's (kept[] = 1 / 0, -1 unsearched).  Points are independent within a pass. */
+int orc_dvc_run_start(const orc_run *rc, const void *refv, const void *tarv, const double *p, int npts,
+                      const int *order, const int *neigh, int k, int nproc, int nthreads, int mode,
+                      const double *start, orc_result *prop, orc_result *judge, orc_result *results,
+                      int *kept, double *seconds) {
+    auto clear = [&](orc_result *r) {
+        if (!r) return;
+        for (int i = 0; i < npts; i++) {
+            r[i].status = -3; r[i].iters = 0; r[i].obj = 0.0;
+            for (int q = 0; q < 12; q++) r[i].par[q] = 0.0;
+        }
+    };
+    clear(results); clear(prop); clear(judge);
+    if (mode == 2 && (!prop || !judge || !kept)) return -1;
+    Searcher S(*rc, make_vol(rc, refv), make_vol(rc, tarv), g_sbi(), nthreads);
+    unsigned int N = (unsigned int)npts;
+    if ((unsigned int)nproc > 1 && (unsigned int)nproc < N) N = (unsigned int)nproc;
+    std::vector<int> pos(npts, npts);
+    for (unsigned int i = 0; i < N; i++) pos[order[i]] = (int)i;
+    auto store = [&](orc_result *r, int n, int st) {
+        r[n].iters = S.iters;
+        if (st == 0) {
+            r[n].status = 0;
+            r[n].obj = S.obj_min;
+            for (int q = 0; q < rc->ndof; q++) r[n].par[q] = S.par_min[q];
+        } else {
+            r[n].status = st;
+        }
+    };
+    auto t0 = std::chrono::steady_clock::now();
+    orc_result *first = (mode == 2) ? prop : results;
+    std::vector<const Result *> nres;
+    for (unsigned int i = 0; i < N; i++) {
+        int n = order[i];
+        Pt sp(p[3 * n], p[3 * n + 1], p[3 * n + 2]);
+        store(first, n, S.process_point(sp, nres, start + (size_t)12 * n));
+    }
+    if (mode == 2) {
+        static const orc_result none = {-3, 0, 0.0, {0}};
+        for (unsigned int i = 0; i < N; i++) {
+            int n = order[i];
+            nres.clear();
+            for (int j = 0; j < k; j++) {
+                int m = neigh[(size_t)n * k + j];
+                nres.push_back(pos[m] < (int)i ? (const Result *)&prop[m] : (const Result *)&none);
+            }
+            Pt sp(p[3 * n], p[3 * n + 1], p[3 * n + 2]);
+            store(judge, n, S.process_point(sp, nres));
+        }
+        for (int n = 0; n < npts; n++) {
+            if (pos[n] >= (int)N) { kept[n] = -1; continue; }
+            bool keep = prop[n].status == 0 && (judge[n].status != 0 || prop[n].obj <= judge[n].obj + 1e-6);
+            results[n] = keep ? prop[n] : judge[n];
+            kept[n] = keep ? 1 : 0;
         }
     }
     auto t1 = std::chrono::steady_clock::now();

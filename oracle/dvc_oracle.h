@@ -81,6 +81,16 @@ int    orc_dvc_run(const orc_run *rc, const void *ref, const void *tar,
                    const double *pts, int npts, const int *order, const int *neigh, int k,
                    int nproc, int nthreads, orc_result *results, double *seconds);
 
+/* Given-start passes (no reference counterpart): mode 1 searches every point of the
+ * first nproc of order[] from start[12n..]; mode 2 adds the judge pass (the reference's
+ * start guess over the first pass's results at earlier positions) and keeps the first
+ * pass's result where it is good and no worse than the judge's objective + 1e-6.
+ * prop/judge/kept are required for mode 2 (kept: 1 kept, 0 judge, -1 unsearched). */
+int    orc_dvc_run_start(const orc_run *rc, const void *ref, const void *tar,
+                         const double *pts, int npts, const int *order, const int *neigh, int k,
+                         int nproc, int nthreads, int mode, const double *start, orc_result *prop,
+                         orc_result *judge, orc_result *results, int *kept, double *seconds);
+
 /* strain.cpp:402-591.  out_dis[n*3], out_npts[n], out_swr[n], out_E[n*9], out_L[n*9],
  * out_DG[n*9].  planar: bit0 yz_only, bit1 zx_only, bit2 xy_only forced; auto-detected
  * planes are added as strain.cpp:340-365 does. */

@@ -65,6 +65,7 @@ def load() -> C.CDLL:
         "orc_objective": (D, [I, P(D), P(D), I, P(D)]),
         "orc_sort_order_neighbors": (I, [P(D), I, P(D), P(I), P(I), I]),
         "orc_dvc_run": (I, [P(Run), VP, VP, P(D), I, P(I), P(I), I, I, I, VP, P(D)]),
+        "orc_dvc_run_start": (I, [P(Run), VP, VP, P(D), I, P(I), P(I), I, I, I, I, P(D), VP, VP, VP, P(I), P(D)]),
         "orc_strain": (I, [P(D), P(D), P(I), P(D), P(I), I, I, I, D, I, I, P(D), P(I), P(D), P(D), P(D), P(D)]),
         "orc_colpiv_solve": (I, [P(D), P(D), I, P(D)]),
         "orc_fullpiv_solve": (I, [P(D), P(D), I, P(D)]),
@@ -195,6 +196,27 @@ def dvc_run(run: Run, ref: np.ndarray, tar: np.ndarray, points, order, neigh, np
                            len(p), _ip(o), _ip(nb), nb.shape[1], nproc, nthreads,
                            res.ctypes.data_as(C.c_void_p), C.byref(sec))
     return res, sec.value, n
+
+
+def dvc_run_start(run: Run, ref: np.ndarray, tar: np.ndarray, points, order, neigh, start, mode=1, nproc=0,
+                  nthreads=1):
+    """Given-start passes (mode 1) or given-start + judge passes (mode 2); returns
+    (results, proposal-pass results, judge-pass results, kept, seconds)."""
+    p = np.ascontiguousarray(points, dtype=np.float64)
+    o = np.ascontiguousarray(order, dtype=np.int32)
+    nb = np.ascontiguousarray(neigh, dtype=np.int32)
+    st = np.ascontiguousarray(start, dtype=np.float64).reshape(len(p), 12)
+    ref = np.ascontiguousarray(ref)
+    tar = np.ascontiguousarray(tar)
+    res = np.zeros(len(p), dtype=RESULT_DTYPE)
+    prop = np.zeros(len(p), dtype=RESULT_DTYPE)
+    judge = np.zeros(len(p), dtype=RESULT_DTYPE)
+    kept = np.zeros(len(p), dtype=np.int32)
+    sec = C.c_double()
+    vp = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+    load().orc_dvc_run_start(C.byref(run), vp(ref), vp(tar), _dp(p), len(p), _ip(o), _ip(nb), nb.shape[1], nproc,
+                             nthreads, int(mode), _dp(st), vp(prop), vp(judge), vp(res), _ip(kept), C.byref(sec))
+    return res, prop, judge, kept, sec.value
 
 
 def strain(pts, disp, status, objmin, neigh, ndp=25, objt=1.0, refill=False, planar=0):

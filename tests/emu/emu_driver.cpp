@@ -50,6 +50,7 @@ extern "C" int emu_search(const dvc_run_params *rc, const void *ref, const void 
     if (gather_mode == 0) a.gather32 = 0;          // force the 64-bit row addressing
     a.sph = sph.data(); a.pts = pts; a.order = order; a.pos = pos.data(); a.neigh = neigh;
     a.res = res.data();
+    a.nbres = res.data();
     size_t lds = (size_t)a.nsamp_pad * 4 + (size_t)(block_threads / 64) * 64 * 17 * 4;   // samples + staging
     std::vector<float> rvals;
     std::vector<RefStats> rstat;

@@ -37,8 +37,11 @@ def test_sort_matches_oracle_and_known_answer(oracle):
 
 
 def test_no_device_is_an_error():
-    import torch  # noqa: F401  (device count only; no GPU initialisation here)
-    if torch.cuda.device_count() > 0:
-        pytest.skip("a GPU is present")
-    with pytest.raises(D.DvcError):
-        D.Context(0)
+    # without a gfx950 device dvc_create fails with an error code, never a crash
+    try:
+        c = D.Context(0)
+    except D.DvcError as e:
+        assert e.code == -5
+        return
+    c.close()
+    pytest.skip("a GPU is present")

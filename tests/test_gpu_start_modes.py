@@ -1,0 +1,126 @@
+"""Given-start and judged searches on the GPU (dvc_set_start_mode, dvc_search_set_starts,
+dvc_search_fetch_passes) against the oracle's passes on the same starts.
+
+Bar: statuses and iteration counts equal, displacements within 1e-3 voxel, objectives
+within 1e-5 relative; the keep/fallback choice equal wherever the two objectives are
+not within 1e-7 of the selection threshold (obj_proposal <= obj_judge + 1e-6).
+
+Points that end at the iteration limit (20 Gauss-Newton evaluations, Search.cpp:210)
+did not converge: their objective follows the exact trajectory, and the oracle's own
+two Jacobians (the reference's forward difference, h = 1e-10, and the analytic one)
+already differ there by 3.4e-5 relative on the 12-DOF prefix below.  Those points are
+held to 1e-4 relative; converged points to 1e-5.
+"""
+import numpy as np
+import pytest
+
+import digitalvolumecorrelation_amd as D
+from digitalvolumecorrelation_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _close(g, o, label):
+    assert np.array_equal(g["status"], o["status"]), label
+    assert np.array_equal(g["iters"], o["iters"]), label
+    good = o["status"] == 0
+    conv = good & (o["iters"] < 20)
+    limit = good & (o["iters"] >= 20)
+    du = np.abs(g["par"][:, :3] - o["par"][:, :3]).max()
+    rel = np.abs(g["obj"] - o["obj"]) / np.where(good, np.abs(o["obj"]), 1.0)
+    rc = rel[conv].max() if conv.any() else 0.0
+    rl = rel[limit].max() if limit.any() else 0.0
+    print(f"  {label}: max |du| {du:.2e}  max obj rel {rc:.2e} (converged), {rl:.2e} ({limit.sum()} at the "
+          f"iteration limit)  good {good.sum()}/{len(good)}")
+    assert du <= 1e-3, label
+    assert rc <= 1e-5, label
+    assert rl <= 1e-4, label
+
+
+def _starts(case, seed, offset_every=0):
+    rng = np.random.default_rng(seed)
+    st = np.zeros((len(case.points), 12))
+    st[:, :3] = case.truth + rng.uniform(-0.6, 0.6, (len(case.points), 3))
+    if offset_every:
+        st[::offset_every, :3] += 3.0
+    return st
+
+
+def _run(ctx, oracle, case, params, starts, mode, nproc=0):
+    order, neigh = D.sort_order_neighbors(case.points, start=case.start)
+    run = oracle.make_run(shape=case.ref.shape, bytes_per=case.ref.dtype.itemsize, **params)
+    ores = oracle.dvc_run_start(run, case.ref, case.tar, case.points, order, neigh, starts, mode=mode, nproc=nproc,
+                                nthreads=8)
+    ctx.upload(0, case.ref)
+    ctx.upload(1, case.tar)
+    ctx.set_start_mode(mode)
+    try:
+        ctx.prepare(D.make_params(**params), case.points, order, neigh, nproc=nproc)
+        assert ctx.levels() == 1
+        ctx.set_starts(starts)
+        ctx.execute()
+        gres = ctx.fetch()
+        passes = ctx.fetch_passes() if mode == D.START_JUDGED else None
+    finally:
+        ctx.set_start_mode(D.START_CHAIN)
+    return gres, passes, ores
+
+
+@pytest.fixture(scope="module")
+def case12():
+    return synth.config2(size=96, n=3, spacing=12, npts=2000, diameter=25)
+
+
+@pytest.fixture(scope="module")
+def case3():
+    return synth.config1(size=80, n=4, spacing=8)
+
+
+def test_given_starts_12dof(ctx, oracle, case12):
+    p = dict(case12.params, disp_max=6)
+    g, _, o = _run(ctx, oracle, case12, p, _starts(case12, 1), D.START_GIVEN)
+    _close(g, o[0], "given")
+    assert (g["status"] == 0).all()
+
+
+def test_judged_3dof_with_bad_proposals(ctx, oracle, case3):
+    starts = _starts(case3, 2, offset_every=4)
+    starts[5, 1] = np.inf                       # non-finite: that point's proposal fails
+    g, (gp, gj, gk), o = _run(ctx, oracle, case3, case3.params, starts, D.START_JUDGED)
+    ores, oprop, ojudge, okept, _ = o
+    _close(gp, oprop, "proposal pass")
+    _close(gj, ojudge, "judge pass")
+    _close(g, ores, "selected")
+    assert gp["status"][5] == D.RANGE_FAIL and gk[5] == 0
+    margin = np.abs(oprop["obj"] - ojudge["obj"] - 1e-6)
+    clear = (margin > 1e-7) | (oprop["status"] != 0) | (ojudge["status"] != 0)
+    assert np.array_equal(gk[clear], okept[clear])
+    print(f"  kept proposals {int((gk == 1).sum())} of {len(gk)}")
+
+
+def test_judged_12dof_prefix(ctx, oracle, case12):
+    p = dict(case12.params, disp_max=6)
+    g, (gp, gj, gk), o = _run(ctx, oracle, case12, p, _starts(case12, 3, offset_every=5), D.START_JUDGED, nproc=20)
+    ores, oprop, ojudge, okept, _ = o
+    _close(gp, oprop, "proposal pass")
+    _close(gj, ojudge, "judge pass")
+    _close(g, ores, "selected")
+    assert (gk[okept == -1] == -1).all() and (g["status"][okept == -1] == D.NOT_SEARCH).all()
+
+
+def test_starts_required_and_chain_unchanged(ctx, oracle, case3):
+    order, neigh = D.sort_order_neighbors(case3.points)
+    ctx.upload(0, case3.ref)
+    ctx.upload(1, case3.tar)
+    ctx.set_start_mode(D.START_GIVEN)
+    try:
+        ctx.prepare(D.make_params(**case3.params), case3.points, order, neigh)
+        with pytest.raises(D.DvcError):
+            ctx.execute()                        # no starts given yet
+        # dvc_search_points always runs the serial chain
+        chain = ctx.search(D.make_params(**case3.params), case3.points, order, neigh)
+    finally:
+        ctx.set_start_mode(D.START_CHAIN)
+    run = oracle.make_run(shape=case3.ref.shape, bytes_per=1, **case3.params)
+    ores, _, _ = oracle.dvc_run(run, case3.ref, case3.tar, case3.points, order, neigh, nthreads=8)
+    _close(chain, ores, "chain")

@@ -1,0 +1,113 @@
+"""Given-start and judged passes of the oracle, pinned to the reference's serial loop.
+
+The given-start pass (no reference counterpart) must reduce to Search::process_point
+(Core/Search.cpp:95-192) when each point is handed exactly the start guess that
+starting_param (Core/Search.cpp:277-377) computes inside the serial loop of
+Core/dvc.cpp:233-303; and a judge pass run on the serial loop's own results must
+reproduce those results.  Both are bitwise checks of the oracle's restatement.
+"""
+import numpy as np
+import pytest
+
+from digitalvolumecorrelation_amd import synth
+
+
+def chain_starts(res, order, neigh, ndof, rigid):
+    """starting_param of every point of the serial loop, from its finished results:
+    the mean translation of good neighbours at earlier positions (in neighbour order),
+    else rigid_trans; rotation and strain 0 (Search.cpp:277-377)."""
+    pos = np.empty(len(order), np.int64)
+    pos[order] = np.arange(len(order))
+    st = np.zeros((len(order), 12))
+    for i, n in enumerate(order):
+        sx = sy = sz = 0.0
+        good = 0
+        for m in neigh[n]:
+            if pos[m] < i and res["status"][m] == 0:
+                sx += res["par"][m, 0]; sy += res["par"][m, 1]; sz += res["par"][m, 2]; good += 1
+        if good:
+            st[n, :3] = (sx / good, sy / good, sz / good)
+        else:
+            st[n, :3] = rigid
+    return st
+
+
+@pytest.fixture(scope="module")
+def case():
+    return synth.config1(size=64, n=3, spacing=8)
+
+
+def _run(oracle, case, **kw):
+    return oracle.make_run(shape=case.ref.shape, bytes_per=case.ref.dtype.itemsize, **dict(case.params, **kw))
+
+
+def test_given_start_reduces_to_serial_loop(oracle, case):
+    run = _run(oracle, case)
+    order, neigh = oracle.sort_order_neighbors(case.points)
+    chain, _, _ = oracle.dvc_run(run, case.ref, case.tar, case.points, order, neigh)
+    starts = chain_starts(chain, order, neigh, 3, case.params["rigid_trans"])
+    res, _, _, _, _ = oracle.dvc_run_start(run, case.ref, case.tar, case.points, order, neigh, starts, mode=1)
+    assert np.array_equal(res["status"], chain["status"])
+    assert np.array_equal(res["iters"], chain["iters"])
+    assert np.array_equal(res["obj"], chain["obj"])
+    assert np.array_equal(res["par"], chain["par"])
+
+
+def test_judge_pass_on_serial_results_reproduces_them(oracle, case):
+    # the judge pass starts every point from starting_param over the given-start
+    # pass's results; when that pass is handed the serial loop's own starts its results
+    # are the serial loop's, so the judge pass must return them unchanged
+    run = _run(oracle, case)
+    order, neigh = oracle.sort_order_neighbors(case.points)
+    chain, _, _ = oracle.dvc_run(run, case.ref, case.tar, case.points, order, neigh)
+    starts = chain_starts(chain, order, neigh, 3, case.params["rigid_trans"])
+    res, prop, judge, kept, _ = oracle.dvc_run_start(run, case.ref, case.tar, case.points, order, neigh, starts,
+                                                     mode=2)
+    for r in (prop, judge, res):
+        assert np.array_equal(r["par"], chain["par"])
+        assert np.array_equal(r["status"], chain["status"])
+    # equal objectives: the given-start result is kept everywhere
+    assert (kept == 1).all()
+
+
+def test_bad_proposal_falls_back_to_judge(oracle, case):
+    # starts 4 voxels off the imposed translation on every axis, plus a non-finite one:
+    # the judge pass (reference start guess on the neighbours' results) decides
+    run = _run(oracle, case)
+    order, neigh = oracle.sort_order_neighbors(case.points)
+    starts = np.zeros((len(case.points), 12))
+    starts[:, :3] = np.asarray(case.truth)
+    starts[::3, :3] += 4.0
+    starts[1, 0] = np.nan
+    res, prop, judge, kept, _ = oracle.dvc_run_start(run, case.ref, case.tar, case.points, order, neigh, starts,
+                                                     mode=2)
+    assert prop["status"][1] == -1            # non-finite start: range failure of that pass
+    assert kept[1] == 0 and res["status"][1] == judge["status"][1]
+    # selection rule, point by point
+    for n in range(len(case.points)):
+        keep = prop["status"][n] == 0 and (judge["status"][n] != 0 or prop["obj"][n] <= judge["obj"][n] + 1e-6)
+        assert kept[n] == int(keep)
+        src = prop if keep else judge
+        assert res["status"][n] == src["status"][n] and np.array_equal(res["par"][n], src["par"][n])
+    # a selected result is good wherever either pass converged, and then accurate
+    good = res["status"] == 0
+    assert np.array_equal(good, (prop["status"] == 0) | (judge["status"] == 0))
+    # points handed the imposed translation converge to it in the given-start pass
+    # (the offset ones may settle in another local minimum or leave the volume)
+    near = np.ones(len(good), bool)
+    near[::3] = False
+    near[1] = False
+    assert (prop["status"][near] == 0).all()
+    assert np.abs(prop["par"][near, :3] - case.truth[near]).max() < 0.05
+
+
+def test_unsearched_points_are_marked(oracle, case):
+    run = _run(oracle, case)
+    order, neigh = oracle.sort_order_neighbors(case.points)
+    starts = np.zeros((len(case.points), 12))
+    starts[:, :3] = np.asarray(case.truth)
+    res, prop, judge, kept, _ = oracle.dvc_run_start(run, case.ref, case.tar, case.points, order, neigh, starts,
+                                                     mode=2, nproc=5)
+    rest = order[5:]
+    assert (kept[rest] == -1).all() and (res["status"][rest] == -3).all()
+    assert (res["status"][order[:5]] == 0).all()
