@@ -81,10 +81,13 @@ def proposal_leg(ctx, a, case, params, order, neigh, plain, D):
     if not wpath.exists():
         log(f"no proposal weights at {wpath}: NN-proposal leg skipped")
         return None
-    net = P.ProposalNet(ctx, max_batch=256, params=P.ProposalNet.load_params(wpath))
+    net = P.ProposalNet(ctx, max_batch=1024, params=P.ProposalNet.load_params(wpath))
     ctx.set_start_mode(D.START_JUDGED)
     ctx.prepare(params, case.points, order, neigh)
     ctx.set_start_mode(D.START_CHAIN)
+    net.propose()
+    tuned = ctx.autotune(2) if not a.threads else (ctx.block_threads(), None)
+    log(f"NN proposal: threads per point {tuned[0]}")
     for _ in range(a.warmup):
         net.propose()
         ctx.execute()
@@ -104,7 +107,7 @@ def proposal_leg(ctx, a, case, params, order, neigh, plain, D):
     pg = prop["status"] == 0
     rel = np.abs(res["obj"][both] - plain["obj"][both]) / np.abs(plain["obj"][both])
     return {"value": n / (wall_ms / 1e3), "unit": "points/s", "ms_per_step": wall_ms,
-            "speedup_vs_plain": None, "weights": wpath.name,
+            "speedup_vs_plain": None, "weights": wpath.name, "threads_per_point": tuned[0],
             "mode": "judged: proposal pass + judge pass from the reference start guess, keep-if-no-worse",
             "kept_proposal": int((kept == 1).sum()), "fallback_to_judge": int((kept == 0).sum()),
             "status_equal_plain": bool(np.array_equal(res["status"], plain["status"])),
@@ -149,6 +152,10 @@ def main():
         ctx.set_block_threads(a.threads)
     params = D.make_params(**p)
     ctx.prepare(params, case.points, order, neigh)
+    tuned = None
+    if not a.threads:
+        tuned = ctx.autotune(2)
+        log(f"autotuned threads per point: {tuned[0]} ({tuned[1]:.2f} ms per step)")
 
     for _ in range(a.warmup):
         ctx.execute()

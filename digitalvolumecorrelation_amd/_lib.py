@@ -67,6 +67,7 @@ def load() -> C.CDLL:
         "dvc_last_kernel_ms": (I, [VP, P(C.c_float)]),
         "dvc_volume_upload": (I, [VP, C.c_int, VP, C.c_int64, C.c_int, C.c_int, C.c_int64, C.c_int64, C.c_int64]),
         "dvc_volume_attach": (I, [VP, C.c_int, VP, C.c_int, C.c_int64, C.c_int64, C.c_int64]),
+        "dvc_volume_download": (I, [VP, C.c_int, VP]),
         "dvc_interpolate": (I, [VP, C.c_int, P(RunParams), P(D), P(D), I, P(D), C.c_int, P(D), P(D)]),
         "dvc_interpolate_box": (I, [VP, C.c_int, P(D), P(D), P(D), P(D), I, P(D), C.c_int, P(D), P(D)]),
         "dvc_objective": (I, [VP, C.c_int, P(D), P(D), I, P(D), P(D)]),
@@ -78,6 +79,7 @@ def load() -> C.CDLL:
         "dvc_set_launch_mode": (I, [VP, C.c_int]),
         "dvc_search_info": (I, [VP, C.c_void_p, C.c_void_p, C.c_void_p]),
         "dvc_set_start_mode": (I, [VP, C.c_int]),
+        "dvc_search_autotune": (I, [VP, C.c_int, P(I), P(C.c_float)]),
         "dvc_search_set_starts": (I, [VP, VP, C.c_int]),
         "dvc_search_fetch_passes": (I, [VP, VP, VP, VP]),
         "dvc_strain_fit": (I, [VP, P(D), P(D), P(I), P(D), P(I), I, I, I, D, I, I,
@@ -87,7 +89,7 @@ def load() -> C.CDLL:
         "dvc_net_last_error": (C.c_char_p, [VP]),
         "dvc_net_num_params": (C.c_int64, []),
         "dvc_net_params": (I, [VP, P(C.c_float), C.c_int64, C.c_int]),
-        "dvc_net_crop": (I, [VP, VP, VP, I, VP, VP]),
+        "dvc_net_crop": (I, [VP, VP, VP, I, VP, VP, VP]),
         "dvc_device_alloc": (I, [VP, P(VP), C.c_int64]),
         "dvc_device_free": (I, [VP, VP]),
         "dvc_device_copy": (I, [VP, VP, VP, C.c_int64, C.c_int]),
@@ -96,6 +98,7 @@ def load() -> C.CDLL:
         "dvc_net_gradients": (I, [VP, VP, VP, I, P(C.c_float), P(C.c_float)]),
         "dvc_net_train_step": (I, [VP, VP, VP, I, C.c_float, P(C.c_float)]),
         "dvc_search_propose": (I, [VP, VP]),
+        "dvc_synth_warp_tiles": (I, [VP, C.c_int, P(D), C.c_int64]),
         "dvc_sort_order_neighbors": (I, [P(D), I, P(D), P(I), P(I), I]),
         "dvc_knn_order_neighbors": (I, [P(D), I, P(D), I, P(I), P(I), I]),
     }
@@ -157,8 +160,19 @@ class Context:
         if rc != 0:
             raise DvcError(rc, "dvc_create failed (a gfx950 GPU is required)")
         self.h = h
+        self._deps = []            # weak references to networks and device arrays of this context
+
+    def _adopt(self, obj):
+        import weakref
+        self._deps.append(weakref.ref(obj))
 
     def close(self):
+        # networks and device arrays first: their destructors queue on this context
+        for r in getattr(self, "_deps", []):
+            o = r()
+            if o is not None:
+                o.close() if hasattr(o, "close") else o.free()
+        self._deps = []
         if getattr(self, "h", None):
             self.lib.dvc_destroy(self.h)
             self.h = None
@@ -196,6 +210,17 @@ class Context:
         raw = np.frombuffer(header + vol.tobytes(), dtype=np.uint8) if header else vol.view(np.uint8).reshape(-1)
         self._check(self.lib.dvc_volume_upload(self.h, slot, raw.ctypes.data_as(C.c_void_p), len(header),
                                                vol.dtype.itemsize, int(big_endian), w, h, t))
+
+    def download(self, slot: int, shape, dtype) -> np.ndarray:
+        """The volume in `slot` (shape (tall, high, wide)) copied back to the host."""
+        out = np.empty(shape, dtype)
+        self._check(self.lib.dvc_volume_download(self.h, slot, out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def warp_tiles(self, tile: int, maps: np.ndarray):
+        """dvc_synth_warp_tiles: slot 1 = slot 0 warped tile by tile (maps [ntiles, 12])."""
+        m = np.ascontiguousarray(maps, dtype=np.float64)
+        self._check(self.lib.dvc_synth_warp_tiles(self.h, int(tile), _dp(m), len(m)))
 
     def attach(self, slot: int, dev_ptr: int, bytes_per: int, shape):
         t, h, w = shape
@@ -239,6 +264,12 @@ class Context:
         self._npts = self._pts.shape[0]
         self._check(self.lib.dvc_search_prepare(self.h, C.byref(params), _dp(self._pts), self._npts, _ip(self._ord),
                                                 nproc, _ip(self._nb), self._nb.shape[1]))
+
+    def autotune(self, reps: int = 3):
+        """Pick the fastest threads-per-point for the prepared search; (threads, ms)."""
+        bt, ms = C.c_int32(), C.c_float()
+        self._check(self.lib.dvc_search_autotune(self.h, reps, C.byref(bt), C.byref(ms)))
+        return bt.value, ms.value
 
     def set_start_mode(self, mode: int):
         """START_CHAIN (the reference's serial start guess), START_GIVEN or START_JUDGED;
@@ -328,6 +359,7 @@ class DeviceArray:
         p = C.c_void_p()
         ctx._check(ctx.lib.dvc_device_alloc(ctx.h, C.byref(p), self.nbytes))
         self.ptr = int(p.value or 0)
+        ctx._adopt(self)
 
     @classmethod
     def from_host(cls, ctx: Context, arr: np.ndarray) -> "DeviceArray":

@@ -533,6 +533,16 @@ static int enqueue_search(dvc_ctx *ctx, hipStream_t st) {
     return DVC_OK;
 }
 
+int dvc_volume_download(dvc_ctx *ctx, int slot, void *host) {
+    if (!ctx || !host || slot < 0 || slot > 1) return DVC_ERR_ARG;
+    const Volume &v = ctx->vol[slot];
+    if (!v.d) return fail(ctx, DVC_ERR_STATE, "no volume in that slot");
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipMemcpyAsync(host, v.d, (size_t)(v.W * v.H * v.T * v.bpv), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return DVC_OK;
+}
+
 int dvc_device_alloc(dvc_ctx *ctx, void **ptr, int64_t bytes) {
     if (!ctx || !ptr || bytes < 0) return DVC_ERR_ARG;
     *ptr = nullptr;
@@ -637,6 +647,41 @@ int dvc_search_execute(dvc_ctx *ctx) {
     }
     HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
     ctx->timing_pending = true;
+    return DVC_OK;
+}
+
+int dvc_search_autotune(dvc_ctx *ctx, int reps, int32_t *best_threads, float *best_ms) {
+    if (!ctx || reps < 1 || reps > 100) return DVC_ERR_ARG;
+    if (!ctx->plan_ready) return fail(ctx, DVC_ERR_STATE, "dvc_search_prepare first");
+    if (ctx->plan_mode != DVC_START_CHAIN && !ctx->starts_set)
+        return fail(ctx, DVC_ERR_STATE, "dvc_search_set_starts first");
+    const int bpv = ctx->vol[0].bpv;
+    const int nsamp_pad = (ctx->geo.subv_num + 3) & ~3;
+    const int cands[5] = {64, 128, 256, 512, 1024};
+    int best = 0;
+    float bms = 0.0f;
+    for (int c : cands) {
+        size_t st = 0;
+        if (search_static_lds(bpv, c, &st) != 0) return fail(ctx, DVC_ERR_HIP, "kernel attributes");
+        if (search_lds_bytes(nsamp_pad, c, ctx->d_rvals == nullptr) + st > ctx->lds_max) continue;
+        ctx->chosen_threads = c;
+        drop_graph(ctx);
+        if (int e = dvc_search_execute(ctx)) return e;          // warm-up, captures the graph
+        float tot = 0.0f;
+        for (int r = 0; r < reps; r++) {
+            if (int e = dvc_search_execute(ctx)) return e;
+            float ms = 0.0f;
+            if (int e = dvc_last_kernel_ms(ctx, &ms)) return e;
+            tot += ms;
+        }
+        const float avg = tot / (float)reps;
+        if (best == 0 || avg < bms) { best = c; bms = avg; }
+    }
+    if (best == 0) return fail(ctx, DVC_ERR_STATE, "no workgroup size fits");
+    ctx->chosen_threads = best;
+    drop_graph(ctx);
+    if (best_threads) *best_threads = best;
+    if (best_ms) *best_ms = bms;
     return DVC_OK;
 }
 

@@ -48,7 +48,7 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
 // repeat the nearest face voxel.
 __global__ void __launch_bounds__(256) crop_kernel(const uint8_t *v0, const uint8_t *v1, int bpv, int W, int H, int T,
                                                    const double *pts, const int *idx, int n, uint16_t *out,
-                                                   const int *slot) {
+                                                   const int *slot, const int *flip) {
     __shared__ double red[2][4][8];
     __shared__ float stat[4];
     const int p = blockIdx.x;
@@ -58,8 +58,10 @@ __global__ void __launch_bounds__(256) crop_kernel(const uint8_t *v0, const uint
     const int oy = (int)floor(pts[3 * (size_t)pt + 1]) - 15;
     const int oz = (int)floor(pts[3 * (size_t)pt + 2]) - 15;
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int fl = flip ? flip[p] : 0;          // bit 0/1/2: mirror the crop along x/y/z
     auto val = [&](const uint8_t *v, int i) -> float {
-        int x = ox + (i & 31), y = oy + ((i >> 5) & 31), z = oz + (i >> 10);
+        const int cx = i & 31, cy = (i >> 5) & 31, cz = i >> 10;
+        int x = ox + ((fl & 1) ? 31 - cx : cx), y = oy + ((fl & 2) ? 31 - cy : cy), z = oz + ((fl & 4) ? 31 - cz : cz);
         x = x < 0 ? 0 : (x >= W ? W - 1 : x);
         y = y < 0 ? 0 : (y >= H ? H - 1 : y);
         z = z < 0 ? 0 : (z >= T ? T - 1 : z);
@@ -212,6 +214,85 @@ __global__ void __launch_bounds__(256) gemm_nt_kernel(GemmArgs g) {
             if (g.relu) v = v > 0.0f ? v : 0.0f;
             if (g.outf) g.outf[(long long)m * g.ldf + n] = v;
             if (g.outb) g.outb[(long long)m * g.ldb16 + n] = f2bf(v);
+        }
+}
+
+// The forward convolution as an implicit GEMM: the same tiles and lane mapping as
+// gemm_nt_kernel, but each lane gathers its 8 patch values of A straight from the
+// channels-last input [B][D][D][D][C] (stride 2, padding 1), so the patch matrix is
+// never stored.  C a multiple of 8: one 16-byte load per lane and k-step (8 channels
+// of one tap); C = 2: four 4-byte loads (2 channels of 4 taps).  Taps past 27 and
+// voxels outside the input read as zero.  Epilogue: bias, ReLU, BF16.
+template <int NB, int C>
+__global__ void __launch_bounds__(256) conv_fwd_kernel(const uint16_t *in, int B, int D, const uint16_t *Wb, int Kp,
+                                                       int N, const float *bias, uint16_t *out) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int Do = D >> 1;
+    const int M = B * Do * Do * Do;
+    const int row0 = blockIdx.x * 64 + wave * 16;
+    const int col0 = blockIdx.y * (16 * NB);
+    const int kq = (lane >> 4) * 8;
+    const int arow = row0 + (lane & 15);
+    const bool aok = arow < M;
+    int b = 0, oz = 0, oy = 0, ox = 0;
+    if (aok) {
+        const int d3 = Do * Do * Do;
+        b = arow / d3;
+        const int r = arow - b * d3;
+        oz = r / (Do * Do); oy = (r / Do) % Do; ox = r % Do;
+    }
+    const uint16_t *base = in + (long long)b * D * D * D * C;
+    floatx4 acc[NB];
+#pragma unroll
+    for (int t = 0; t < NB; t++) acc[t] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+    const uint4 zero = {0u, 0u, 0u, 0u};
+    const uint16_t *bp[NB];
+    bool bok[NB];
+#pragma unroll
+    for (int t = 0; t < NB; t++) {
+        const int ncol = col0 + t * 16 + (lane & 15);
+        bok[t] = ncol < N;
+        bp[t] = Wb + (long long)(bok[t] ? ncol : 0) * Kp + kq;
+    }
+    auto voxel_ptr = [&](int tap, bool &ok) -> const uint16_t * {
+        const int kz = tap / 9, ky = (tap / 3) % 3, kx = tap % 3;
+        const int z = 2 * oz + kz - 1, y = 2 * oy + ky - 1, x = 2 * ox + kx - 1;
+        ok = aok && tap < 27 && z >= 0 && z < D && y >= 0 && y < D && x >= 0 && x < D;
+        return base + (((long long)z * D + y) * D + x) * C;
+    };
+    for (int k = 0; k < Kp; k += 32) {
+        uint4 av;
+        if (C % 8 == 0) {
+            const int kk = k + kq;
+            bool ok;
+            const uint16_t *ptr = voxel_ptr(kk / C, ok);
+            av = ok ? *(const uint4 *)(ptr + (kk % C)) : zero;
+        } else {
+            uint32_t w[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                bool ok;
+                const uint16_t *ptr = voxel_ptr((k + kq) / 2 + j, ok);
+                w[j] = ok ? *(const uint32_t *)ptr : 0u;
+            }
+            av = uint4{w[0], w[1], w[2], w[3]};
+        }
+        const bf16x8 a8 = __builtin_bit_cast(bf16x8, av);
+#pragma unroll
+        for (int t = 0; t < NB; t++) {
+            const uint4 bv = bok[t] ? *(const uint4 *)(bp[t] + k) : zero;
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a8, __builtin_bit_cast(bf16x8, bv), acc[t], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < NB; t++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int m = row0 + 4 * (lane >> 4) + r;
+            const int n = col0 + t * 16 + (lane & 15);
+            if (m >= M || n >= N) continue;
+            float v = acc[t][r] + bias[n];
+            out[(long long)m * N + n] = f2bf(v > 0.0f ? v : 0.0f);
         }
 }
 
@@ -409,6 +490,59 @@ __global__ void pack_kernel(const float *W, int Co, int K, int Kp, uint16_t *Wb,
     WbT[(long long)k * Co + co] = h;
 }
 
+// ------------------------------------------------------------------ training pairs
+
+// Target volume of a training pair: every tile x tile x tile block t of the volume is
+// the reference resampled under that tile's own affine map, out(x) = ref(M_t x + o_t)
+// (maps[t] = M row-major, then o), by separable Catmull-Rom (the interior form of the
+// path's tricubic interpolant) with indices clamped to the volume, rounded to the
+// nearest integer and clamped to the voxel type's range.  Tiles are numbered x fastest.
+template <int BPV>
+__global__ void warp_tiles_kernel(const uint8_t *src, uint8_t *dst, int W, int H, int T, int tile, int ntx, int nty,
+                                  const double *maps) {
+    const long long n = (long long)W * H * T;
+    const long long st = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += st) {
+        const int x = (int)(i % W), y = (int)((i / W) % H), z = (int)(i / ((long long)W * H));
+        const int t = (z / tile * nty + y / tile) * ntx + x / tile;
+        const double *m = maps + (long long)t * 12;
+        const double X = m[0] * x + m[1] * y + m[2] * z + m[9];
+        const double Y = m[3] * x + m[4] * y + m[5] * z + m[10];
+        const double Z = m[6] * x + m[7] * y + m[8] * z + m[11];
+        const double fx = floor(X), fy = floor(Y), fz = floor(Z);
+        const int ix = (int)fx, iy = (int)fy, iz = (int)fz;
+        double w[3][4];
+        const double tt[3] = {X - fx, Y - fy, Z - fz};
+        for (int a = 0; a < 3; a++) {
+            const double u = tt[a], u2 = u * u, u3 = u2 * u;
+            w[a][0] = 0.5 * (-u3 + 2 * u2 - u);
+            w[a][1] = 0.5 * (3 * u3 - 5 * u2 + 2);
+            w[a][2] = 0.5 * (-3 * u3 + 4 * u2 + u);
+            w[a][3] = 0.5 * (u3 - u2);
+        }
+        double acc = 0.0;
+        for (int c = 0; c < 4; c++) {
+            const int zz = min(max(iz - 1 + c, 0), T - 1);
+            for (int b = 0; b < 4; b++) {
+                const int yy = min(max(iy - 1 + b, 0), H - 1);
+                const long long row = ((long long)zz * H + yy) * W;
+                double r = 0.0;
+                for (int a = 0; a < 4; a++) {
+                    const int xx = min(max(ix - 1 + a, 0), W - 1);
+                    const double v = BPV == 1 ? (double)src[row + xx] : (double)((const uint16_t *)src)[row + xx];
+                    r += w[0][a] * v;
+                }
+                acc += w[2][c] * w[1][b] * r;
+            }
+        }
+        const double top = BPV == 1 ? 255.0 : 65535.0;
+        double v = rint(acc);
+        v = v < 0.0 ? 0.0 : (v > top ? top : v);
+        if (BPV == 1) dst[i] = (uint8_t)v;
+        else ((uint16_t *)dst)[i] = (uint16_t)v;
+    }
+}
+
 // ------------------------------------------------------------------ layout
 
 constexpr int kLayers = 4;
@@ -524,9 +658,32 @@ int transpose(dvc_net *n, hipStream_t st, const uint16_t *in, long long R, int C
     return launch_ok(n, "transpose");
 }
 
+int conv_fwd(dvc_net *n, hipStream_t st, int l, int B) {
+    const int D = kDin[l], N = kCout[l], Kp = kKp(l);
+    const long long M = layer_rows(l, B);
+    if (M > 2147483647LL) return net_fail(n, DVC_ERR_ARG, "batch too large");
+    const int nb = N >= 64 ? 4 : (N >= 32 ? 2 : 1);
+    dim3 grid((unsigned)((M + 63) / 64), (unsigned)((N + 16 * nb - 1) / (16 * nb)));
+    const uint16_t *in = n->act[l];
+    const float *bias = n->p + n->L.b[l];
+    uint16_t *out = n->act[l + 1];
+    switch (l) {
+        case 0: hipLaunchKernelGGL((conv_fwd_kernel<1, 2>), grid, dim3(256), 0, st, in, B, D, n->wb[l], Kp, N, bias, out); break;
+        case 1: hipLaunchKernelGGL((conv_fwd_kernel<2, 16>), grid, dim3(256), 0, st, in, B, D, n->wb[l], Kp, N, bias, out); break;
+        case 2: hipLaunchKernelGGL((conv_fwd_kernel<4, 32>), grid, dim3(256), 0, st, in, B, D, n->wb[l], Kp, N, bias, out); break;
+        default: hipLaunchKernelGGL((conv_fwd_kernel<4, 64>), grid, dim3(256), 0, st, in, B, D, n->wb[l], Kp, N, bias, out); break;
+    }
+    return launch_ok(n, "conv forward");
+}
+
 int forward(dvc_net *n, hipStream_t st, int B, double *par, const int *idx) {
     n->act[0] = n->in;
     for (int l = 0; l < kLayers; l++) {
+        if (!n->training) {
+            // inference: implicit GEMM, no patch matrix
+            if (int e = conv_fwd(n, st, l, B)) return e;
+            continue;
+        }
         if (int e = im2col(n, st, n->act[l], B, l, n->col[l])) return e;
         GemmArgs a{};
         a.A = n->col[l]; a.lda = kKp(l);
@@ -629,7 +786,7 @@ int dvc_net_create(dvc_ctx *ctx, int max_batch, int training, dvc_net **out) {
     for (int l = 0; l < kLayers && !e; l++) {
         e = net_alloc(n, &n->wb[l], (size_t)kCout[l] * kKp(l));
         e = e ? e : net_alloc(n, &n->wbT[l], (size_t)kCout[l] * kKp(l));
-        e = e ? e : net_alloc(n, &n->col[l], (size_t)layer_rows(l, B) * kKp(l));
+        if (training) e = e ? e : net_alloc(n, &n->col[l], (size_t)layer_rows(l, B) * kKp(l));
         e = e ? e : net_alloc(n, &n->act[l + 1], (size_t)layer_rows(l, B) * kCout[l]);
     }
     e = e ? e : net_alloc(n, &n->feat, (size_t)B * kFeat);
@@ -710,7 +867,7 @@ int dvc_net_params(dvc_net *n, float *host, int64_t count, int set) {
 }
 
 int dvc_net_crop(dvc_net *n, const double *points_dev, const int32_t *idx_dev, int32_t count, void *out_dev,
-                 const int32_t *slot_dev) {
+                 const int32_t *slot_dev, const int32_t *flip_dev) {
     if (!n || !points_dev || count < 0) return DVC_ERR_ARG;
     if (count == 0) return DVC_OK;
     const uint8_t *v0, *v1;
@@ -720,7 +877,7 @@ int dvc_net_crop(dvc_net *n, const double *points_dev, const int32_t *idx_dev, i
     uint16_t *out = out_dev ? (uint16_t *)out_dev : n->in;
     if (!out_dev && (count > n->cap || slot_dev)) return net_fail(n, DVC_ERR_ARG, "more crops than the batch capacity");
     hipLaunchKernelGGL(crop_kernel, dim3(count), dim3(256), 0, st, v0, v1, bpv, W, H, T, points_dev, idx_dev, count, out,
-                       slot_dev);
+                       slot_dev, flip_dev);
     return launch_ok(n, "crop");
 }
 
@@ -781,6 +938,34 @@ int dvc_net_train_step(dvc_net *n, const void *in_dev, const float *label_dev, i
     return DVC_OK;
 }
 
+int dvc_synth_warp_tiles(dvc_ctx *ctx, int tile, const double *maps, int64_t nmaps) {
+    if (!ctx || !maps || tile < 1) return DVC_ERR_ARG;
+    const uint8_t *v0, *v1;
+    int bpv, W, H, T;
+    if (ctx_volumes(ctx, &v0, &v1, &bpv, &W, &H, &T) != 0) return DVC_ERR_STATE;
+    const int ntx = (W + tile - 1) / tile, nty = (H + tile - 1) / tile, ntz = (T + tile - 1) / tile;
+    if (nmaps != (int64_t)ntx * nty * ntz) return DVC_ERR_ARG;
+    hipStream_t st = (hipStream_t)ctx_stream(ctx);
+    if (hipSetDevice(ctx_device(ctx)) != hipSuccess) return DVC_ERR_HIP;
+    double *d = nullptr;
+    if (hipMalloc(&d, sizeof(double) * 12 * nmaps) != hipSuccess) return DVC_ERR_HIP;
+    int rc = DVC_OK;
+    if (hipMemcpyAsync(d, maps, sizeof(double) * 12 * nmaps, hipMemcpyHostToDevice, st) != hipSuccess) rc = DVC_ERR_HIP;
+    const long long n = (long long)W * H * T;
+    if (rc == DVC_OK) {
+        if (bpv == 1)
+            hipLaunchKernelGGL(warp_tiles_kernel<1>, dim3(grid_for(n, 256, 1 << 20)), dim3(256), 0, st, v0,
+                               (uint8_t *)v1, W, H, T, tile, ntx, nty, d);
+        else
+            hipLaunchKernelGGL(warp_tiles_kernel<2>, dim3(grid_for(n, 256, 1 << 20)), dim3(256), 0, st, v0,
+                               (uint8_t *)v1, W, H, T, tile, ntx, nty, d);
+        if (hipGetLastError() != hipSuccess) rc = DVC_ERR_HIP;
+    }
+    if (hipStreamSynchronize(st) != hipSuccess) rc = DVC_ERR_HIP;
+    hipFree(d);
+    return rc;
+}
+
 int dvc_search_propose(dvc_ctx *ctx, dvc_net *n) {
     if (!ctx || !n || n->ctx != ctx) return DVC_ERR_ARG;
     const double *pts;
@@ -792,7 +977,7 @@ int dvc_search_propose(dvc_ctx *ctx, dvc_net *n) {
     hipStream_t st = (hipStream_t)ctx_stream(ctx);
     for (int i0 = 0; i0 < nproc; i0 += n->cap) {
         const int B = std::min(n->cap, nproc - i0);
-        if (int e = dvc_net_crop(n, pts, order + i0, B, nullptr, nullptr)) return e;
+        if (int e = dvc_net_crop(n, pts, order + i0, B, nullptr, nullptr, nullptr)) return e;
         if (int e = forward(n, st, B, start, order + i0)) return e;
     }
     ctx_mark_starts(ctx);

@@ -61,12 +61,15 @@ class ProposalNet:
         if rc != 0:
             raise DvcError(rc, "dvc_net_create failed")
         self.h = h
+        ctx._adopt(self)
         self.set_params(init_params(0) if params is None else params)
 
     def close(self):
-        if getattr(self, "h", None):
+        # a network must be destroyed before its context (dvc_net_destroy waits on the
+        # context's stream); Context.close() closes its networks first
+        if getattr(self, "h", None) and getattr(self.ctx, "h", None):
             self.lib.dvc_net_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:
@@ -100,11 +103,12 @@ class ProposalNet:
 
     # ---- kernels
     def crop(self, points_dev: int, count: int, out_dev: int | None = None, idx_dev: int | None = None,
-             slot_dev: int | None = None):
+             slot_dev: int | None = None, flip_dev: int | None = None):
         """Crops around device points (float64 [n, 3]) into out_dev (BF16 [slots, 32^3, 2]),
-        crop i at slot slot_dev[i] (default i)."""
+        crop i at slot slot_dev[i] (default i), mirrored along the axes of flip_dev[i]."""
         v = lambda x: C.c_void_p(x) if x else None  # noqa: E731
-        self._check(self.lib.dvc_net_crop(self.h, C.c_void_p(points_dev), v(idx_dev), count, v(out_dev), v(slot_dev)))
+        self._check(self.lib.dvc_net_crop(self.h, C.c_void_p(points_dev), v(idx_dev), count, v(out_dev), v(slot_dev),
+                                          v(flip_dev)))
 
     def forward(self, in_dev: int | None, B: int, par_dev: int | None = None, y_dev: int | None = None):
         self._check(self.lib.dvc_net_forward(self.h, C.c_void_p(in_dev) if in_dev else None, B,
@@ -144,17 +148,6 @@ class ProposalNet:
 
 # ---------------------------------------------------------------- training data
 
-@dataclass
-class TrainVolume:
-    """One synthetic pair with a random smooth deformation: target(x) = ref(psi(x)),
-    psi = the inverse of an affine map (translation, rotation about a random axis
-    through the centre, symmetric strain) minus a sinusoidal field."""
-    ref: np.ndarray
-    tar: np.ndarray
-    points: np.ndarray
-    start_disp: np.ndarray      # displacement at the first point of the order (for rigid_trans)
-
-
 def _rotation(axis, deg):
     a = np.asarray(axis, dtype=np.float64)
     a = a / np.linalg.norm(a)
@@ -163,57 +156,96 @@ def _rotation(axis, deg):
     return np.eye(3) + np.sin(t) * K + (1 - np.cos(t)) * K @ K
 
 
-def train_volume(seed: int, size: int = 128, spacing: float = 3.0, margin: float = 31.0,
-                 trans=8.0, rot_deg=3.0, strain=0.02, amp=1.0, wavelength=64.0) -> TrainVolume:
+@dataclass
+class TiledPair:
+    """A training pair whose target is the reference warped tile by tile, each tile
+    under its own affine map phi_t(X) = c_t + t_t + F_t (X - c_t) about the tile centre:
+    thousands of independent deformations per volume pair."""
+    ref: np.ndarray
+    maps: np.ndarray          # [ntiles, 12] inverse maps (dvc_synth_warp_tiles)
+    points: np.ndarray        # [n, 3]
+    disp: np.ndarray          # [n, 3] imposed displacement phi_t(X) - X at each point
+    tile_of: np.ndarray       # [n] tile index
+
+
+def reference_volume(seed: int, size=384):
+    """Smoothed-noise uint8 reference (sigma 1.5-2.5 drawn from the seed)."""
     from . import synth
+    rng = np.random.default_rng(30_000 + seed)
+    return synth.smoothed_noise((size,) * 3, rng.uniform(1.5, 2.5), np.uint8, seed=20_000 + seed)
+
+
+def tiled_pair(seed: int, size=384, tile=64, trans=8.0, rot_deg=3.0, strain=0.02, spacing=3.0, per_axis=5, ref=None):
+    """Reference volume (smoothed noise, sigma 1.5-2.5, or `ref`) and per-tile affine
+    maps drawn from `seed`: translation up to `trans` voxels per axis, rotation up to
+    `rot_deg` about a random axis through the tile centre, symmetric strain up to
+    `strain`; per_axis^3 points around each tile centre at `spacing`."""
     rng = np.random.default_rng(seed)
-    sigma = rng.uniform(1.5, 2.5)
-    ref = synth.smoothed_noise((size,) * 3, sigma, np.uint8, seed=10_000 + seed)
-    c = np.full(3, (size - 1) / 2.0)
-    t = rng.uniform(-trans, trans, 3)
-    R = _rotation(rng.normal(size=3), rng.uniform(-rot_deg, rot_deg))
-    E = rng.uniform(-strain, strain, (3, 3))
-    E = 0.5 * (E + E.T)
-    F = R @ (np.eye(3) + E)
-    Fi = np.linalg.inv(F)
-    k = rng.normal(size=(3, 3))
-    k = k / np.linalg.norm(k, axis=1, keepdims=True) * (2 * np.pi / wavelength)
-    ph = rng.uniform(0, 2 * np.pi, 3)
-
-    def psi(x):
-        X = c + (x - c - t) @ Fi.T
-        return X - amp * np.sin(x @ k.T + ph)
-
-    tar = synth.warp(ref, psi)
-    n = int(np.floor((size - 1 - 2 * margin) / spacing)) + 1
-    off = margin + np.arange(n) * spacing
-    z, y, x = np.meshgrid(off, off, off, indexing="ij")
-    pts = np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1)
-    # displacement at the volume centre: x with psi(x) = c, by fixed-point iteration
-    xc = c.copy()
-    for _ in range(50):
-        xc = xc + (c - psi(xc[None])[0])
-    return TrainVolume(ref, tar, pts, xc - c)
+    if ref is None:
+        ref = reference_volume(seed, size)
+    size = ref.shape[0]
+    nt = (size + tile - 1) // tile
+    maps, pts, disp, tix = [], [], [], []
+    g = (np.arange(per_axis) - (per_axis - 1) / 2.0) * spacing
+    gz, gy, gx = np.meshgrid(g, g, g, indexing="ij")
+    local = np.stack([gx.ravel(), gy.ravel(), gz.ravel()], axis=1)
+    for tz in range(nt):
+        for ty in range(nt):
+            for tx in range(nt):
+                c = np.array([tx, ty, tz], dtype=np.float64) * tile + (tile - 1) / 2.0
+                t = rng.uniform(-trans, trans, 3)
+                E = rng.uniform(-strain, strain, (3, 3))
+                F = _rotation(rng.normal(size=3), rng.uniform(-rot_deg, rot_deg)) @ (np.eye(3) + 0.5 * (E + E.T))
+                Fi = np.linalg.inv(F)
+                maps.append(np.concatenate([Fi.ravel(), c - Fi @ (c + t)]))
+                X = c + local
+                pts.append(X)
+                disp.append(t + (X - c) @ (F - np.eye(3)).T)
+                tix.append(np.full(len(X), len(maps) - 1))
+    return TiledPair(ref, np.array(maps), np.concatenate(pts), np.concatenate(disp), np.concatenate(tix))
 
 
-LABEL_PARAMS = dict(ndof=12, obj="znssd", interp="tricubic", geom="cube", subvol_size=31, subvol_npts=2197,
+LABEL_PARAMS = dict(ndof=12, obj="znssd", interp="tricubic", geom="cube", subvol_size=21, subvol_npts=1331,
                     disp_max=12, basin_radius=0.0)
 
 
-def label_volume(ctx: Context, tv: TrainVolume, params=None):
-    """Converged 12-DOF warps of the plain search (the serial chain from the volume
-    centre) at the volume's points: (points, labels [n, 12]) of the points whose search
-    converged (good status, fewer than 20 Gauss-Newton evaluations)."""
+def label_pair(ctx: Context, tp: TiledPair, tile=64, params=None):
+    """Upload the pair (target made on the device) and label its points with the plain
+    search's converged 12-DOF warps, each point started from its tile's imposed
+    translation (DVC_START_GIVEN).  Keeps points that converged (good status, fewer than
+    20 Gauss-Newton evaluations) within half a voxel of the imposed displacement.
+    Leaves the pair in the context; returns (points, labels [n, 12])."""
     p = dict(LABEL_PARAMS if params is None else params)
-    p["rigid_trans"] = tuple(float(np.rint(v)) for v in tv.start_disp)
-    size = tv.ref.shape[0]
-    centre = np.full(3, (size - 1) / 2.0)
-    order, neigh = _lib.knn_order_neighbors(tv.points, start=centre)
-    ctx.upload(0, tv.ref)
-    ctx.upload(1, tv.tar)
-    res = ctx.search(_lib.make_params(**p), tv.points, order, neigh)
-    ok = (res["status"] == 0) & (res["iters"] < 20)
-    return tv.points[ok], res["par"][ok].copy()
+    ctx.upload(0, tp.ref)
+    ctx.upload(1, tp.ref)
+    ctx.warp_tiles(tile, tp.maps)
+    order, neigh = _lib.knn_order_neighbors(tp.points, k=8)
+    starts = np.zeros((len(tp.points), 12))
+    starts[:, :3] = tp.disp
+    ctx.set_start_mode(_lib.START_GIVEN)
+    try:
+        ctx.prepare(_lib.make_params(**p), tp.points, order, neigh)
+        ctx.set_starts(starts)
+        ctx.execute()
+        res = ctx.fetch()
+    finally:
+        ctx.set_start_mode(_lib.START_CHAIN)
+    ok = (res["status"] == 0) & (res["iters"] < 20) & (np.abs(res["par"][:, :3] - tp.disp).max(axis=1) < 0.5)
+    return tp.points[ok], res["par"][ok].copy()
+
+
+def flip_labels(labels: np.ndarray, flips: np.ndarray) -> np.ndarray:
+    """Warp parameters of mirrored crop pairs.  Mirroring axes with signs s = diag(D)
+    maps the warp's rotation and strain to D A D and D S D: u_a -> s_a u_a; the angles
+    phi, theta, psi (rotation in the xy, xz, yz planes, SearchParams.cpp:53-107) and the
+    shears exy, eyz, exz take the product of their two axes' signs; normal strains keep
+    theirs."""
+    sx = np.where(flips & 1, -1.0, 1.0)
+    sy = np.where(flips & 2, -1.0, 1.0)
+    sz = np.where(flips & 4, -1.0, 1.0)
+    one = np.ones_like(sx)
+    f = np.stack([sx, sy, sz, sx * sy, sx * sz, sy * sz, one, one, one, sx * sy, sy * sz, sx * sz], axis=1)
+    return labels * f
 
 
 @dataclass
@@ -224,45 +256,56 @@ class TrainSet:
     n: int
 
 
-def build_train_set(ctx: Context, net: ProposalNet, seeds, size=128, spacing=3.0, log=print, shuffle_seed=0):
-    """Crops and plain-search labels of the training volumes, resident in HBM, stored
-    in a random order over all volumes (so every contiguous batch mixes them)."""
-    vols, labels = [], []
-    for s in seeds:
+def build_train_set(ctx: Context, net: ProposalNet, seeds, size=384, tile=64, log=print, shuffle_seed=0,
+                    maps_per_ref=1, per_axis=5, spacing=3.0, flip=True):
+    """Crops and plain-search labels of tiled pairs, resident in HBM, in a random order
+    over all pairs (every contiguous batch mixes them).  Each reference volume serves
+    `maps_per_ref` pairs with different tile maps, so a texture seen in training comes
+    with several deformations and cannot stand for one."""
+    pairs, labels = [], []
+    ref = None
+    for i, s in enumerate(seeds):
         t0 = time.time()
-        tv = train_volume(int(s), size=size, spacing=spacing)
-        pts, lab = label_volume(ctx, tv)
-        vols.append((tv, pts))
+        if i % maps_per_ref == 0:
+            ref = reference_volume(int(s), size)
+        tp = tiled_pair(int(s), size=size, tile=tile, per_axis=per_axis, spacing=spacing, ref=ref)
+        pts, lab = label_pair(ctx, tp, tile)
+        pairs.append((tp, pts))
         labels.append(lab)
-        log(f"training volume {s}: {len(pts)} labelled points of {len(tv.points)} ({time.time() - t0:.1f} s)")
+        log(f"training pair {s}: {len(pts)} labelled points of {len(tp.points)} in {len(tp.maps)} tiles "
+            f"({time.time() - t0:.1f} s)")
     n = sum(len(l) for l in labels)
-    perm = np.random.default_rng(shuffle_seed).permutation(n).astype(np.int32)
+    rng = np.random.default_rng(shuffle_seed)
+    perm = rng.permutation(n).astype(np.int32)
+    flips = rng.integers(0, 8, n).astype(np.int32) if flip else np.zeros(n, np.int32)
     crops = DeviceArray(ctx, (n, CROP_ELEMS), np.uint16)
-    lab_all = np.concatenate(labels).astype(np.float32)
+    lab_all = flip_labels(np.concatenate(labels), flips).astype(np.float32)
     lab_store = np.empty_like(lab_all)
     lab_store[perm] = lab_all                 # sample i is stored at slot perm[i]
     off = 0
-    for tv, pts in vols:
-        ctx.upload(0, tv.ref)
-        ctx.upload(1, tv.tar)
+    for tp, pts in pairs:
+        ctx.upload(0, tp.ref)
+        ctx.upload(1, tp.ref)
+        ctx.warp_tiles(tile, tp.maps)
         dp = DeviceArray.from_host(ctx, pts)
         slots = DeviceArray.from_host(ctx, perm[off:off + len(pts)])
-        net.crop(dp.ptr, len(pts), crops.ptr, slot_dev=slots.ptr)
+        fl = DeviceArray.from_host(ctx, flips[off:off + len(pts)])
+        net.crop(dp.ptr, len(pts), crops.ptr, slot_dev=slots.ptr, flip_dev=fl.ptr)
         ctx.synchronize()
         off += len(pts)
     return TrainSet(crops, DeviceArray.from_host(ctx, lab_store), lab_store, n)
 
 
-def train(net: ProposalNet, ts: TrainSet, epochs=4, batch=256, lr=1e-3, lr_final=1e-4, log=print, holdout=2048):
+def train(net: ProposalNet, ts: TrainSet, held: TrainSet, epochs=4, batch=256, lr=1e-3, lr_final=1e-4, log=print):
     """Adam over contiguous batches of the training set in a random order per epoch,
-    cosine-decayed learning rate; the last `holdout` samples are held out and their loss
+    cosine-decayed learning rate; the loss of `held` (pairs never trained on) is
     reported per epoch.  Returns the per-epoch (train, held-out) losses."""
-    n_train = ts.n - holdout
-    nb = n_train // batch
+    nb = ts.n // batch
     steps = nb * epochs
     rng = np.random.default_rng(1)
     hist = []
     k = 0
+    best, best_params = None, None
     for ep in range(epochs):
         t0 = time.time()
         run = []
@@ -275,21 +318,27 @@ def train(net: ProposalNet, ts: TrainSet, epochs=4, batch=256, lr=1e-3, lr_final
             if want:
                 run.append(l)
             k += 1
-        held = evaluate(net, ts, n_train, ts.n, batch)
-        hist.append((float(np.mean(run)) if run else float("nan"), held))
-        log(f"epoch {ep + 1}/{epochs}: train loss {hist[-1][0]:.4f}, held-out loss {held:.4f} "
+        hl = evaluate(net, held, 0, held.n, batch)
+        hist.append((float(np.mean(run)) if run else float("nan"), hl))
+        if best is None or hl < best:
+            best, best_params = hl, net.params()
+        log(f"epoch {ep + 1}/{epochs}: train loss {hist[-1][0]:.4f}, held-out loss {hl:.4f} "
             f"({time.time() - t0:.1f} s, {nb} steps of {batch})")
+    # keep the epoch with the lowest held-out loss (early stopping)
+    net.set_params(best_params)
     return hist
 
 
-def evaluate(net: ProposalNet, ts: TrainSet, i0: int, i1: int, batch=256) -> float:
-    """Mean loss of samples i0..i1 (forward only)."""
+def evaluate(net: ProposalNet, ts: TrainSet, i0: int, i1: int, batch=256, per_output=False):
+    """Mean loss of samples i0..i1 (forward only); per_output: the mean squared error
+    of each of the 12 scaled outputs instead."""
     y = DeviceArray(net.ctx, (batch, 12), np.float32)
-    tot, cnt = 0.0, 0
+    tot, cnt = np.zeros(12), 0
     for o in range(i0, i1, batch):
         b = min(batch, i1 - o)
         net.forward(ts.crops.row(o), b, None, y.ptr)
         e = y.download()[:b].astype(np.float64) - ts.labels_host[o:o + b] / SCALE
-        tot += float((e * e).sum())
+        tot += (e * e).sum(axis=0)
         cnt += b
-    return tot / max(1, cnt)
+    tot /= max(1, cnt)
+    return tot if per_output else float(tot.sum())

@@ -114,6 +114,10 @@ int dvc_volume_upload(dvc_ctx *ctx, int slot, const void *host, int64_t header_b
 int dvc_volume_attach(dvc_ctx *ctx, int slot, const void *device_ptr, int bytes_per,
                       int64_t wide, int64_t high, int64_t tall);
 
+/* Copy the volume in `slot` back to host memory (wide*high*tall*bytes_per bytes, native
+ * endianness); no reference counterpart. */
+int dvc_volume_download(dvc_ctx *ctx, int slot, void *host);
+
 /* ---------------------------------------------------------- interpolation */
 
 /* Interpolate::center_on + Interpolate::kernels + tri_cub_Lek / tri_lin / nearest
@@ -194,6 +198,13 @@ int dvc_search_fetch_passes(dvc_ctx *ctx, dvc_point_result *proposal, dvc_point_
 /* Launch tuning: threads per point (64, 128, 256, 512 or 1024; 0 = automatic), applied
  * by the next dvc_search_prepare. */
 int dvc_set_block_threads(dvc_ctx *ctx, int threads);
+/* Autotuner of a prepared search (no reference counterpart): times reps executes at
+ * each threads-per-point count whose LDS fits (64, 128, 256, 512, 1024; one point per
+ * workgroup) and keeps the fastest for later executes (until the next prepare).  The
+ * given starts must be set first in the given-start and judged modes.  Results change
+ * only in the summation order of the per-point sums.  best_threads / best_ms (device
+ * milliseconds per execute) may be NULL. */
+int dvc_search_autotune(dvc_ctx *ctx, int reps, int32_t *best_threads, float *best_ms);
 /* 1 (default): the level launches of a prepared search are captured into a HIP graph
  * once and replayed by dvc_search_execute; 0: launched one by one. */
 int dvc_set_launch_mode(dvc_ctx *ctx, int graph);
@@ -218,6 +229,7 @@ typedef struct dvc_net dvc_net;
 /* max_batch crops per forward (1..4096); training != 0 also allocates gradient and
  * optimiser buffers (then max_batch must be a multiple of 4). */
 int  dvc_net_create(dvc_ctx *ctx, int max_batch, int training, dvc_net **net);
+/* Destroy every network of a context before the context itself. */
 void dvc_net_destroy(dvc_net *net);
 const char *dvc_net_last_error(const dvc_net *net);
 int64_t dvc_net_num_params(void);
@@ -228,9 +240,11 @@ int  dvc_net_params(dvc_net *net, float *host, int64_t count, int set);
  * point i), i < count, into slot slot_dev[i] (NULL: slot i) of out_dev (BF16
  * [slots][32][32][32][2], channel 0 reference, 1 target, each normalised to zero mean
  * and unit variance; out_dev NULL: the network's own input buffer, count <= max_batch).
- * All pointers are device pointers; the caller sizes out_dev for every slot named. */
+ * flip_dev[i] (NULL: 0) mirrors crop i along x (bit 0), y (bit 1), z (bit 2), for
+ * training-set augmentation.  All pointers are device pointers; the caller sizes
+ * out_dev for every slot named. */
 int  dvc_net_crop(dvc_net *net, const double *points_dev, const int32_t *idx_dev, int32_t count, void *out_dev,
-                  const int32_t *slot_dev);
+                  const int32_t *slot_dev, const int32_t *flip_dev);
 /* Forward pass of B crops (in_dev: device BF16 crops, NULL: the network's input buffer)
  * into par_dev[B*12] (FP64 warp parameters, may be NULL) and y_dev[B*12] (FP32 unscaled
  * head output, may be NULL). */
@@ -243,6 +257,13 @@ int  dvc_net_gradients(dvc_net *net, const void *in_dev, const float *label_dev,
 /* One Adam step (beta 0.9 / 0.999, eps 1e-8) on that gradient. */
 int  dvc_net_train_step(dvc_net *net, const void *in_dev, const float *label_dev, int32_t B, float lr,
                         float *loss_host);
+/* Training pairs for the network: overwrite the context's correlated volume (slot 1)
+ * with the reference volume (slot 0) resampled tile by tile, each tile x tile x tile
+ * block t (numbered x fastest) under its own affine map out(x) = ref(M_t x + o_t),
+ * maps[12*t..] = M row-major then o, nmaps = the number of tiles; separable Catmull-Rom
+ * with clamped indices, rounded and clamped to the voxel range.  Both slots must hold
+ * volumes of one shape and type. */
+int  dvc_synth_warp_tiles(dvc_ctx *ctx, int tile, const double *maps, int64_t nmaps);
 /* Proposals for every point of a prepared DVC_START_GIVEN / DVC_START_JUDGED search,
  * written straight into its start table (replaces dvc_search_set_starts). */
 int  dvc_search_propose(dvc_ctx *ctx, dvc_net *net);

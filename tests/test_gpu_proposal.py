@@ -30,11 +30,12 @@ def _bits(d):
     return u.view(np.float32).astype(np.float64)
 
 
-def _crops(ctx, net, pts, slots=None):
+def _crops(ctx, net, pts, slots=None, flips=None):
     dp = D.DeviceArray.from_host(ctx, np.ascontiguousarray(pts, dtype=np.float64))
     out = D.DeviceArray(ctx, (len(pts), P.CROP_ELEMS), np.uint16)
     sl = D.DeviceArray.from_host(ctx, np.asarray(slots, np.int32)) if slots is not None else None
-    net.crop(dp.ptr, len(pts), out.ptr, slot_dev=sl.ptr if sl else None)
+    fl = D.DeviceArray.from_host(ctx, np.asarray(flips, np.int32)) if flips is not None else None
+    net.crop(dp.ptr, len(pts), out.ptr, slot_dev=sl.ptr if sl else None, flip_dev=fl.ptr if fl else None)
     ctx.synchronize()
     return out
 
@@ -50,6 +51,11 @@ def test_crops_match_oracle(ctx, pair):
     # slot table: crop i lands in slot [2, 0, 1][i]
     h = _bits(_crops(ctx, net, pts, slots=[2, 0, 1])).reshape(3, 32, 32, 32, 2)
     assert np.array_equal(h[[2, 0, 1]], g)
+    # mirrors: bit 0 x (last array axis), bit 1 y, bit 2 z
+    f = _bits(_crops(ctx, net, pts, flips=[1, 6, 7])).reshape(3, 32, 32, 32, 2)
+    assert np.array_equal(f[0], g[0][:, :, ::-1])
+    assert np.array_equal(f[1], g[1][::-1, ::-1, :])
+    assert np.array_equal(f[2], g[2][::-1, ::-1, ::-1])
 
 
 def test_forward_and_gradients_match_oracle(ctx, pair):
@@ -114,3 +120,60 @@ def test_propose_equals_forward(ctx):
         ctx.set_start_mode(D.START_CHAIN)
     assert np.array_equal(a["status"], b["status"])
     assert np.array_equal(a["par"], b["par"]) and np.array_equal(a["obj"], b["obj"])
+
+
+def test_inference_forward_equals_training_forward_and_oracle(ctx, pair):
+    # inference nets gather patches inside the GEMM (conv_fwd_kernel); training nets
+    # store the im2col matrix first: same values in the same k order, so the outputs
+    # are identical; both match the oracle
+    ctx.upload(0, pair.ref)
+    ctx.upload(1, pair.tar)
+    params = P.init_params(12)
+    rng = np.random.default_rng(13)
+    params[:] += rng.normal(0, 0.02, params.size).astype(np.float32) * (params == 0)
+    inf = P.ProposalNet(ctx, max_batch=8, params=params)
+    trn = P.ProposalNet(ctx, max_batch=8, training=True, params=params)
+    crops = _crops(ctx, inf, rng.uniform(16, 48, (6, 3)))
+    ya = D.DeviceArray(ctx, (6, 12), np.float32)
+    yb = D.DeviceArray(ctx, (6, 12), np.float32)
+    inf.forward(crops.ptr, 6, None, ya.ptr)
+    trn.forward(crops.ptr, 6, None, yb.ptr)
+    a, b = ya.download(), yb.download()
+    assert np.array_equal(a, b)
+    yo, _ = PN.forward(PN.unpack(params.astype(np.float64)), _bits(crops).reshape(6, 32, 32, 32, 2), bf=True)
+    assert np.linalg.norm(a - yo) <= 1e-2 * np.linalg.norm(yo)
+
+
+def test_tile_warp_matches_catmull_rom(ctx):
+    rng = np.random.default_rng(14)
+    ref = synth.smoothed_noise((40, 48, 56), 1.5, np.uint8, seed=3)
+    tile = 24
+    nt = [(s + tile - 1) // tile for s in (56, 48, 40)]           # x, y, z
+    maps = []
+    for _ in range(nt[0] * nt[1] * nt[2]):
+        F = np.eye(3) + rng.uniform(-0.03, 0.03, (3, 3))
+        maps.append(np.concatenate([F.ravel(), rng.uniform(-4, 4, 3)]))
+    maps = np.array(maps)
+    ctx.upload(0, ref)
+    ctx.upload(1, ref)
+    ctx.warp_tiles(tile, maps)
+    got = ctx.download(1, ref.shape, np.uint8).astype(np.float64)
+    zz, yy, xx = np.meshgrid(np.arange(40), np.arange(48), np.arange(56), indexing="ij")
+    t = ((zz // tile) * nt[1] + yy // tile) * nt[0] + xx // tile
+    X = np.stack([xx, yy, zz], -1).reshape(-1, 3).astype(np.float64)
+    m = maps[t.reshape(-1)]
+    src = np.einsum("nij,nj->ni", m[:, :9].reshape(-1, 3, 3), X) + m[:, 9:]
+    want = np.clip(np.rint(synth.catmull_rom(ref, src)), 0, 255).reshape(ref.shape)
+    # FP64 sums in another order: a value at a rounding tie may land one level away
+    assert np.abs(got - want).max() <= 1.0
+    assert (got == want).mean() > 0.999
+
+
+def test_tiled_labels_recover_imposed_warps(ctx):
+    tp = P.tiled_pair(5, size=128, tile=64)
+    pts, lab = P.label_pair(ctx, tp, 64)
+    assert len(pts) >= 0.5 * len(tp.points)
+    idx = np.array([np.flatnonzero((tp.points == q).all(axis=1))[0] for q in pts])
+    err = np.abs(lab[:, :3] - tp.disp[idx]).max(axis=1)
+    print(f"  labelled {len(pts)} of {len(tp.points)}; max |label - imposed| {err.max():.3f} voxel")
+    assert np.median(err) < 0.05

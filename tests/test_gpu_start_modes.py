@@ -124,3 +124,31 @@ def test_starts_required_and_chain_unchanged(ctx, oracle, case3):
     run = oracle.make_run(shape=case3.ref.shape, bytes_per=1, **case3.params)
     ores, _, _ = oracle.dvc_run(run, case3.ref, case3.tar, case3.points, order, neigh, nthreads=8)
     _close(chain, ores, "chain")
+
+
+def test_autotune_keeps_parity(ctx, oracle, case3):
+    # the autotuner only changes threads per point (summation order of per-point sums)
+    order, neigh = D.sort_order_neighbors(case3.points)
+    ctx.upload(0, case3.ref)
+    ctx.upload(1, case3.tar)
+    rp = D.make_params(**case3.params)
+    run = oracle.make_run(shape=case3.ref.shape, bytes_per=1, **case3.params)
+    ores, _, _ = oracle.dvc_run(run, case3.ref, case3.tar, case3.points, order, neigh, nthreads=8)
+    ctx.prepare(rp, case3.points, order, neigh)
+    bt, ms = ctx.autotune(1)
+    print(f"  chain: {bt} threads per point, {ms:.3f} ms")
+    assert bt in (64, 128, 256, 512, 1024) and ctx.block_threads() == bt and ms > 0
+    ctx.execute()
+    _close(ctx.fetch(), ores, "chain after autotune")
+    starts = _starts(case3, 4)
+    g = oracle.dvc_run_start(run, case3.ref, case3.tar, case3.points, order, neigh, starts, mode=1, nthreads=8)[0]
+    ctx.set_start_mode(D.START_GIVEN)
+    try:
+        ctx.prepare(rp, case3.points, order, neigh)
+        ctx.set_starts(starts)
+        bt, ms = ctx.autotune(1)
+        print(f"  given starts: {bt} threads per point, {ms:.3f} ms")
+        ctx.execute()
+        _close(ctx.fetch(), g, "given after autotune")
+    finally:
+        ctx.set_start_mode(D.START_CHAIN)

@@ -103,3 +103,24 @@ def test_bf16_rounding():
     assert r[2] == 1.0 + 2 ** -6           # tie, rounds up to even
     assert r[3] == -2.5
     assert abs(r[4] - 3.0e-3) <= 3.0e-3 * 2 ** -8
+
+
+def test_mirrored_labels_mirror_the_warp(oracle):
+    # flip_labels (augmentation of mirrored crop pairs) against the oracle's warp
+    # (FloatingCloud::affine_to): warping the cube cloud with the mirrored parameters
+    # gives the mirror image, about the point, of the cloud warped with the originals
+    from digitalvolumecorrelation_amd.proposal import flip_labels
+    run = oracle.make_run(ndof=12, geom="cube", subvol_size=11, subvol_npts=27)
+    pt = np.array([40.0, 41.0, 42.0])
+    stable, _, _ = oracle.cloud(run, pt)
+    rng = np.random.default_rng(15)
+    for fl in range(8):
+        par = np.concatenate([rng.uniform(-3, 3, 3), rng.uniform(-0.05, 0.05, 3), rng.uniform(-0.03, 0.03, 6)])
+        d = np.array([-1.0 if fl & b else 1.0 for b in (1, 2, 4)])
+        moved, _ = oracle.affine(run, pt, par, 12)
+        moved_f, _ = oracle.affine(run, pt, flip_labels(par[None], np.array([fl]))[0], 12)
+        # stable point j and its mirror image about pt share an index map on the grid
+        mirror = pt + (stable - pt) * d
+        idx = [int(np.argmin(np.abs(stable - q).sum(axis=1))) for q in mirror]
+        assert np.allclose(stable[idx], mirror, atol=1e-9)
+        assert np.allclose(moved_f[idx], pt + (moved - pt) * d, atol=1e-9), fl
