@@ -106,6 +106,7 @@ def proposal_leg(ctx, a, case, params, order, neigh, plain, D):
     du = np.abs(res["par"][both, :3] - plain["par"][both, :3]).max(axis=1) if both.any() else np.zeros(0)
     pg = prop["status"] == 0
     rel = np.abs(res["obj"][both] - plain["obj"][both]) / np.abs(plain["obj"][both])
+    excess = (res["obj"][both] - plain["obj"][both]) / np.abs(plain["obj"][both])   # > 0: worse than plain
     return {"value": n / (wall_ms / 1e3), "unit": "points/s", "ms_per_step": wall_ms,
             "speedup_vs_plain": None, "weights": wpath.name, "threads_per_point": tuned[0],
             "mode": "judged: proposal pass + judge pass from the reference start guess, keep-if-no-worse",
@@ -115,7 +116,9 @@ def proposal_leg(ctx, a, case, params, order, neigh, plain, D):
             "mean_iters_plain": float(plain["iters"][plain["status"] == 0].mean()),
             "max_du_vs_plain_voxel": float(du.max()) if du.size else None,
             "points_within_1e-3_of_plain": int((du <= 1e-3).sum()),
-            "max_obj_rel_vs_plain": float(rel.max()) if rel.size else None}
+            "max_obj_rel_vs_plain": float(rel.max()) if rel.size else None,
+            "max_obj_excess_vs_plain": float(excess.max()) if excess.size else None,
+            "points_obj_worse_than_plain_by_1e-5": int((excess > 1e-5).sum())}
 
 
 def main():
