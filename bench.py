@@ -44,6 +44,11 @@ def parse():
     ap.add_argument("--bytes-sample", type=int, default=400, help="points sampled for the algorithmic byte count")
     ap.add_argument("--cache", default="", help="directory to keep the generated volumes in between runs")
     ap.add_argument("--no-proposal", action="store_true", help="skip the learned-proposal leg")
+    ap.add_argument("--workload", default="affine", choices=["affine", "full"],
+                    help="affine: the 12-DOF 256^3 workload (default); full: the 1536^3 / 10^6-point case "
+                         "(volumes made on the device), reported in its own JSON line")
+    ap.add_argument("--full-size", type=int, default=1536)
+    ap.add_argument("--full-grid", type=int, default=100)
     ap.add_argument("--weights", default="", help="proposal network parameters (.npz); default: the package's")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a plain device-to-device copy of this many bytes (3 times), the known-size "
@@ -121,6 +126,92 @@ def proposal_leg(ctx, a, case, params, order, neigh, plain, D):
             "points_obj_worse_than_plain_by_1e-5": int((excess > 1e-5).sum())}
 
 
+def full_scale(a, D, synth, workload):
+    """The full-scale case on one GPU: both volumes made on the device, every point of
+    the grid searched by the serial chain (plain method) and by the judged proposal mode,
+    the strain field of the plain result, accuracy against the imposed field, and the
+    oracle on a prefix of the processing order (volumes copied back to the host)."""
+    t0 = time.time()
+    ctx = D.Context(0)
+    size, n = a.full_size, a.full_grid
+    case = synth.config5(ctx, size=size, n=n, spacing=14.0 * 100 / n if size == 1536 else (size - 150) / (n - 1))
+    t_gen = time.time() - t0
+    log(f"volumes made on the device in {t_gen:.1f} s: {case.shape} uint16")
+    t1 = time.time()
+    order, neigh = D.knn_order_neighbors(case.points, start=case.start)
+    t_knn = time.time() - t1
+    log(f"order and 75 neighbours of {len(case.points)} points in {t_knn:.1f} s")
+    params = D.make_params(**case.params)
+    t1 = time.time()
+    ctx.prepare(params, case.points, order, neigh)
+    t_prep = time.time() - t1
+    levels = ctx.levels()
+    log(f"prepared in {t_prep:.1f} s: {levels} dependency levels")
+    ctx.execute()
+    ms_first = ctx.kernel_ms()
+    dev = []
+    t1 = time.perf_counter()
+    for _ in range(a.steps):
+        ctx.execute()
+        dev.append(ctx.kernel_ms())
+    wall = (time.perf_counter() - t1) * 1e3 / a.steps
+    res = ctx.fetch()
+    npts = len(case.points)
+    good = res["status"] == 0
+    err = np.abs(res["par"][good, :3] - case.truth[good]).max(axis=1)
+    log(f"plain: {wall:.1f} ms per step ({npts / wall * 1e3:.0f} points/s), good {good.sum()}")
+    ctx.strain(case.points, res["par"][:, :3].copy(), res["status"], res["obj"], neigh, ndp=27)
+    strain_ms = ctx.kernel_ms()
+    # cube cloud offsets about the point (FloatingCloud.cpp:84-128): g^3 grid over [-rad, rad]
+    g = int(np.ceil(case.params["subvol_npts"] ** (1.0 / 3.0)))
+    rad = case.params["subvol_size"] / 2.0
+    k = -rad + np.arange(g) * (2.0 * rad / (g - 1))
+    kz, ky, kx = np.meshgrid(k, k, k, indexing="ij")
+    offs = np.stack([kx.ravel(), ky.ravel(), kz.ravel()], axis=1)
+    per_pt, ns = workload.search_bytes(case.points, offs, res, 12, 2, sample=a.bytes_sample)
+    roof = None
+    if per_pt:
+        achieved = per_pt * int(good.sum()) / (np.mean(dev) / 1e3) / 1e9
+        roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "algorithmic_bytes_per_point": round(per_pt, 1)}
+    nn = None
+    if not a.no_proposal:
+        nn = proposal_leg(ctx, a, case, params, order, neigh, res, D)
+    cpu = None
+    if not a.no_cpu:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import oracle as O
+        ref = ctx.download(0, case.shape, np.uint16)
+        tar = ctx.download(1, case.shape, np.uint16)
+        thr = a.cpu_threads or min(16, os.cpu_count() or 1)
+        M = min(a.cpu_points, npts)
+        run = O.make_run(shape=case.shape, bytes_per=2, jacobian=0, **case.params)
+        ores, sec, m = O.dvc_run(run, ref, tar, case.points, order, neigh, nproc=M, nthreads=thr)
+        idx = order[:m]
+        ok = ores["status"][idx] == 0
+        du = np.abs(res["par"][idx][ok, :3] - ores["par"][idx][ok, :3]).max() if ok.any() else 0.0
+        rel = (np.abs(res["obj"][idx][ok] - ores["obj"][idx][ok]) / ores["obj"][idx][ok]).max() if ok.any() else 0.0
+        cpu = {"value": m / sec, "unit": "points/s", "cores": thr, "kind": "port",
+               "sample": f"first {m} points of the processing order, {sec:.1f} s",
+               "agreement": {"status_equal": bool(np.array_equal(res["status"][idx], ores["status"][idx])),
+                             "iters_equal": int((res["iters"][idx] == ores["iters"][idx]).sum()),
+                             "max_du_voxel": float(du), "max_obj_rel": float(rel)}}
+        del ref, tar
+    out = {"workload": f"full scale: {size}^3 u16 pair made on the device (8 Gaussian bumps of 4 voxels, width 200, "
+                       f"+1% volumetric strain), {n}^3 grid ({npts} points), cube 41 / 8000 samples, ZNSSD, "
+                       f"tricubic, 12-DOF, disp_max 10",
+           "points": npts, "good": int(good.sum()), "levels": levels,
+           "plain": {"value": npts / (wall / 1e3), "unit": "points/s", "ms_per_step": wall,
+                     "device_ms_per_step": float(np.mean(dev)), "first_execute_ms": ms_first},
+           "nn_proposal": nn, "strain_fit_device_ms": strain_ms,
+           "max_err_vs_imposed_voxel": float(err.max()) if err.size else None,
+           "median_err_vs_imposed_voxel": float(np.median(err)) if err.size else None,
+           "setup_s": {"volumes": round(t_gen, 1), "knn": round(t_knn, 1), "prepare": round(t_prep, 1)},
+           "roofline": roof, "cpu_baseline": cpu}
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -140,6 +231,9 @@ def main():
 
     import digitalvolumecorrelation_amd as D
     from digitalvolumecorrelation_amd import synth, workload
+
+    if a.workload == "full":
+        return full_scale(a, D, synth, workload)
 
     t0 = time.time()
     case = load_case(a.cache, synth)

@@ -37,7 +37,7 @@ def _run(cmd: list[str]) -> None:
 
 
 def build_library() -> Path:
-    srcs = [CSRC / "dvc_kernels.hip", CSRC / "dvc_api.cpp", CSRC / "dvc_net.hip"]
+    srcs = [CSRC / "dvc_kernels.hip", CSRC / "dvc_api.cpp", CSRC / "dvc_net.hip", CSRC / "dvc_synth.hip"]
     objs = []
     for s in srcs:
         o = CSRC / (s.stem + ".o")

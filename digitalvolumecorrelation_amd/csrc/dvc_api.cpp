@@ -164,6 +164,29 @@ int ctx_volumes(dvc_ctx *ctx, const uint8_t **v0, const uint8_t **v1, int *bpv, 
     return 0;
 }
 
+int ctx_volume_info(dvc_ctx *ctx, int slot, const uint8_t **p, int *bpv, int *W, int *H, int *T) {
+    if (slot != 0 && slot != 1) return -1;
+    const Volume &v = ctx->vol[slot];
+    if (!v.d || v.W > INT32_MAX || v.H > INT32_MAX || v.T > INT32_MAX) return -1;
+    *p = v.d; *bpv = v.bpv; *W = (int)v.W; *H = (int)v.H; *T = (int)v.T;
+    return 0;
+}
+
+uint8_t *ctx_volume_alloc(dvc_ctx *ctx, int slot, int bpv, int64_t w, int64_t h, int64_t t) {
+    if (slot != 0 && slot != 1) return nullptr;
+    if (hipSetDevice(ctx->device) != hipSuccess) return nullptr;
+    drop_graph(ctx);
+    Volume &v = ctx->vol[slot];
+    if (v.owned) hipFree(v.d);
+    v = Volume();
+    const size_t bytes = (size_t)w * h * t * bpv;
+    if (hipMalloc((void **)&v.d, bytes + 64) != hipSuccess) { v.d = nullptr; return nullptr; }
+    v.owned = true;
+    v.bpv = bpv; v.W = w; v.H = h; v.T = t;
+    if (hipMemsetAsync(v.d + bytes, 0, 64, ctx->stream) != hipSuccess) return nullptr;
+    return v.d;
+}
+
 int ctx_prepared(dvc_ctx *ctx, const double **pts, const int **order, int *npts, int *nproc, double **start) {
     if (!ctx->plan_ready || ctx->plan_mode == DVC_START_CHAIN) return -1;
     *pts = ctx->d_pts; *order = ctx->d_order; *npts = ctx->npts; *nproc = ctx->nproc; *start = ctx->d_start;

@@ -76,5 +76,9 @@ int ctx_volumes(dvc_ctx *ctx, const uint8_t **v0, const uint8_t **v1, int *bpv, 
 // a prepared given-start or judged search: its device points, order, sizes and start table
 int ctx_prepared(dvc_ctx *ctx, const double **pts, const int **order, int *npts, int *nproc, double **start);
 void ctx_mark_starts(dvc_ctx *ctx);
+// one volume slot: 0, else -1 (empty)
+int ctx_volume_info(dvc_ctx *ctx, int slot, const uint8_t **p, int *bpv, int *W, int *H, int *T);
+// (re)allocate a volume slot on the device (64 zero bytes of padding); null on failure
+uint8_t *ctx_volume_alloc(dvc_ctx *ctx, int slot, int bpv, int64_t w, int64_t h, int64_t t);
 
 }  // namespace dvc

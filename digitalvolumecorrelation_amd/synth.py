@@ -174,3 +174,54 @@ def config2(size: int = 256, n: int = 15, spacing: float = 12.0, seed: int = 2,
                   subvol_npts=npts, disp_max=10, basin_radius=0.0,
                   rigid_trans=tuple(float(np.rint(v)) for v in u0))
     return Case("affine-u16", ref, tar, pts, truth, params, start)
+
+
+@dataclass
+class DeviceCase:
+    """A pair generated on the device (dvc_synth_noise / dvc_synth_warp_field): the
+    volumes live only in the context's slots."""
+    name: str
+    shape: tuple
+    dtype: type
+    points: np.ndarray
+    truth: np.ndarray
+    params: dict
+    start: np.ndarray
+    bumps: np.ndarray
+    strain: float
+
+
+def bump_field(X, bumps, strain, center):
+    """u(X) of dvc_synth_warp_field: Gaussian bumps plus a uniform strain about center."""
+    X = np.asarray(X, dtype=np.float64)
+    u = strain * (X - np.asarray(center))
+    for b in np.asarray(bumps).reshape(-1, 7):
+        g = np.exp(-((X - b[:3]) ** 2).sum(axis=-1) / (2.0 * b[3] ** 2))
+        u = u + g[..., None] * b[4:7]
+    return u
+
+
+def config5(ctx, size: int = 1536, n: int = 100, spacing: float = 14.0, seed: int = 5, nbumps: int = 8,
+            amplitude: float = 4.0, width: float = 200.0, volumetric: float = 0.01, npts: int = 8000,
+            diameter: int = 41) -> DeviceCase:
+    """Full-scale 12-DOF case made on the device: uint16 size^3 smoothed noise (sigma 2),
+    target under 8 Gaussian bumps (amplitude 4 voxels, width 200) plus 1 % volumetric
+    strain about the centre; n^3 grid at `spacing`, cube 41 / 8000 samples, ZNSSD,
+    disp_max 10, processing order from the volume centre."""
+    rng = np.random.default_rng(seed)
+    shape = (size, size, size)
+    ctx.synth_noise(0, shape, np.uint16, 2.0, 1000 + seed)
+    dirs = rng.normal(size=(nbumps, 3))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    bumps = np.concatenate([rng.uniform(0, size - 1, (nbumps, 3)), np.full((nbumps, 1), width),
+                            amplitude * dirs], axis=1)
+    strain = (1.0 + volumetric) ** (1.0 / 3.0) - 1.0
+    center = np.full(3, (size - 1) / 2.0)
+    ctx.synth_warp_field(bumps, strain, center)
+    pts = grid_points(n, spacing, (size / 2.0,) * 3)
+    truth = bump_field(pts, bumps, strain, center)
+    start = np.full(3, size / 2.0)
+    u0 = bump_field(start[None], bumps, strain, center)[0]
+    params = dict(ndof=12, obj="znssd", interp="tricubic", geom="cube", subvol_size=diameter, subvol_npts=npts,
+                  disp_max=10, basin_radius=0.0, rigid_trans=tuple(float(np.rint(v)) for v in u0))
+    return DeviceCase("full-u16", shape, np.uint16, pts, truth, params, start, bumps, strain)

@@ -118,6 +118,21 @@ int dvc_volume_attach(dvc_ctx *ctx, int slot, const void *device_ptr, int bytes_
  * endianness); no reference counterpart. */
 int dvc_volume_download(dvc_ctx *ctx, int slot, void *host);
 
+/* Synthetic volumes made on the device, for workloads too large to generate on the host
+ * (no reference counterpart; the reference's test data are files).
+ * dvc_synth_noise: fill `slot` with a wide x high x tall volume of bytes_per-byte voxels:
+ * uniform noise from `seed` (splitmix64 of the voxel index), separable Gaussian
+ * smoothing with `sigma` (truncated at 4 sigma, mirror-reflected at the faces), rescaled
+ * so that its minimum and maximum map to 0 and the type's maximum, rounded.
+ * dvc_synth_warp_field: fill the correlated slot (1) with the reference (slot 0) under
+ * the displacement field u(X) = sum_b a_b exp(-|X - c_b|^2 / (2 w_b^2)) + strain (X - center):
+ * target(x) = ref(X) where X + u(X) = x (`iterations` fixed-point steps), separable
+ * Catmull-Rom, rounded and clamped.  bumps[7*b..] = c_b (xyz), w_b, a_b (xyz). */
+int dvc_synth_noise(dvc_ctx *ctx, int slot, int bytes_per, int64_t wide, int64_t high, int64_t tall, double sigma,
+                    uint64_t seed);
+int dvc_synth_warp_field(dvc_ctx *ctx, const double *bumps, int32_t nbumps, double strain, const double center[3],
+                         int32_t iterations);
+
 /* ---------------------------------------------------------- interpolation */
 
 /* Interpolate::center_on + Interpolate::kernels + tri_cub_Lek / tri_lin / nearest
