@@ -46,44 +46,27 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
 // truncated position, from both volumes, each normalised to zero mean and unit
 // variance, stored channels-last [32][32][32][2] in BF16.  Voxels outside the volume
 // repeat the nearest face voxel.
-__global__ void __launch_bounds__(256) crop_kernel(const uint8_t *v0, const uint8_t *v1, int bpv, int W, int H, int T,
-                                                   const double *pts, const int *idx, int n, uint16_t *out,
-                                                   const int *slot, const int *flip) {
-    __shared__ double red[2][4][8];
-    __shared__ float stat[4];
-    const int p = blockIdx.x;
-    if (p >= n) return;
-    const int pt = idx ? idx[p] : p;
-    const int ox = (int)floor(pts[3 * (size_t)pt + 0]) - 15;
-    const int oy = (int)floor(pts[3 * (size_t)pt + 1]) - 15;
-    const int oz = (int)floor(pts[3 * (size_t)pt + 2]) - 15;
+constexpr int kCropThreads = 1024;      // threads of the crop kernels (16 waves)
+constexpr int kCropBatch = 8;           // voxels per thread whose loads are issued together
+
+// Sums of both channels over the box (every thread its voxels i = tid + k*1024 in order,
+// then the wave sums, then the 16 wave results in order) -> mean and 1/std per channel.
+// Shared by crop_kernel and crop_conv1_kernel so that both normalise identically.
+__device__ __forceinline__ void crop_stats(double s0, double q0, double s1, double q1, double (*red)[16], float *stat) {
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    const int fl = flip ? flip[p] : 0;          // bit 0/1/2: mirror the crop along x/y/z
-    auto val = [&](const uint8_t *v, int i) -> float {
-        const int cx = i & 31, cy = (i >> 5) & 31, cz = i >> 10;
-        int x = ox + ((fl & 1) ? 31 - cx : cx), y = oy + ((fl & 2) ? 31 - cy : cy), z = oz + ((fl & 4) ? 31 - cz : cz);
-        x = x < 0 ? 0 : (x >= W ? W - 1 : x);
-        y = y < 0 ? 0 : (y >= H ? H - 1 : y);
-        z = z < 0 ? 0 : (z >= T ? T - 1 : z);
-        const long long o = ((long long)z * H + y) * W + x;
-        return bpv == 1 ? (float)v[o] : (float)((const uint16_t *)v)[o];
-    };
-    double s[2] = {0.0, 0.0}, q[2] = {0.0, 0.0};
-    for (int i = tid; i < 32768; i += 256) {
-        float a = val(v0, i), b = val(v1, i);
-        s[0] += a; q[0] += (double)a * a;
-        s[1] += b; q[1] += (double)b * b;
-    }
-    double r4[4] = {s[0], q[0], s[1], q[1]};
+    double r4[4] = {s0, q0, s1, q1};
     for (int j = 0; j < 4; j++) {
         double x = r4[j];
         for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-        if (lane == 0) red[0][j][wid] = x;
+        if (lane == 0) red[j][wid] = x;
     }
     __syncthreads();
     if (tid == 0) {
         double t[4];
-        for (int j = 0; j < 4; j++) t[j] = red[0][j][0] + red[0][j][1] + red[0][j][2] + red[0][j][3];
+        for (int j = 0; j < 4; j++) {
+            t[j] = 0.0;
+            for (int w = 0; w < kCropThreads / 64; w++) t[j] += red[j][w];
+        }
         for (int c = 0; c < 2; c++) {
             double mean = t[2 * c] / 32768.0;
             double var = t[2 * c + 1] / 32768.0 - mean * mean;
@@ -92,11 +75,152 @@ __global__ void __launch_bounds__(256) crop_kernel(const uint8_t *v0, const uint
         }
     }
     __syncthreads();
+}
+
+// Raw voxel of crop position i (x fastest) of the box starting at (ox, oy, oz), mirrored
+// along the axes of fl, coordinates clamped to the volume.
+__device__ __forceinline__ uint32_t crop_voxel(const uint8_t *v, int bpv, int W, int H, int T, int ox, int oy, int oz,
+                                               int fl, int i) {
+    const int cx = i & 31, cy = (i >> 5) & 31, cz = i >> 10;
+    int x = ox + ((fl & 1) ? 31 - cx : cx), y = oy + ((fl & 2) ? 31 - cy : cy), z = oz + ((fl & 4) ? 31 - cz : cz);
+    x = x < 0 ? 0 : (x >= W ? W - 1 : x);
+    y = y < 0 ? 0 : (y >= H ? H - 1 : y);
+    z = z < 0 ? 0 : (z >= T ? T - 1 : z);
+    const long long o = ((long long)z * H + y) * W + x;
+    return bpv == 1 ? (uint32_t)v[o] : (uint32_t)((const uint16_t *)v)[o];
+}
+
+// One workgroup per point: the 32^3 box whose voxel 15 (per axis) holds the point's
+// truncated position, from both volumes, each normalised to zero mean and unit
+// variance, stored channels-last [32][32][32][2] in BF16.  Voxels outside the volume
+// repeat the nearest face voxel.  The raw box is kept in registers between the two
+// passes (32 voxels per thread and channel).
+__global__ void __launch_bounds__(kCropThreads) crop_kernel(const uint8_t *v0, const uint8_t *v1, int bpv, int W,
+                                                            int H, int T, const double *pts, const int *idx, int n,
+                                                            uint16_t *out, const int *slot, const int *flip) {
+    __shared__ double red[4][16];
+    __shared__ float stat[4];
+    const int p = blockIdx.x;
+    if (p >= n) return;
+    const int pt = idx ? idx[p] : p;
+    const int ox = (int)floor(pts[3 * (size_t)pt + 0]) - 15;
+    const int oy = (int)floor(pts[3 * (size_t)pt + 1]) - 15;
+    const int oz = (int)floor(pts[3 * (size_t)pt + 2]) - 15;
+    const int tid = threadIdx.x;
+    const int fl = flip ? flip[p] : 0;          // bit 0/1/2: mirror the crop along x/y/z
+    constexpr int PER = 32768 / kCropThreads;
+    double s0 = 0.0, q0 = 0.0, s1 = 0.0, q1 = 0.0;
+    for (int b0 = 0; b0 < PER; b0 += kCropBatch) {
+        uint32_t ra[kCropBatch], rb[kCropBatch];
+#pragma unroll
+        for (int k = 0; k < kCropBatch; k++) {
+            ra[k] = crop_voxel(v0, bpv, W, H, T, ox, oy, oz, fl, tid + (b0 + k) * kCropThreads);
+            rb[k] = crop_voxel(v1, bpv, W, H, T, ox, oy, oz, fl, tid + (b0 + k) * kCropThreads);
+        }
+#pragma unroll
+        for (int k = 0; k < kCropBatch; k++) {
+            const float a = (float)ra[k], b = (float)rb[k];
+            s0 += a; q0 += (double)a * a;
+            s1 += b; q1 += (double)b * b;
+        }
+    }
+    crop_stats(s0, q0, s1, q1, red, stat);
     const float m0 = stat[0], i0 = stat[1], m1 = stat[2], i1 = stat[3];
     uint32_t *o32 = (uint32_t *)(out + (size_t)(slot ? slot[p] : p) * 65536);
-    for (int i = tid; i < 32768; i += 256) {
-        uint32_t a = f2bf((val(v0, i) - m0) * i0), b = f2bf((val(v1, i) - m1) * i1);
-        o32[i] = a | (b << 16);
+    for (int b0 = 0; b0 < PER; b0 += kCropBatch) {
+        uint32_t ra[kCropBatch], rb[kCropBatch];
+#pragma unroll
+        for (int k = 0; k < kCropBatch; k++) {
+            ra[k] = crop_voxel(v0, bpv, W, H, T, ox, oy, oz, fl, tid + (b0 + k) * kCropThreads);
+            rb[k] = crop_voxel(v1, bpv, W, H, T, ox, oy, oz, fl, tid + (b0 + k) * kCropThreads);
+        }
+#pragma unroll
+        for (int k = 0; k < kCropBatch; k++) {
+            const uint32_t a = f2bf(((float)ra[k] - m0) * i0), b = f2bf(((float)rb[k] - m1) * i1);
+            o32[tid + (b0 + k) * kCropThreads] = a | (b << 16);
+        }
+    }
+}
+
+// Crop, normalisation and the first convolution in one workgroup per point (inference):
+// the point's two 32^3 boxes are read once into LDS (raw voxels, both channels per
+// dword), normalised there with the statistics of crop_kernel (same thread mapping and
+// summation order, so the BF16 values are identical), and the 2 -> 16 convolution of
+// conv_fwd_kernel<1, 2> runs on them from LDS (same lane mapping and k order, so the
+// output is identical too).  out: [n][16^3][16] BF16 after bias and ReLU.
+__global__ void __launch_bounds__(kCropThreads) crop_conv1_kernel(const uint8_t *v0, const uint8_t *v1, int bpv,
+                                                                  int W, int H, int T, const double *pts,
+                                                                  const int *idx, int n, const uint16_t *Wb,
+                                                                  const float *bias, uint16_t *out) {
+    extern __shared__ __align__(16) uint32_t box[];       // 32768 dwords: channel 0 low half, channel 1 high
+    __shared__ double red[4][16];
+    __shared__ float stat[4];
+    const int p = blockIdx.x;
+    if (p >= n) return;
+    const int pt = idx ? idx[p] : p;
+    const int ox = (int)floor(pts[3 * (size_t)pt + 0]) - 15;
+    const int oy = (int)floor(pts[3 * (size_t)pt + 1]) - 15;
+    const int oz = (int)floor(pts[3 * (size_t)pt + 2]) - 15;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    constexpr int PER = 32768 / kCropThreads;
+    double s0 = 0.0, q0 = 0.0, s1 = 0.0, q1 = 0.0;
+    for (int b0 = 0; b0 < PER; b0 += kCropBatch) {
+        uint32_t ra[kCropBatch], rb[kCropBatch];
+#pragma unroll
+        for (int k = 0; k < kCropBatch; k++) {
+            ra[k] = crop_voxel(v0, bpv, W, H, T, ox, oy, oz, 0, tid + (b0 + k) * kCropThreads);
+            rb[k] = crop_voxel(v1, bpv, W, H, T, ox, oy, oz, 0, tid + (b0 + k) * kCropThreads);
+        }
+#pragma unroll
+        for (int k = 0; k < kCropBatch; k++) {
+            const float a = (float)ra[k], b = (float)rb[k];
+            s0 += a; q0 += (double)a * a;
+            s1 += b; q1 += (double)b * b;
+            box[tid + (b0 + k) * kCropThreads] = ra[k] | (rb[k] << 16);      // raw, both channels
+        }
+    }
+    crop_stats(s0, q0, s1, q1, red, stat);        // ends with a barrier: box complete
+    {
+        const float m0 = stat[0], i0 = stat[1], m1 = stat[2], i1 = stat[3];
+        for (int k = 0; k < PER; k++) {
+            const uint32_t r = box[tid + k * kCropThreads];        // this thread's own entries
+            const uint32_t a = f2bf(((float)(r & 0xffffu) - m0) * i0), b = f2bf(((float)(r >> 16) - m1) * i1);
+            box[tid + k * kCropThreads] = a | (b << 16);
+        }
+    }
+    __syncthreads();
+    // conv 2 -> 16, stride 2, pad 1: 4096 output rows, 16 columns, K = 54 padded to 64
+    const int kq = (lane >> 4) * 8;
+    uint4 bw[2];
+#pragma unroll
+    for (int ks = 0; ks < 2; ks++) bw[ks] = *(const uint4 *)(Wb + (lane & 15) * 64 + ks * 32 + kq);
+    const float bn = bias[lane & 15];
+    uint16_t *o = out + (size_t)p * 4096 * 16;
+    for (int tile = wid; tile < 256; tile += kCropThreads / 64) {
+        const int m = tile * 16 + (lane & 15);
+        const int qz = m >> 8, qy = (m >> 4) & 15, qx = m & 15;
+        floatx4 acc = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int ks = 0; ks < 2; ks++) {
+            uint32_t w4[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int tap = (ks * 32 + kq) / 2 + j;
+                const int kz = tap / 9, ky = (tap / 3) % 3, kx = tap % 3;
+                const int z = 2 * qz + kz - 1, y = 2 * qy + ky - 1, x = 2 * qx + kx - 1;
+                const bool ok = tap < 27 && z >= 0 && z < 32 && y >= 0 && y < 32 && x >= 0 && x < 32;
+                w4[j] = ok ? box[(z * 32 + y) * 32 + x] : 0u;
+            }
+            const uint4 av = uint4{w4[0], w4[1], w4[2], w4[3]};
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, av),
+                                                          __builtin_bit_cast(bf16x8, bw[ks]), acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int row = tile * 16 + 4 * (lane >> 4) + r;
+            const float v = acc[r] + bn;
+            o[row * 16 + (lane & 15)] = f2bf(v > 0.0f ? v : 0.0f);
+        }
     }
 }
 
@@ -189,6 +313,7 @@ __global__ void __launch_bounds__(256) gemm_nt_kernel(GemmArgs g) {
         bp[t] = g.Bm + (long long)(bok[t] ? ncol : 0) * g.ldb + kq;
     }
     const bool aok = arow < g.M;
+#pragma unroll 4
     for (int k = kbeg; k < kend; k += 32) {
         const uint4 av = aok ? *(const uint4 *)(ap + k) : zero;
         const bf16x8 a8 = __builtin_bit_cast(bf16x8, av);
@@ -260,6 +385,7 @@ __global__ void __launch_bounds__(256) conv_fwd_kernel(const uint16_t *in, int B
         ok = aok && tap < 27 && z >= 0 && z < D && y >= 0 && y < D && x >= 0 && x < D;
         return base + (((long long)z * D + y) * D + x) * C;
     };
+#pragma unroll 4
     for (int k = 0; k < Kp; k += 32) {
         uint4 av;
         if (C % 8 == 0) {
@@ -698,6 +824,22 @@ int forward(dvc_net *n, hipStream_t st, int B, double *par, const int *idx) {
     return launch_ok(n, "head");
 }
 
+// inference from points: fused crop + first convolution, then layers 2-4 and the head;
+// par[idx[b]*12..] receives the proposals
+int propose_fused(dvc_net *n, hipStream_t st, const double *pts, const int *idx, int B, double *par) {
+    const uint8_t *v0, *v1;
+    int bpv, W, H, T;
+    if (ctx_volumes(n->ctx, &v0, &v1, &bpv, &W, &H, &T) != 0) return net_fail(n, DVC_ERR_STATE, "no volume pair");
+    hipLaunchKernelGGL(crop_conv1_kernel, dim3(B), dim3(kCropThreads), 32768 * 4, st, v0, v1, bpv, W, H, T, pts, idx,
+                       B, n->wb[0], n->p + n->L.b[0], n->act[1]);
+    if (int e = launch_ok(n, "crop + first convolution")) return e;
+    for (int l = 1; l < kLayers; l++)
+        if (int e = conv_fwd(n, st, l, B)) return e;
+    hipLaunchKernelGGL(head_forward_kernel, dim3(B), dim3(128), 0, st, n->act[kLayers], n->p + n->L.fcw,
+                       n->p + n->L.fcb, B, n->feat, n->y, par, idx);
+    return launch_ok(n, "head");
+}
+
 int pack(dvc_net *n, hipStream_t st) {
     for (int l = 0; l < kLayers; l++) {
         const int items = kCout[l] * kKp(l);
@@ -779,6 +921,11 @@ int dvc_net_create(dvc_ctx *ctx, int max_batch, int training, dvc_net **out) {
     n->cap = max_batch;
     n->training = training ? 1 : 0;
     if (hipSetDevice(ctx_device(ctx)) != hipSuccess) { delete n; return DVC_ERR_HIP; }
+    if (hipFuncSetAttribute((const void *)crop_conv1_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 32768 * 4) !=
+        hipSuccess) {
+        delete n;
+        return DVC_ERR_HIP;
+    }
     const int B = max_batch;
     int e = 0;
     e = e ? e : net_alloc(n, &n->p, (size_t)n->L.total);
@@ -876,8 +1023,8 @@ int dvc_net_crop(dvc_net *n, const double *points_dev, const int32_t *idx_dev, i
     hipStream_t st = (hipStream_t)ctx_stream(n->ctx);
     uint16_t *out = out_dev ? (uint16_t *)out_dev : n->in;
     if (!out_dev && (count > n->cap || slot_dev)) return net_fail(n, DVC_ERR_ARG, "more crops than the batch capacity");
-    hipLaunchKernelGGL(crop_kernel, dim3(count), dim3(256), 0, st, v0, v1, bpv, W, H, T, points_dev, idx_dev, count, out,
-                       slot_dev, flip_dev);
+    hipLaunchKernelGGL(crop_kernel, dim3(count), dim3(kCropThreads), 0, st, v0, v1, bpv, W, H, T, points_dev, idx_dev,
+                       count, out, slot_dev, flip_dev);
     return launch_ok(n, "crop");
 }
 
@@ -940,13 +1087,15 @@ int dvc_net_train_step(dvc_net *n, const void *in_dev, const float *label_dev, i
 
 int dvc_synth_warp_tiles(dvc_ctx *ctx, int tile, const double *maps, int64_t nmaps) {
     if (!ctx || !maps || tile < 1) return DVC_ERR_ARG;
-    const uint8_t *v0, *v1;
+    const uint8_t *v0;
     int bpv, W, H, T;
-    if (ctx_volumes(ctx, &v0, &v1, &bpv, &W, &H, &T) != 0) return DVC_ERR_STATE;
+    if (ctx_volume_info(ctx, 0, &v0, &bpv, &W, &H, &T) != 0) return DVC_ERR_STATE;
     const int ntx = (W + tile - 1) / tile, nty = (H + tile - 1) / tile, ntz = (T + tile - 1) / tile;
     if (nmaps != (int64_t)ntx * nty * ntz) return DVC_ERR_ARG;
     hipStream_t st = (hipStream_t)ctx_stream(ctx);
     if (hipSetDevice(ctx_device(ctx)) != hipSuccess) return DVC_ERR_HIP;
+    uint8_t *v1 = ctx_volume_alloc(ctx, 1, bpv, W, H, T);     // the target, same shape and type
+    if (!v1) return DVC_ERR_HIP;
     double *d = nullptr;
     if (hipMalloc(&d, sizeof(double) * 12 * nmaps) != hipSuccess) return DVC_ERR_HIP;
     int rc = DVC_OK;
@@ -954,11 +1103,11 @@ int dvc_synth_warp_tiles(dvc_ctx *ctx, int tile, const double *maps, int64_t nma
     const long long n = (long long)W * H * T;
     if (rc == DVC_OK) {
         if (bpv == 1)
-            hipLaunchKernelGGL(warp_tiles_kernel<1>, dim3(grid_for(n, 256, 1 << 20)), dim3(256), 0, st, v0,
-                               (uint8_t *)v1, W, H, T, tile, ntx, nty, d);
+            hipLaunchKernelGGL(warp_tiles_kernel<1>, dim3(grid_for(n, 256, 1 << 20)), dim3(256), 0, st, v0, v1, W,
+                               H, T, tile, ntx, nty, d);
         else
-            hipLaunchKernelGGL(warp_tiles_kernel<2>, dim3(grid_for(n, 256, 1 << 20)), dim3(256), 0, st, v0,
-                               (uint8_t *)v1, W, H, T, tile, ntx, nty, d);
+            hipLaunchKernelGGL(warp_tiles_kernel<2>, dim3(grid_for(n, 256, 1 << 20)), dim3(256), 0, st, v0, v1, W,
+                               H, T, tile, ntx, nty, d);
         if (hipGetLastError() != hipSuccess) rc = DVC_ERR_HIP;
     }
     if (hipStreamSynchronize(st) != hipSuccess) rc = DVC_ERR_HIP;
@@ -977,8 +1126,12 @@ int dvc_search_propose(dvc_ctx *ctx, dvc_net *n) {
     hipStream_t st = (hipStream_t)ctx_stream(ctx);
     for (int i0 = 0; i0 < nproc; i0 += n->cap) {
         const int B = std::min(n->cap, nproc - i0);
-        if (int e = dvc_net_crop(n, pts, order + i0, B, nullptr, nullptr, nullptr)) return e;
-        if (int e = forward(n, st, B, start, order + i0)) return e;
+        if (n->training) {
+            if (int e = dvc_net_crop(n, pts, order + i0, B, nullptr, nullptr, nullptr)) return e;
+            if (int e = forward(n, st, B, start, order + i0)) return e;
+        } else if (int e = propose_fused(n, st, pts, order + i0, B, start)) {
+            return e;
+        }
     }
     ctx_mark_starts(ctx);
     return DVC_OK;

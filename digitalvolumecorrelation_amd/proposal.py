@@ -209,15 +209,22 @@ LABEL_PARAMS = dict(ndof=12, obj="znssd", interp="tricubic", geom="cube", subvol
                     disp_max=12, basin_radius=0.0)
 
 
-def label_pair(ctx: Context, tp: TiledPair, tile=64, params=None):
+def device_reference(ctx: Context, seed: int, size=384):
+    """A smoothed-noise uint8 reference made in slot 0 on the device (dvc_synth_noise,
+    sigma 1.5-2.5 drawn from the seed)."""
+    rng = np.random.default_rng(40_000 + seed)
+    ctx.synth_noise(0, (size,) * 3, np.uint8, float(rng.uniform(1.5, 2.5)), 50_000 + seed)
+
+
+def label_pair(ctx: Context, tp: TiledPair, tile=64, params=None, on_device=False):
     """Upload the pair (target made on the device) and label its points with the plain
     search's converged 12-DOF warps, each point started from its tile's imposed
     translation (DVC_START_GIVEN).  Keeps points that converged (good status, fewer than
     20 Gauss-Newton evaluations) within half a voxel of the imposed displacement.
     Leaves the pair in the context; returns (points, labels [n, 12])."""
     p = dict(LABEL_PARAMS if params is None else params)
-    ctx.upload(0, tp.ref)
-    ctx.upload(1, tp.ref)
+    if not on_device:
+        ctx.upload(0, tp.ref)
     ctx.warp_tiles(tile, tp.maps)
     order, neigh = _lib.knn_order_neighbors(tp.points, k=8)
     starts = np.zeros((len(tp.points), 12))
@@ -257,7 +264,7 @@ class TrainSet:
 
 
 def build_train_set(ctx: Context, net: ProposalNet, seeds, size=384, tile=64, log=print, shuffle_seed=0,
-                    maps_per_ref=1, per_axis=5, spacing=3.0, flip=True):
+                    maps_per_ref=1, per_axis=5, spacing=3.0, flip=True, device_textures=False):
     """Crops and plain-search labels of tiled pairs, resident in HBM, in a random order
     over all pairs (every contiguous batch mixes them).  Each reference volume serves
     `maps_per_ref` pairs with different tile maps, so a texture seen in training comes
@@ -266,11 +273,17 @@ def build_train_set(ctx: Context, net: ProposalNet, seeds, size=384, tile=64, lo
     ref = None
     for i, s in enumerate(seeds):
         t0 = time.time()
-        if i % maps_per_ref == 0:
+        if device_textures:
+            device_reference(ctx, int(s), size)
+            ref = np.zeros((size, size, size), np.uint8)     # shape only; the texture is in slot 0
+        elif i % maps_per_ref == 0:
             ref = reference_volume(int(s), size)
         tp = tiled_pair(int(s), size=size, tile=tile, per_axis=per_axis, spacing=spacing, ref=ref)
-        pts, lab = label_pair(ctx, tp, tile)
-        pairs.append((tp, pts))
+        pts, lab = label_pair(ctx, tp, tile, on_device=device_textures)
+        if device_textures:
+            # crop now, while the pair is in the context (slots hold one pair at a time)
+            tp.ref = None
+        pairs.append((tp, pts, (int(s) if device_textures else None)))
         labels.append(lab)
         log(f"training pair {s}: {len(pts)} labelled points of {len(tp.points)} in {len(tp.maps)} tiles "
             f"({time.time() - t0:.1f} s)")
@@ -283,9 +296,11 @@ def build_train_set(ctx: Context, net: ProposalNet, seeds, size=384, tile=64, lo
     lab_store = np.empty_like(lab_all)
     lab_store[perm] = lab_all                 # sample i is stored at slot perm[i]
     off = 0
-    for tp, pts in pairs:
-        ctx.upload(0, tp.ref)
-        ctx.upload(1, tp.ref)
+    for tp, pts, dev_seed in pairs:
+        if dev_seed is not None:
+            device_reference(ctx, dev_seed, size)
+        else:
+            ctx.upload(0, tp.ref)
         ctx.warp_tiles(tile, tp.maps)
         dp = DeviceArray.from_host(ctx, pts)
         slots = DeviceArray.from_host(ctx, perm[off:off + len(pts)])

@@ -35,6 +35,8 @@ def main(argv=None):
     ap.add_argument("--per-axis", type=int, default=3, help="points per tile and axis")
     ap.add_argument("--spacing", type=float, default=5.0)
     ap.add_argument("--held-pairs", type=int, default=2)
+    ap.add_argument("--device-textures", action="store_true",
+                    help="make every reference texture on the device (dvc_synth_noise), one per pair")
     ap.add_argument("--size", type=int, default=384)
     ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--batch", type=int, default=256)
@@ -51,7 +53,8 @@ def main(argv=None):
     ctx = Context(0)
     net = P.ProposalNet(ctx, max_batch=a.batch, training=True, params=P.init_params(a.seed))
     s0 = a.volume_seed0
-    kw = dict(size=a.size, log=log, maps_per_ref=a.maps_per_ref, per_axis=a.per_axis, spacing=a.spacing)
+    kw = dict(size=a.size, log=log, maps_per_ref=a.maps_per_ref, per_axis=a.per_axis, spacing=a.spacing,
+              device_textures=a.device_textures)
     ts = P.build_train_set(ctx, net, range(s0, s0 + a.pairs), **kw)
     held = P.build_train_set(ctx, net, range(s0 + 1000, s0 + 1000 + a.held_pairs), **kw)
     t1 = time.time()

@@ -272,12 +272,12 @@ int  dvc_net_gradients(dvc_net *net, const void *in_dev, const float *label_dev,
 /* One Adam step (beta 0.9 / 0.999, eps 1e-8) on that gradient. */
 int  dvc_net_train_step(dvc_net *net, const void *in_dev, const float *label_dev, int32_t B, float lr,
                         float *loss_host);
-/* Training pairs for the network: overwrite the context's correlated volume (slot 1)
- * with the reference volume (slot 0) resampled tile by tile, each tile x tile x tile
+/* Training pairs for the network: fill the context's correlated volume (slot 1, made
+ * the shape and type of slot 0) with the reference volume (slot 0) resampled tile by
+ * tile, each tile x tile x tile
  * block t (numbered x fastest) under its own affine map out(x) = ref(M_t x + o_t),
  * maps[12*t..] = M row-major then o, nmaps = the number of tiles; separable Catmull-Rom
- * with clamped indices, rounded and clamped to the voxel range.  Both slots must hold
- * volumes of one shape and type. */
+ * with clamped indices, rounded and clamped to the voxel range. */
 int  dvc_synth_warp_tiles(dvc_ctx *ctx, int tile, const double *maps, int64_t nmaps);
 /* Proposals for every point of a prepared DVC_START_GIVEN / DVC_START_JUDGED search,
  * written straight into its start table (replaces dvc_search_set_starts). */
