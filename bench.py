@@ -44,9 +44,10 @@ def parse():
     ap.add_argument("--bytes-sample", type=int, default=400, help="points sampled for the algorithmic byte count")
     ap.add_argument("--cache", default="", help="directory to keep the generated volumes in between runs")
     ap.add_argument("--no-proposal", action="store_true", help="skip the learned-proposal leg")
-    ap.add_argument("--workload", default="affine", choices=["affine", "full"],
-                    help="affine: the 12-DOF 256^3 workload (default); full: the 1536^3 / 10^6-point case "
-                         "(volumes made on the device), reported in its own JSON line")
+    ap.add_argument("--workload", default="affine", choices=["affine", "full", "learned"],
+                    help="affine: the 12-DOF 256^3 workload (default); full: the 1536^3 / 10^6-point case; "
+                         "learned: the 1024^3 / 10^5-point evaluation of the learned proposal (volumes of both "
+                         "made on the device, each reported in its own JSON line)")
     ap.add_argument("--full-size", type=int, default=1536)
     ap.add_argument("--full-grid", type=int, default=100)
     ap.add_argument("--weights", default="", help="proposal network parameters (.npz); default: the package's")
@@ -124,6 +125,81 @@ def proposal_leg(ctx, a, case, params, order, neigh, plain, D):
             "max_obj_rel_vs_plain": float(rel.max()) if rel.size else None,
             "max_obj_excess_vs_plain": float(excess.max()) if excess.size else None,
             "points_obj_worse_than_plain_by_1e-5": int((excess > 1e-5).sum())}
+
+
+def learned_eval(a, D, synth):
+    """The learned proposal's evaluation on a 1024^3 uint8 pair (translation up to 10
+    voxels per axis plus a sinusoidal field, 10^5 random points): the plain serial
+    chain, the judged proposal mode, the proposal pass alone, and the same one-launch
+    search from one rigid start for every point (the method without the network):
+    points/s, Gauss-Newton evaluations, fallback rate and agreement with the plain
+    search."""
+    from digitalvolumecorrelation_amd import proposal as P
+    t0 = time.time()
+    ctx = D.Context(0)
+    case = synth.config4_eval(ctx)
+    log(f"volumes made on the device in {time.time() - t0:.1f} s: {case.shape} uint8")
+    order, neigh = D.knn_order_neighbors(case.points, start=case.start)
+    params = D.make_params(**case.params)
+    n = len(case.points)
+
+    def timed(prepare_mode, starts=None, propose=None):
+        ctx.set_start_mode(prepare_mode)
+        ctx.prepare(params, case.points, order, neigh)
+        ctx.set_start_mode(D.START_CHAIN)
+        if a.threads:
+            ctx.set_block_threads(a.threads)
+        if starts is not None:
+            ctx.set_starts(starts)
+        for _ in range(max(1, a.warmup)):
+            if propose:
+                propose()
+            ctx.execute()
+            ctx.kernel_ms()
+        t = time.perf_counter()
+        for _ in range(a.steps):
+            if propose:
+                propose()
+            ctx.execute()
+        ctx.kernel_ms()
+        ms = (time.perf_counter() - t) * 1e3 / a.steps
+        return ms, ctx.fetch()
+
+    ms_plain, plain = timed(D.START_CHAIN)
+    levels = ctx.levels()
+    log(f"plain chain: {ms_plain:.1f} ms per step, {levels} levels")
+    net = P.ProposalNet(ctx, max_batch=1024, params=P.ProposalNet.load_params(Path(a.weights) if a.weights else P.WEIGHTS))
+    ms_j, judged = timed(D.START_JUDGED, propose=net.propose)
+    prop, judge, kept = ctx.fetch_passes()
+    log(f"judged proposals: {ms_j:.1f} ms per step")
+    ms_g, given = timed(D.START_GIVEN, propose=net.propose)
+    rigid = np.zeros((n, 12))
+    rigid[:, :3] = case.params["rigid_trans"]
+    ms_r, flat = timed(D.START_GIVEN, starts=rigid)
+    ok = plain["status"] == 0
+
+    def agree(r):
+        good = ok & (r["status"] == 0)
+        du = np.abs(r["par"][:, :3] - plain["par"][:, :3]).max(axis=1)
+        return {"good": int((r["status"] == 0).sum()), "within_1e-3_of_plain": int((good & (du <= 1e-3)).sum()),
+                "within_1e-2_of_plain": int((good & (du <= 1e-2)).sum()),
+                "mean_evaluations": float(r["iters"][r["status"] == 0].mean()) if (r["status"] == 0).any() else None}
+
+    err = np.abs(plain["par"][ok, :3] - case.truth[ok]).max(axis=1)
+    out = {"workload": f"learned-proposal evaluation: {case.shape[0]}^3 u8 pair made on the device (translation "
+                       f"{np.round(case.bumps[0, 4:7], 2).tolist()} + sinusoids of amplitude 2, wavelength 256), "
+                       f"{n} random points, cube 31 / 2197 samples, ZNSSD, 12-DOF, disp_max 12",
+           "levels": levels,
+           "plain": {"value": n / (ms_plain / 1e3), "ms_per_step": ms_plain, "good": int(ok.sum()),
+                     "mean_evaluations": float(plain["iters"][ok].mean()),
+                     "median_err_vs_imposed_voxel": float(np.median(err)), "max_err_vs_imposed_voxel": float(err.max())},
+           "nn_judged": {"value": n / (ms_j / 1e3), "ms_per_step": ms_j, "speedup_vs_plain": ms_plain / ms_j,
+                         "kept_proposal": int((kept == 1).sum()), "fallback_to_judge": int((kept == 0).sum()),
+                         "proposal_pass": agree(prop), "selected": agree(judged)},
+           "nn_given_only": {"value": n / (ms_g / 1e3), "ms_per_step": ms_g, **agree(given)},
+           "rigid_start_given": {"value": n / (ms_r / 1e3), "ms_per_step": ms_r, **agree(flat)}}
+    ctx.close()
+    print(json.dumps(out), flush=True)
 
 
 def full_scale(a, D, synth, workload):
@@ -234,6 +310,8 @@ def main():
 
     if a.workload == "full":
         return full_scale(a, D, synth, workload)
+    if a.workload == "learned":
+        return learned_eval(a, D, synth)
 
     t0 = time.time()
     case = load_case(a.cache, synth)

@@ -100,7 +100,7 @@ def load() -> C.CDLL:
         "dvc_search_propose": (I, [VP, VP]),
         "dvc_synth_warp_tiles": (I, [VP, C.c_int, P(D), C.c_int64]),
         "dvc_synth_noise": (I, [VP, C.c_int, C.c_int, C.c_int64, C.c_int64, C.c_int64, D, C.c_uint64]),
-        "dvc_synth_warp_field": (I, [VP, P(D), I, D, P(D), I]),
+        "dvc_synth_warp_field": (I, [VP, P(D), I, P(D), I, D, P(D), I]),
         "dvc_sort_order_neighbors": (I, [P(D), I, P(D), P(I), P(I), I]),
         "dvc_knn_order_neighbors": (I, [P(D), I, P(D), I, P(I), P(I), I]),
     }
@@ -229,11 +229,14 @@ class Context:
         t, h, w = shape
         self._check(self.lib.dvc_synth_noise(self.h, slot, np.dtype(dtype).itemsize, w, h, t, float(sigma), int(seed)))
 
-    def synth_warp_field(self, bumps: np.ndarray, strain: float, center, iterations: int = 6):
-        """dvc_synth_warp_field: slot 1 = slot 0 under Gaussian bumps + uniform strain."""
+    def synth_warp_field(self, bumps: np.ndarray, strain: float, center, iterations: int = 6, waves=None):
+        """dvc_synth_warp_field: slot 1 = slot 0 under Gaussian bumps, plane sinusoids and
+        a uniform strain."""
         b = np.ascontiguousarray(bumps, dtype=np.float64).reshape(-1, 7)
+        w = np.ascontiguousarray(np.zeros((0, 7)) if waves is None else waves, dtype=np.float64).reshape(-1, 7)
         c = np.ascontiguousarray(center, dtype=np.float64)
-        self._check(self.lib.dvc_synth_warp_field(self.h, _dp(b), len(b), float(strain), _dp(c), int(iterations)))
+        self._check(self.lib.dvc_synth_warp_field(self.h, _dp(b), len(b), _dp(w), len(w), float(strain), _dp(c),
+                                                  int(iterations)))
 
     def attach(self, slot: int, dev_ptr: int, bytes_per: int, shape):
         t, h, w = shape

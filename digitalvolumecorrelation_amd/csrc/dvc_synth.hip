@@ -95,11 +95,15 @@ __global__ void quantize_kernel(const float *v, long long n, const float *part, 
     }
 }
 
-// Displacement field u(X) = sum_b a_b exp(-|X - c_b|^2 / (2 w_b^2)) + e (X - C): Gaussian
-// bumps (bumps[7b..]: centre xyz, width, amplitude xyz) plus a uniform strain e about C.
+// Displacement field u(X) = sum_b a_b exp(-|X - c_b|^2 / (2 w_b^2))
+//                        + sum_s a_s sin(k_s . X + phi_s) + e (X - C):
+// Gaussian bumps (bumps[7b..]: centre xyz, width, amplitude xyz), plane sinusoids
+// (waves[7s..]: wave vector xyz, phase, amplitude xyz) and a uniform strain e about C.
 struct Field {
     const double *bumps;
     int nb;
+    const double *waves;
+    int nw;
     double e, C[3];
 };
 
@@ -111,6 +115,13 @@ __device__ __forceinline__ void disp(const Field &f, const double X[3], double u
         const double *p = f.bumps + 7 * b;
         const double dx = X[0] - p[0], dy = X[1] - p[1], dz = X[2] - p[2];
         const double g = exp(-(dx * dx + dy * dy + dz * dz) / (2.0 * p[3] * p[3]));
+        u[0] += p[4] * g;
+        u[1] += p[5] * g;
+        u[2] += p[6] * g;
+    }
+    for (int s = 0; s < f.nw; s++) {
+        const double *p = f.waves + 7 * s;
+        const double g = sin(p[0] * X[0] + p[1] * X[1] + p[2] * X[2] + p[3]);
         u[0] += p[4] * g;
         u[1] += p[5] * g;
         u[2] += p[6] * g;
@@ -214,9 +225,10 @@ int dvc_synth_noise(dvc_ctx *ctx, int slot, int bytes_per, int64_t wide, int64_t
     return rc;
 }
 
-int dvc_synth_warp_field(dvc_ctx *ctx, const double *bumps, int32_t nbumps, double strain, const double center[3],
-                         int32_t iterations) {
-    if (!ctx || nbumps < 0 || nbumps > 1024 || (nbumps > 0 && !bumps) || !center || iterations < 1 || iterations > 50)
+int dvc_synth_warp_field(dvc_ctx *ctx, const double *bumps, int32_t nbumps, const double *waves, int32_t nwaves,
+                         double strain, const double center[3], int32_t iterations) {
+    if (!ctx || nbumps < 0 || nbumps > 1024 || (nbumps > 0 && !bumps) || nwaves < 0 || nwaves > 1024 ||
+        (nwaves > 0 && !waves) || !center || iterations < 1 || iterations > 50)
         return DVC_ERR_ARG;
     for (int b = 0; b < nbumps; b++)
         if (!(bumps[7 * b + 3] > 0.0)) return DVC_ERR_ARG;
@@ -228,12 +240,15 @@ int dvc_synth_warp_field(dvc_ctx *ctx, const double *bumps, int32_t nbumps, doub
     uint8_t *v1 = ctx_volume_alloc(ctx, 1, bpv, W, H, T);     // the target, same shape and type
     if (!v1) return DVC_ERR_HIP;
     double *d = nullptr;
-    if (hipMalloc(&d, sizeof(double) * 7 * (nbumps > 0 ? nbumps : 1)) != hipSuccess) return DVC_ERR_HIP;
+    if (hipMalloc(&d, sizeof(double) * 7 * (nbumps + nwaves + 1)) != hipSuccess) return DVC_ERR_HIP;
     int rc = DVC_OK;
     if (nbumps > 0 && hipMemcpyAsync(d, bumps, sizeof(double) * 7 * nbumps, hipMemcpyHostToDevice, st) != hipSuccess)
         rc = DVC_ERR_HIP;
+    if (nwaves > 0 &&
+        hipMemcpyAsync(d + 7 * nbumps, waves, sizeof(double) * 7 * nwaves, hipMemcpyHostToDevice, st) != hipSuccess)
+        rc = DVC_ERR_HIP;
     Field f;
-    f.bumps = d; f.nb = nbumps; f.e = strain;
+    f.bumps = d; f.nb = nbumps; f.waves = d + 7 * nbumps; f.nw = nwaves; f.e = strain;
     for (int q = 0; q < 3; q++) f.C[q] = center[q];
     const long long n = (long long)W * H * T;
     if (rc == DVC_OK) {

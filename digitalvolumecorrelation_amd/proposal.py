@@ -264,7 +264,7 @@ class TrainSet:
 
 
 def build_train_set(ctx: Context, net: ProposalNet, seeds, size=384, tile=64, log=print, shuffle_seed=0,
-                    maps_per_ref=1, per_axis=5, spacing=3.0, flip=True, device_textures=False):
+                    maps_per_ref=1, per_axis=5, spacing=3.0, flip=True, device_textures=False, trans=8.0):
     """Crops and plain-search labels of tiled pairs, resident in HBM, in a random order
     over all pairs (every contiguous batch mixes them).  Each reference volume serves
     `maps_per_ref` pairs with different tile maps, so a texture seen in training comes
@@ -278,7 +278,7 @@ def build_train_set(ctx: Context, net: ProposalNet, seeds, size=384, tile=64, lo
             ref = np.zeros((size, size, size), np.uint8)     # shape only; the texture is in slot 0
         elif i % maps_per_ref == 0:
             ref = reference_volume(int(s), size)
-        tp = tiled_pair(int(s), size=size, tile=tile, per_axis=per_axis, spacing=spacing, ref=ref)
+        tp = tiled_pair(int(s), size=size, tile=tile, per_axis=per_axis, spacing=spacing, ref=ref, trans=trans)
         pts, lab = label_pair(ctx, tp, tile, on_device=device_textures)
         if device_textures:
             # crop now, while the pair is in the context (slots hold one pair at a time)

@@ -191,13 +191,17 @@ class DeviceCase:
     strain: float
 
 
-def bump_field(X, bumps, strain, center):
-    """u(X) of dvc_synth_warp_field: Gaussian bumps plus a uniform strain about center."""
+def bump_field(X, bumps, strain, center, waves=None):
+    """u(X) of dvc_synth_warp_field: Gaussian bumps, plane sinusoids and a uniform
+    strain about center."""
     X = np.asarray(X, dtype=np.float64)
     u = strain * (X - np.asarray(center))
     for b in np.asarray(bumps).reshape(-1, 7):
         g = np.exp(-((X - b[:3]) ** 2).sum(axis=-1) / (2.0 * b[3] ** 2))
         u = u + g[..., None] * b[4:7]
+    if waves is not None:
+        for w in np.asarray(waves).reshape(-1, 7):
+            u = u + np.sin(X @ w[:3] + w[3])[..., None] * w[4:7]
     return u
 
 
@@ -225,3 +229,37 @@ def config5(ctx, size: int = 1536, n: int = 100, spacing: float = 14.0, seed: in
     params = dict(ndof=12, obj="znssd", interp="tricubic", geom="cube", subvol_size=diameter, subvol_npts=npts,
                   disp_max=10, basin_radius=0.0, rigid_trans=tuple(float(np.rint(v)) for v in u0))
     return DeviceCase("full-u16", shape, np.uint16, pts, truth, params, start, bumps, strain)
+
+
+def config4_eval(ctx, size: int = 1024, npoints: int = 100_000, seed: int = 4, trans: float = 10.0,
+                 amplitude: float = 2.0, wavelength: float = 256.0, margin: float = 48.0) -> DeviceCase:
+    """Evaluation pair of the learned proposal, made on the device: uint8 size^3 smoothed
+    noise (sigma 2); the target under a uniform translation (each axis up to `trans`
+    voxels; a bump far wider than the volume) plus a sinusoidal field (amplitude
+    `amplitude` per axis, wavelength `wavelength`, random direction and phase per axis);
+    `npoints` points uniform inside `margin`; cube 31 / 2197 samples, ZNSSD, 12-DOF,
+    disp_max 12, processing order from the volume centre."""
+    rng = np.random.default_rng(seed)
+    shape = (size, size, size)
+    ctx.synth_noise(0, shape, np.uint8, 2.0, 2000 + seed)
+    t = rng.uniform(-trans, trans, 3)
+    bumps = np.array([[0.0, 0.0, 0.0, 1e9, *t]])
+    waves = []
+    for ax in range(3):
+        k = rng.normal(size=3)
+        k = k / np.linalg.norm(k) * (2 * np.pi / wavelength)
+        a = np.zeros(3)
+        a[ax] = amplitude
+        waves.append(np.concatenate([k, [rng.uniform(0, 2 * np.pi)], a]))
+    waves = np.array(waves)
+    center = np.full(3, (size - 1) / 2.0)
+    ctx.synth_warp_field(bumps, 0.0, center, waves=waves)
+    pts = rng.uniform(margin, size - 1 - margin, (npoints, 3))
+    truth = bump_field(pts, bumps, 0.0, center, waves)
+    start = np.full(3, size / 2.0)
+    u0 = bump_field(start[None], bumps, 0.0, center, waves)[0]
+    params = dict(ndof=12, obj="znssd", interp="tricubic", geom="cube", subvol_size=31, subvol_npts=2197,
+                  disp_max=12, basin_radius=0.0, rigid_trans=tuple(float(np.rint(v)) for v in u0))
+    case = DeviceCase("learned-eval-u8", shape, np.uint8, pts, truth, params, start, bumps, 0.0)
+    case.waves = waves
+    return case

@@ -35,6 +35,7 @@ def main(argv=None):
     ap.add_argument("--per-axis", type=int, default=3, help="points per tile and axis")
     ap.add_argument("--spacing", type=float, default=5.0)
     ap.add_argument("--held-pairs", type=int, default=2)
+    ap.add_argument("--trans", type=float, default=8.0, help="largest translation per axis of a tile's map")
     ap.add_argument("--device-textures", action="store_true",
                     help="make every reference texture on the device (dvc_synth_noise), one per pair")
     ap.add_argument("--size", type=int, default=384)
@@ -54,7 +55,7 @@ def main(argv=None):
     net = P.ProposalNet(ctx, max_batch=a.batch, training=True, params=P.init_params(a.seed))
     s0 = a.volume_seed0
     kw = dict(size=a.size, log=log, maps_per_ref=a.maps_per_ref, per_axis=a.per_axis, spacing=a.spacing,
-              device_textures=a.device_textures)
+              device_textures=a.device_textures, trans=a.trans)
     ts = P.build_train_set(ctx, net, range(s0, s0 + a.pairs), **kw)
     held = P.build_train_set(ctx, net, range(s0 + 1000, s0 + 1000 + a.held_pairs), **kw)
     t1 = time.time()
@@ -66,7 +67,7 @@ def main(argv=None):
     rmse = (np.sqrt(per) * P.SCALE).round(5).tolist()
     lab = ts.labels_host
     net.save(a.out, history=np.array(hist), samples=ts.n, pairs=a.pairs, maps_per_ref=a.maps_per_ref,
-             epochs=a.epochs, batch=a.batch,
+             epochs=a.epochs, batch=a.batch, trans=a.trans, device_textures=a.device_textures,
              label_std=lab.std(axis=0), held_rmse=np.array(rmse))
     print(json.dumps({"samples": ts.n, "held_out": held.n, "epochs": a.epochs, "batch": a.batch,
                       "data_s": round(t1 - t0, 1), "train_s": round(t2 - t1, 1), "loss": hist,

@@ -125,13 +125,15 @@ int dvc_volume_download(dvc_ctx *ctx, int slot, void *host);
  * smoothing with `sigma` (truncated at 4 sigma, mirror-reflected at the faces), rescaled
  * so that its minimum and maximum map to 0 and the type's maximum, rounded.
  * dvc_synth_warp_field: fill the correlated slot (1) with the reference (slot 0) under
- * the displacement field u(X) = sum_b a_b exp(-|X - c_b|^2 / (2 w_b^2)) + strain (X - center):
- * target(x) = ref(X) where X + u(X) = x (`iterations` fixed-point steps), separable
- * Catmull-Rom, rounded and clamped.  bumps[7*b..] = c_b (xyz), w_b, a_b (xyz). */
+ * the displacement field u(X) = sum_b a_b exp(-|X - c_b|^2 / (2 w_b^2))
+ * + sum_s a_s sin(k_s . X + phi_s) + strain (X - center): target(x) = ref(X) where
+ * X + u(X) = x (`iterations` fixed-point steps), separable Catmull-Rom, rounded and
+ * clamped.  bumps[7*b..] = c_b (xyz), w_b, a_b (xyz); waves[7*s..] = k_s (xyz), phi_s,
+ * a_s (xyz). */
 int dvc_synth_noise(dvc_ctx *ctx, int slot, int bytes_per, int64_t wide, int64_t high, int64_t tall, double sigma,
                     uint64_t seed);
-int dvc_synth_warp_field(dvc_ctx *ctx, const double *bumps, int32_t nbumps, double strain, const double center[3],
-                         int32_t iterations);
+int dvc_synth_warp_field(dvc_ctx *ctx, const double *bumps, int32_t nbumps, const double *waves, int32_t nwaves,
+                         double strain, const double center[3], int32_t iterations);
 
 /* ---------------------------------------------------------- interpolation */
 

@@ -40,6 +40,9 @@ def test_identity_and_uniform_translation(ctx):
     c = np.array([20.0, 18.0, 16.0])
     ctx.synth_warp_field(np.zeros((0, 7)), 0.0, c)
     assert np.array_equal(ctx.download(1, shape, np.uint16), ref)
+    # a sinusoid of zero amplitude is the identity too
+    ctx.synth_warp_field(np.zeros((0, 7)), 0.0, c, waves=np.array([[0.1, 0.2, 0.3, 1.0, 0.0, 0.0, 0.0]]))
+    assert np.array_equal(ctx.download(1, shape, np.uint16), ref)
     t = np.array([1.25, -0.5, 2.75])
     ctx.synth_warp_field(np.array([[0, 0, 0, 1e9, *t]]), 0.0, c)
     got = ctx.download(1, shape, np.uint16).astype(np.int64)
