@@ -44,9 +44,10 @@ def parse():
     ap.add_argument("--bytes-sample", type=int, default=400, help="points sampled for the algorithmic byte count")
     ap.add_argument("--cache", default="", help="directory to keep the generated volumes in between runs")
     ap.add_argument("--no-proposal", action="store_true", help="skip the learned-proposal leg")
-    ap.add_argument("--workload", default="affine", choices=["affine", "full", "learned"],
+    ap.add_argument("--workload", default="affine", choices=["affine", "full", "learned", "uneven"],
                     help="affine: the 12-DOF 256^3 workload (default); full: the 1536^3 / 10^6-point case; "
-                         "learned: the 1024^3 / 10^5-point evaluation of the learned proposal (volumes of both "
+                         "learned: the 1024^3 / 10^5-point evaluation of the learned proposal; uneven: 512^3 with "
+                         "5*10^4 points of per-point subvolume shape, size and sample count (volumes of these "
                          "made on the device, each reported in its own JSON line)")
     ap.add_argument("--full-size", type=int, default=1536)
     ap.add_argument("--full-grid", type=int, default=100)
@@ -125,6 +126,119 @@ def proposal_leg(ctx, a, case, params, order, neigh, plain, D):
             "max_obj_rel_vs_plain": float(rel.max()) if rel.size else None,
             "max_obj_excess_vs_plain": float(excess.max()) if excess.size else None,
             "points_obj_worse_than_plain_by_1e-5": int((excess > 1e-5).sum())}
+
+
+def uneven(a, D, synth):
+    """The batched uneven workload: every point its own subvolume shape, size and sample
+    count (dvc_set_point_geometry), SSD and ZNSSD, 6-DOF.  Per objective: the serial chain
+    with fixed and autotuned threads per point, the judged proposal mode and the one-launch
+    search from one rigid start; agreement with the chain; the oracle on a prefix of the
+    order (one RunControl per point, volumes copied back to the host)."""
+    from digitalvolumecorrelation_amd import proposal as P
+    t0 = time.time()
+    ctx = D.Context(0)
+    case, geom, sub, nsamp = synth.config3(ctx)
+    n = len(case.points)
+    log(f"volumes made on the device in {time.time() - t0:.1f} s; {n} points, samples per point "
+        f"{nsamp.min()}..{nsamp.max()} (cubes rounded up), {len(set(zip(geom, sub, nsamp)))} geometries")
+    order, neigh = D.knn_order_neighbors(case.points, start=case.start)
+    net = P.ProposalNet(ctx, max_batch=1024, params=P.ProposalNet.load_params(Path(a.weights) if a.weights else P.WEIGHTS))
+    out = {"workload": f"batched uneven: {case.shape[0]}^3 u8 pair made on the device (sinusoids of amplitude 3, "
+                       f"wavelength 128), {n} random points, each a cube or sphere of size 21..61 with 1000..10000 "
+                       f"samples, tricubic, 6-DOF, disp_max 6", "points": n}
+    ref = tar = None
+    for obj in ("znssd", "ssd"):
+        params = D.make_params(**dict(case.params, obj=obj))
+
+        def timed(mode, starts=None, propose=None, threads=512, tune=False):
+            ctx.set_point_geometry(geom, sub, nsamp)
+            ctx.set_start_mode(mode)
+            ctx.set_block_threads(threads)
+            t1 = time.time()
+            ctx.prepare(params, case.points, order, neigh)
+            prep = time.time() - t1
+            if starts is not None:
+                ctx.set_starts(starts)
+            if propose:
+                propose()
+            tuned = ctx.autotune(1) if tune else None
+            for _ in range(max(1, a.warmup)):
+                if propose:
+                    propose()
+                ctx.execute()
+                ctx.kernel_ms()
+            t1 = time.perf_counter()
+            for _ in range(a.steps):
+                if propose:
+                    propose()
+                ctx.execute()
+            ctx.kernel_ms()
+            ms = (time.perf_counter() - t1) * 1e3 / a.steps
+            return ms, ctx.fetch(), prep, tuned
+
+        ms_p, plain, prep, _ = timed(D.START_CHAIN)
+        levels = ctx.levels()
+        fixed = {}
+        for t in (128, 256, 1024):
+            fixed[t] = timed(D.START_CHAIN, threads=t)[0]
+        fixed[512] = ms_p
+        ms_t, tuned_r, _, tuned = timed(D.START_CHAIN, tune=True)
+        ms_j, judged, _, _ = timed(D.START_JUDGED, propose=net.propose)
+        kept = ctx.fetch_passes()[2]
+        rigid = np.zeros((n, 12))
+        rigid[:, :3] = case.params["rigid_trans"]
+        ms_r, flat, _, _ = timed(D.START_GIVEN, starts=rigid)
+        ok = plain["status"] == 0
+        err = np.abs(plain["par"][ok, :3] - case.truth[ok]).max(axis=1)
+
+        def agree(r):
+            good = ok & (r["status"] == 0)
+            du = np.abs(r["par"][:, :3] - plain["par"][:, :3]).max(axis=1)
+            dobj = r["obj"][good] - plain["obj"][good]
+            e = np.abs(r["par"][good, :3] - case.truth[good]).max(axis=1)
+            return {"good": int((r["status"] == 0).sum()), "within_1e-3_of_plain": int((good & (du <= 1e-3)).sum()),
+                    "obj_not_worse_than_plain_by_1e-6": int((dobj <= 1e-6).sum()),
+                    "obj_excess_vs_plain_p50_p99_max": [float(np.quantile(dobj, 0.5)), float(np.quantile(dobj, 0.99)),
+                                                        float(dobj.max())],
+                    "median_err_vs_imposed_voxel": float(np.median(e)),
+                    "mean_evaluations": float(r["iters"][r["status"] == 0].mean())}
+
+        leg = {"levels": levels, "prepare_s": round(prep, 1),
+               "plain": {"value": n / (ms_p / 1e3), "ms_per_step": ms_p, "threads_per_point": 512, "good": int(ok.sum()),
+                         "mean_evaluations": float(plain["iters"][ok].mean()),
+                         "median_err_vs_imposed_voxel": float(np.median(err)),
+                         "max_err_vs_imposed_voxel": float(err.max())},
+               "plain_fixed_threads_ms_per_step": {str(t): fixed[t] for t in sorted(fixed)},
+               "plain_autotuned": {"value": n / (ms_t / 1e3), "ms_per_step": ms_t, "threads_per_point": tuned[0],
+                                   **agree(tuned_r)},
+               "nn_judged": {"value": n / (ms_j / 1e3), "ms_per_step": ms_j, "speedup_vs_plain": ms_p / ms_j,
+                             "kept_proposal": int((kept == 1).sum()), **agree(judged)},
+               "rigid_start_given": {"value": n / (ms_r / 1e3), "ms_per_step": ms_r, **agree(flat)}}
+        log(f"{obj}: plain {ms_p:.1f} ms ({levels} levels), autotuned {ms_t:.1f} ms ({tuned[0]} threads), "
+            f"judged {ms_j:.1f} ms, rigid start {ms_r:.1f} ms")
+        if not a.no_cpu:
+            sys.path.insert(0, str(ROOT / "oracle"))
+            import oracle as O
+            if ref is None:
+                ref = ctx.download(0, case.shape, np.uint8)
+                tar = ctx.download(1, case.shape, np.uint8)
+            M = min(a.cpu_points, n)
+            thr = a.cpu_threads or min(16, os.cpu_count() or 1)
+            run = O.make_run(shape=case.shape, bytes_per=1, jacobian=0, **dict(case.params, obj=obj))
+            ores, sec, m = O.dvc_run_geo(run, ref, tar, case.points, order, neigh, geom, sub, nsamp, nproc=M,
+                                         nthreads=thr)
+            idx = order[:m]
+            g, o = plain[idx], ores[idx]
+            okk = o["status"] == 0
+            leg["cpu_baseline"] = {"value": m / sec, "unit": "points/s", "cores": thr, "kind": "port",
+                                   "sample": f"first {m} points of the processing order, {sec:.1f} s",
+                                   "status_equal": bool(np.array_equal(g["status"], o["status"])),
+                                   "iters_equal": int((g["iters"] == o["iters"]).sum()),
+                                   "max_du_voxel": float(np.abs(g["par"][okk, :3] - o["par"][okk, :3]).max())}
+        out[obj] = leg
+    ctx.set_point_geometry()
+    ctx.close()
+    print(json.dumps(out), flush=True)
 
 
 def learned_eval(a, D, synth):
@@ -312,6 +426,8 @@ def main():
         return full_scale(a, D, synth, workload)
     if a.workload == "learned":
         return learned_eval(a, D, synth)
+    if a.workload == "uneven":
+        return uneven(a, D, synth)
 
     t0 = time.time()
     case = load_case(a.cache, synth)

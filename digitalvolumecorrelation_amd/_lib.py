@@ -79,6 +79,7 @@ def load() -> C.CDLL:
         "dvc_set_launch_mode": (I, [VP, C.c_int]),
         "dvc_search_info": (I, [VP, C.c_void_p, C.c_void_p, C.c_void_p]),
         "dvc_set_start_mode": (I, [VP, C.c_int]),
+        "dvc_set_point_geometry": (I, [VP, P(I), P(I), P(I), I]),
         "dvc_search_autotune": (I, [VP, C.c_int, P(I), P(C.c_float)]),
         "dvc_search_set_starts": (I, [VP, VP, C.c_int]),
         "dvc_search_fetch_passes": (I, [VP, VP, VP, VP]),
@@ -286,6 +287,20 @@ class Context:
         bt, ms = C.c_int32(), C.c_float()
         self._check(self.lib.dvc_search_autotune(self.h, reps, C.byref(bt), C.byref(ms)))
         return bt.value, ms.value
+
+    def set_point_geometry(self, geom=None, size=None, nsamp=None):
+        """Per-point subvolume shape (0 cube, 1 sphere or names), size and sample count for
+        the next prepare; no arguments: one geometry for all points."""
+        if geom is None:
+            self._check(self.lib.dvc_set_point_geometry(self.h, None, None, None, 0))
+            return
+        g = np.ascontiguousarray([GEOM[x] if isinstance(x, str) else int(x) for x in geom], dtype=np.int32)
+        s = np.ascontiguousarray(size, dtype=np.int32)
+        m = np.ascontiguousarray(nsamp, dtype=np.int32)
+        if not (len(g) == len(s) == len(m)):
+            raise ValueError("geometry arrays differ in length")
+        self._keep_pg = (g, s, m)
+        self._check(self.lib.dvc_set_point_geometry(self.h, _ip(g), _ip(s), _ip(m), len(g)))
 
     def set_start_mode(self, mode: int):
         """START_CHAIN (the reference's serial start guess), START_GIVEN or START_JUDGED;

@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <random>
 #include <string>
 #include <thread>
@@ -56,6 +57,9 @@ struct dvc_ctx {
     std::vector<char> searched;
     std::vector<int> level_off;      // dependency levels (dvc_geometry.h)
     float *d_rvals = nullptr;        // reference samples of every searched point (or null)
+    // per-point geometry for the next prepare (empty: the run parameters' for all points)
+    std::vector<int32_t> pg_geom, pg_size, pg_npts;
+    PtGeo *d_pgeo = nullptr;
     RefStats *d_rstat = nullptr;
     // start modes (DVC_START_*): the mode set for the next prepare and the prepared one
     int start_mode = DVC_START_CHAIN;
@@ -110,7 +114,8 @@ void free_plan(dvc_ctx *c) {
     drop_graph(c);
     hipFree(c->d_pts); hipFree(c->d_sph); hipFree(c->d_order); hipFree(c->d_pos);
     hipFree(c->d_neigh); hipFree(c->d_levels); hipFree(c->d_res);
-    hipFree(c->d_rvals); hipFree(c->d_rstat);
+    hipFree(c->d_rvals); hipFree(c->d_rstat); hipFree(c->d_pgeo);
+    c->d_pgeo = nullptr;
     hipFree(c->d_start); hipFree(c->d_res_prop); hipFree(c->d_res_judge); hipFree(c->d_kept);
     c->d_start = nullptr; c->d_res_prop = c->d_res_judge = nullptr; c->d_kept = nullptr;
     c->starts_set = false;
@@ -445,7 +450,78 @@ int dvc_search_prepare(dvc_ctx *ctx, const dvc_run_params *rc, const double *poi
     // sphere template: identical offsets for every point (one seed, Search.cpp:102-105)
     std::vector<double> sph;
     double omn[3] = {0, 0, 0}, omx[3] = {0, 0, 0};
-    if (rc->geom == DVC_GEOM_SPHERE) {
+    const bool uneven = !ctx->pg_geom.empty();
+    std::vector<PtGeo> pgeo;
+    size_t table_floats = (size_t)N * (size_t)nsamp_pad;
+    if (uneven) {
+        // one geometry class per distinct (shape, size, samples): Search::Search of a
+        // RunControl with those fields (Search.cpp:24-82), the sphere template of its
+        // seed; the reference samples of each point at its own offset of one table
+        if ((int)ctx->pg_geom.size() != npts) return fail(ctx, DVC_ERR_ARG, "point geometry has another point count");
+        if (!table) return fail(ctx, DVC_ERR_ARG, "per-point geometry needs the HBM sample table");
+        std::map<std::tuple<int, int, int>, int> index;      // class of each distinct triple
+        std::vector<std::tuple<int, int, int>> keys;
+        std::vector<int> cls(npts);
+        for (int i = 0; i < npts; i++) {
+            const auto key = std::make_tuple(ctx->pg_geom[i], ctx->pg_size[i], ctx->pg_npts[i]);
+            auto it = index.find(key);
+            if (it == index.end()) {
+                dvc_run_params ri = *rc;
+                ri.geom = ctx->pg_geom[i]; ri.subvol_size = ctx->pg_size[i]; ri.subvol_npts = ctx->pg_npts[i];
+                if (int e = check_params(ctx, &ri)) return e;
+                it = index.emplace(key, (int)keys.size()).first;
+                keys.push_back(key);
+            }
+            cls[i] = it->second;
+        }
+        // every class: its geometry and (spheres) its sample template, on host threads
+        const long long wb = (long long)ctx->vol[0].W * bpv, whb = wb * (long long)ctx->vol[0].H;
+        const int nc = (int)keys.size();
+        std::vector<PtGeo> cg(nc, PtGeo{});
+        std::vector<std::vector<double>> tmpl(nc);
+        auto build = [&](int c0, int c1) {
+            for (int c = c0; c < c1; c++) {
+                dvc_run_params ri = *rc;
+                ri.geom = std::get<0>(keys[c]); ri.subvol_size = std::get<1>(keys[c]); ri.subvol_npts = std::get<2>(keys[c]);
+                const Geometry Gi = make_geometry(ri);
+                PtGeo &g = cg[c];
+                g.geom = ri.geom; g.nsamp = Gi.subv_num; g.g = Gi.g; g.subv_rad = Gi.subv_rad;
+                g.gather32 = (wb % 4 == 0) && ((long long)(Gi.box[2] + 1) * whb + 64 < (1ll << 31)) ? 1 : 0;
+                for (int q = 0; q < 3; q++) {
+                    g.box[q] = Gi.box[q];
+                    g.est_mn[q] = Gi.est_mn[q]; g.est_mx[q] = Gi.est_mx[q]; g.est_cn[q] = Gi.est_cn[q];
+                    g.act_mn[q] = Gi.act_mn[q]; g.act_mx[q] = Gi.act_mx[q]; g.act_cn[q] = Gi.act_cn[q];
+                }
+                if (ri.geom == DVC_GEOM_SPHERE) {
+                    double mn[3], mx[3];
+                    sphere_template(ri, Gi, tmpl[c], mn, mx);
+                    locality_order(tmpl[c]);
+                    for (int q = 0; q < 3; q++) { g.off_min[q] = mn[q]; g.off_max[q] = mx[q]; }
+                }
+            }
+        };
+        const int nth = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
+        std::vector<std::thread> pool;
+        for (int t = 0; t < nth; t++) pool.emplace_back(build, (int)((long long)nc * t / nth), (int)((long long)nc * (t + 1) / nth));
+        for (auto &th : pool) th.join();
+        for (int c = 0; c < nc; c++)
+            if (cg[c].geom == DVC_GEOM_SPHERE) {
+                cg[c].sph_off = (long long)(sph.size() / 3);
+                sph.insert(sph.end(), tmpl[c].begin(), tmpl[c].end());
+            }
+        pgeo.assign(npts, PtGeo{});
+        for (int i = 0; i < npts; i++) pgeo[i] = cg[cls[i]];
+        // reference-sample rows in processing order
+        size_t off = 0;
+        for (int q = 0; q < N; q++) {
+            PtGeo &g = pgeo[order[q]];
+            g.rval_off = (long long)off;
+            off += (size_t)((g.nsamp + 3) & ~3);
+        }
+        table_floats = off;
+        if (table_floats * sizeof(float) > kRefBudgetBytes)
+            return fail(ctx, DVC_ERR_ARG, "reference samples of all points exceed the HBM table budget");
+    } else if (rc->geom == DVC_GEOM_SPHERE) {
         sphere_template(*rc, G, sph, omn, omx);
         locality_order(sph);
     }
@@ -477,8 +553,15 @@ int dvc_search_prepare(dvc_ctx *ctx, const dvc_run_params *rc, const double *poi
         HIPCHK(hipMemsetAsync(ctx->d_kept, 0, sizeof(int) * (size_t)npts, ctx->stream));
     }
     if (table) {
-        if (int e = dev_alloc(ctx, &ctx->d_rvals, (size_t)N * nsamp_pad)) return e;
+        if (int e = dev_alloc(ctx, &ctx->d_rvals, table_floats)) return e;
         if (int e = dev_alloc(ctx, &ctx->d_rstat, (size_t)N)) return e;
+    }
+    if (uneven) {
+        if (int e = dev_alloc(ctx, &ctx->d_pgeo, (size_t)npts)) return e;
+        HIPCHK(hipMemcpyAsync(ctx->d_pgeo, pgeo.data(), sizeof(PtGeo) * npts, hipMemcpyHostToDevice, ctx->stream));
+    } else if (ctx->d_pgeo) {
+        hipFree(ctx->d_pgeo);
+        ctx->d_pgeo = nullptr;
     }
     HIPCHK(hipMemcpyAsync(ctx->d_pts, points, sizeof(double) * 3 * npts, hipMemcpyHostToDevice, ctx->stream));
     if (!sph.empty()) HIPCHK(hipMemcpyAsync(ctx->d_sph, sph.data(), sizeof(double) * sph.size(), hipMemcpyHostToDevice, ctx->stream));
@@ -517,6 +600,7 @@ static int enqueue_search(dvc_ctx *ctx, hipStream_t st) {
     a.start = nullptr;
     a.rvals = ctx->d_rvals;
     a.rstat = ctx->d_rstat;
+    a.pgeo = ctx->d_pgeo;
     int bt = ctx->chosen_threads;
     size_t lds = search_lds_bytes(a.nsamp_pad, bt, a.rvals == nullptr);
     const long long res_words = (long long)ctx->npts * (long long)(sizeof(dvc_point_result) / 4);
@@ -597,6 +681,18 @@ int dvc_synchronize(dvc_ctx *ctx) {
     if (!ctx) return DVC_ERR_ARG;
     HIPCHK(hipSetDevice(ctx->device));
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    return DVC_OK;
+}
+
+int dvc_set_point_geometry(dvc_ctx *ctx, const int32_t *geom, const int32_t *size, const int32_t *nsamp, int32_t n) {
+    if (!ctx || n < 0) return DVC_ERR_ARG;
+    if (n == 0 || !geom || !size || !nsamp) {
+        ctx->pg_geom.clear(); ctx->pg_size.clear(); ctx->pg_npts.clear();
+        return DVC_OK;
+    }
+    ctx->pg_geom.assign(geom, geom + n);
+    ctx->pg_size.assign(size, size + n);
+    ctx->pg_npts.assign(nsamp, nsamp + n);
     return DVC_OK;
 }
 

@@ -14,6 +14,20 @@ struct RefStats {
     double rmean, rbmean, R2, Src, Rn2;
 };
 
+// Sub-volume geometry of one point when points differ in shape, size and sample count
+// (one RunControl per point; the reference has one for all).  The search kernels apply
+// it over their arguments for that point.
+struct PtGeo {
+    int geom, nsamp, g, gather32;
+    int box[3], pad_;
+    double subv_rad;
+    double est_mn[3], est_mx[3], est_cn[3];
+    double act_mn[3], act_mx[3], act_cn[3];
+    double off_min[3], off_max[3];
+    long long sph_off;               // first sample of its sphere template in SearchArgs::sph (samples)
+    long long rval_off;              // first entry of its reference samples in SearchArgs::rvals
+};
+
 // Arguments of the search kernel.  Geometry mirrors Search::Search
 // (Search.cpp:24-82) and Interpolate::Interpolate (Interpolate.cpp:34-56).
 struct SearchArgs {
@@ -38,6 +52,7 @@ struct SearchArgs {
     const double *start;             // [npts*12] given start warps (a proposal pass), or null
     float *rvals;                    // [nproc * nsamp_pad] reference samples by position, or null
     RefStats *rstat;                 // [nproc]
+    const PtGeo *pgeo;               // [npts] per-point geometry, or null (one geometry for all)
 };
 
 struct StrainArgs {
@@ -53,6 +68,7 @@ struct StrainArgs {
 // launchers implemented in dvc_kernels.hip
 int launch_search(const SearchArgs &a, const int *level_pts, int block_threads, int grid_blocks, size_t lds_bytes,
                   void *stream);
+// the geometry a point's search uses (pgeo[n] applied, or a itself)
 int search_set_lds_limit(size_t lds_total);
 int search_static_lds(int bpv, int block_threads, size_t *bytes);
 int search_max_blocks(int bpv, int block_threads, size_t lds_bytes, int *blocks_per_cu);

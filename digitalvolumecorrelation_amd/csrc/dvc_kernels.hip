@@ -554,7 +554,7 @@ __device__ __forceinline__ void warp_pt(const PtShared &S, int ndof, double xs, 
 
 // Interpolate::center_on (Interpolate.cpp:569-574, BoundBox.cpp:75-85) applied to the
 // nominal boxes, then the volume extent check.  Thread 0 only.
-__device__ void place_box(const SearchArgs &a, PtShared &S, double cx, double cy, double cz) {
+__device__ __forceinline__ void place_box(const SearchArgs &a, PtShared &S, double cx, double cy, double cz) {
 #pragma clang fp contract(off)
     double c[3] = {cx, cy, cz};
     S.box_ok = 1;
@@ -574,7 +574,7 @@ __device__ void place_box(const SearchArgs &a, PtShared &S, double cx, double cy
 // Range rule of tri_cub_Lek / tri_lin / nearest: act_box.contains(bbox)
 // (Interpolate.cpp:474-481, BoundBox.cpp:87-99), box inside the volume, and the cells
 // of the extreme samples inside the box (the reference indexes past kern_4d there).
-__device__ bool range_ok(const SearchArgs &a, const PtShared &S, const double bb[6]) {
+__device__ __forceinline__ bool range_ok(const SearchArgs &a, const PtShared &S, const double bb[6]) {
     for (int q = 0; q < 6; q++)
         if (!(bb[q] == bb[q]) || fabs(bb[q]) > 1.0e15) return false;
     if (bb[0] < S.act_mn[0] || bb[1] < S.act_mn[1] || bb[2] < S.act_mn[2] ||
@@ -649,7 +649,7 @@ __device__ __forceinline__ void set_warp(PtShared &S, int ndof) {
 // Search::obj_val_at (Search.cpp:448-488).  With need_j the analytic-Jacobian moments
 // are left in V (16x16, FP64).  Returns false on Range_Fail.
 template <int BPV, int B>
-__device__ bool evaluate(const SearchArgs &a, PtShared &S, int ndof, bool need_j,
+__device__ __forceinline__ bool evaluate(const SearchArgs &a, PtShared &S, int ndof, bool need_j,
                          const float *rsrc, float *stage, double *red, double *V) {
     constexpr int NW = B / 64;
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
@@ -933,7 +933,7 @@ __device__ __forceinline__ int shfl_from(int v, int src) { return __shfl(v, src,
 // every column is updated by its own lane with the same operation order as a serial
 // loop.  Adds the step to S.par and stores it in S.upd (Search.cpp:236-240).
 template <int N>
-__device__ void gn_step(const SearchArgs &a, PtShared &S, const double *V) {
+__device__ __forceinline__ void gn_step(const SearchArgs &a, PtShared &S, const double *V) {
 #pragma clang fp contract(off)
     const int lane = threadIdx.x & 63;
     const double n = (double)a.nsamp;
@@ -1109,7 +1109,7 @@ __device__ void gn_step(const SearchArgs &a, PtShared &S, const double *V) {
 // Point set-up (Search.cpp:98-119, 252-258): cloud geometry, the reference box on the
 // point, its range rule and the integer bias.  Returns false (uniformly) on Range_Fail.
 template <int BPV, int B>
-__device__ bool point_setup(const SearchArgs &a, int n, PtShared &S) {
+__device__ __forceinline__ bool point_setup(const SearchArgs &a, int n, PtShared &S) {
 #pragma clang fp contract(off)
     const int tid = threadIdx.x;
     // ---- cloud geometry and reference box (Search.cpp:98-119, 252-258)
@@ -1155,7 +1155,7 @@ __device__ bool point_setup(const SearchArgs &a, int n, PtShared &S) {
 // The reference sub-volume (Search.cpp:252-258): sample values at the stable cloud,
 // centred, left in rcb[], and their sums in S.
 template <int BPV, int B>
-__device__ void reference_pass(const SearchArgs &a, PtShared &S, float *rcb, double *red) {
+__device__ __forceinline__ void reference_pass(const SearchArgs &a, PtShared &S, float *rcb, double *red) {
 #pragma clang fp contract(off)
     const int tid = threadIdx.x;
     const int ns = a.nsamp;
@@ -1231,7 +1231,7 @@ __device__ void reference_pass(const SearchArgs &a, PtShared &S, float *rcb, dou
 }
 
 template <int BPV, int B>
-__device__ void point_search(const SearchArgs &a, int p, int n, PtShared &S, float *rcb, float *stage,
+__device__ __forceinline__ void point_search(const SearchArgs &a, int p, int n, PtShared &S, float *rcb, float *stage,
                              double *red, double *V) {
 #pragma clang fp contract(off)
     const int tid = threadIdx.x, lane = tid & 63;
@@ -1384,16 +1384,32 @@ __device__ void point_search(const SearchArgs &a, int p, int n, PtShared &S, flo
     }
 }
 
+// Per-point geometry (PtGeo of point n) applied over the launch's arguments, by thread 0
+// into the workgroup's shared copy.  The point's reference samples start at
+// rvals + rval_off, so its copy carries that base and a row stride of 0: the search
+// functions address them as rvals + p * nsamp_pad.
+__device__ __forceinline__ void apply_geo(SearchArgs &la, const SearchArgs &a, int n) {
+    la = a;
+    const PtGeo &g = a.pgeo[n];
+    la.geom = g.geom; la.nsamp = g.nsamp; la.g = g.g; la.gather32 = g.gather32;
+    la.subv_rad = g.subv_rad;
+    for (int q = 0; q < 3; q++) {
+        la.box[q] = g.box[q];
+        la.est_mn[q] = g.est_mn[q]; la.est_mx[q] = g.est_mx[q]; la.est_cn[q] = g.est_cn[q];
+        la.act_mn[q] = g.act_mn[q]; la.act_mx[q] = g.act_mx[q]; la.act_cn[q] = g.act_cn[q];
+        la.off_min[q] = g.off_min[q]; la.off_max[q] = g.off_max[q];
+    }
+    la.sph = a.sph + 3 * g.sph_off;
+    la.rvals = a.rvals + g.rval_off;
+    la.nsamp_pad = 0;
+}
+
 // The reference sub-volumes of all searched points at once: they depend only on the
 // point, not on its neighbours, so they are taken out of the level chain.  Workgroup p
 // handles the point at position p of the order.
 template <int BPV, int B>
-__global__ void __launch_bounds__(B) reference_kernel(SearchArgs a) {
-    DVC_SHARED(PtShared, S);
-    DVC_SHARED(double[(B / 64) * 9], red);
+__device__ __forceinline__ void reference_body(const SearchArgs &a, int p, int n, PtShared &S, double *red) {
     const int tid = threadIdx.x;
-    const int p = blockIdx.x;
-    const int n = a.order[p];
     if (!point_setup<BPV, B>(a, n, S)) return;
     // each thread writes and re-reads only its own samples
     reference_pass<BPV, B>(a, S, a.rvals + (size_t)p * a.nsamp_pad, red);
@@ -1405,9 +1421,25 @@ __global__ void __launch_bounds__(B) reference_kernel(SearchArgs a) {
     }
 }
 
+template <int BPV, int B, bool UNEVEN>
+__global__ void __launch_bounds__(B) reference_kernel(SearchArgs a) {
+    DVC_SHARED(PtShared, S);
+    DVC_SHARED(double[(B / 64) * 9], red);
+    const int p = blockIdx.x;
+    const int n = a.order[p];
+    if constexpr (UNEVEN) {
+        DVC_SHARED(SearchArgs, la);
+        if (threadIdx.x == 0) apply_geo(la, a, n);
+        __syncthreads();
+        reference_body<BPV, B>(la, p, n, S, red);
+    } else {
+        reference_body<BPV, B>(a, p, n, S, red);
+    }
+}
+
 // One dependency level of the point loop: workgroup b searches point level_pts[b].
 // All neighbours a point depends on belong to earlier levels (earlier launches).
-template <int BPV, int B>
+template <int BPV, int B, bool UNEVEN>
 __global__ void __launch_bounds__(B) search_kernel(SearchArgs a, const int *level_pts) {
     DVC_DYN_LDS(float, dyn);
     // [MFMA staging tiles][reference samples, when not read from HBM]
@@ -1419,7 +1451,14 @@ __global__ void __launch_bounds__(B) search_kernel(SearchArgs a, const int *leve
     const int tid = threadIdx.x;
     const int n = level_pts[blockIdx.x];
     const int p = a.pos[n];
-    point_search<BPV, B>(a, p, n, S, rcb, stage, red, V);
+    if constexpr (UNEVEN) {
+        DVC_SHARED(SearchArgs, la);
+        if (tid == 0) apply_geo(la, a, n);
+        __syncthreads();
+        point_search<BPV, B>(la, p, n, S, rcb, stage, red, V);
+    } else {
+        point_search<BPV, B>(a, p, n, S, rcb, stage, red, V);
+    }
     __syncthreads();
     if (tid == 0) {
         dvc_point_result *r = &a.res[n];
@@ -1444,8 +1483,8 @@ size_t search_lds_bytes(int nsamp_pad, int block_threads, bool samples_in_lds) {
 
 template <int BPV, int B>
 static int launch_search_t(const SearchArgs &a, const int *level_pts, int grid, size_t lds, hipStream_t st) {
-    auto k = search_kernel<BPV, B>;
-    hipLaunchKernelGGL(k, dim3(grid), dim3(B), lds, st, a, level_pts);
+    if (a.pgeo) hipLaunchKernelGGL((search_kernel<BPV, B, true>), dim3(grid), dim3(B), lds, st, a, level_pts);
+    else hipLaunchKernelGGL((search_kernel<BPV, B, false>), dim3(grid), dim3(B), lds, st, a, level_pts);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -1454,7 +1493,8 @@ static int launch_search_t(const SearchArgs &a, const int *level_pts, int grid, 
 // (direct or replayed from a graph) never change function attributes.
 template <int BPV, int B>
 static int lds_limit_t(size_t lds_total, int *) {
-    const void *ks[2] = {(const void *)search_kernel<BPV, B>, (const void *)reference_kernel<BPV, B>};
+    const void *ks[4] = {(const void *)search_kernel<BPV, B, false>, (const void *)reference_kernel<BPV, B, false>,
+                         (const void *)search_kernel<BPV, B, true>, (const void *)reference_kernel<BPV, B, true>};
     for (const void *k : ks) {
         hipFuncAttributes fa;
         if (hipFuncGetAttributes(&fa, k) != hipSuccess) return -1;
@@ -1468,13 +1508,14 @@ static int lds_limit_t(size_t lds_total, int *) {
 template <int BPV, int B>
 static int launch_reference_t(const SearchArgs &a, size_t lds, hipStream_t st) {
     (void)lds;
-    hipLaunchKernelGGL((reference_kernel<BPV, B>), dim3(a.nproc), dim3(B), 0, st, a);
+    if (a.pgeo) hipLaunchKernelGGL((reference_kernel<BPV, B, true>), dim3(a.nproc), dim3(B), 0, st, a);
+    else hipLaunchKernelGGL((reference_kernel<BPV, B, false>), dim3(a.nproc), dim3(B), 0, st, a);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 template <int BPV, int B>
 static int occ_t(size_t lds, int *bpc) {
-    auto k = search_kernel<BPV, B>;
+    auto k = search_kernel<BPV, B, false>;
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(bpc, (const void *)k, B, lds) == hipSuccess ? 0 : -1;
 }
 
@@ -1500,7 +1541,8 @@ int launch_search(const SearchArgs &a, const int *level_pts, int block_threads, 
 template <int BPV, int B>
 static int static_lds_t(size_t *bytes, int *) {
     hipFuncAttributes fa;
-    if (hipFuncGetAttributes(&fa, (const void *)search_kernel<BPV, B>) != hipSuccess) return -1;
+    // the per-point-geometry instantiation holds one more shared SearchArgs: the larger
+    if (hipFuncGetAttributes(&fa, (const void *)search_kernel<BPV, B, true>) != hipSuccess) return -1;
     *bytes = fa.sharedSizeBytes;
     return 0;
 }

@@ -263,3 +263,38 @@ def config4_eval(ctx, size: int = 1024, npoints: int = 100_000, seed: int = 4, t
     case = DeviceCase("learned-eval-u8", shape, np.uint8, pts, truth, params, start, bumps, 0.0)
     case.waves = waves
     return case
+
+
+def config3(ctx, size: int = 512, npoints: int = 50_000, seed: int = 3, amplitude: float = 3.0,
+            wavelength: float = 128.0, margin: float = 48.0):
+    """Batched uneven workload made on the device: uint8 size^3 smoothed noise (sigma 2),
+    the target under three plane sinusoids (amplitude 3 voxels along x, y and z in turn,
+    wavelength 128, random directions and phases); `npoints` uniform random points inside
+    `margin`; every point its own subvolume: cube or sphere, size uniform in 21..61,
+    samples uniform in 1000..10000.  Returns (case, geom, size, nsamp); 6-DOF, disp_max 6,
+    processing order from the volume centre."""
+    rng = np.random.default_rng(seed)
+    shape = (size, size, size)
+    ctx.synth_noise(0, shape, np.uint8, 2.0, 3000 + seed)
+    waves = []
+    for ax in range(3):
+        k = rng.normal(size=3)
+        k = k / np.linalg.norm(k) * (2 * np.pi / wavelength)
+        a = np.zeros(3)
+        a[ax] = amplitude
+        waves.append(np.concatenate([k, [rng.uniform(0, 2 * np.pi)], a]))
+    waves = np.array(waves)
+    center = np.full(3, (size - 1) / 2.0)
+    ctx.synth_warp_field(np.zeros((0, 7)), 0.0, center, waves=waves)
+    pts = rng.uniform(margin, size - 1 - margin, (npoints, 3))
+    truth = bump_field(pts, np.zeros((0, 7)), 0.0, center, waves)
+    geom = rng.integers(0, 2, npoints).astype(np.int32)
+    sub = rng.integers(21, 62, npoints).astype(np.int32)
+    nsamp = rng.integers(1000, 10001, npoints).astype(np.int32)
+    start = np.full(3, size / 2.0)
+    u0 = bump_field(start[None], np.zeros((0, 7)), 0.0, center, waves)[0]
+    params = dict(ndof=6, obj="znssd", interp="tricubic", geom="cube", subvol_size=31, subvol_npts=2000,
+                  disp_max=6, basin_radius=0.0, rigid_trans=tuple(float(np.rint(v)) for v in u0))
+    case = DeviceCase("uneven-u8", shape, np.uint8, pts, truth, params, start, np.zeros((0, 7)), 0.0)
+    case.waves = waves
+    return case, geom, sub, nsamp

@@ -183,6 +183,15 @@ int dvc_search_prepare(dvc_ctx *ctx, const dvc_run_params *rc, const double *poi
 int dvc_search_execute(dvc_ctx *ctx);
 int dvc_search_fetch(dvc_ctx *ctx, dvc_point_result *results);
 
+/* Per-point sub-volume geometry for the next dvc_search_prepare (no reference
+ * counterpart: the reference has one RunControl, so one shape, size and sample count for
+ * all points).  Point i is searched as Search::process_point would with a RunControl
+ * whose subvol_geom, subvol_size and subvol_npts are geom[i], size[i], nsamp[i] and whose
+ * other fields are the prepare's run parameters; start guesses still come from the
+ * neighbours' results.  n = npts of the prepare; n = 0 (or NULL arrays) returns to one
+ * geometry for all.  Needs the HBM table of reference samples (their total under 48 GiB). */
+int dvc_set_point_geometry(dvc_ctx *ctx, const int32_t *geom, const int32_t *size, const int32_t *nsamp, int32_t n);
+
 /* Start modes of a prepared search.  The reference has one: the serial loop's start
  * guess (Search.cpp:277-377), DVC_START_CHAIN.  The other two have no counterpart in
  * the reference; they serve starting warps proposed for every point at once (e.g. by a

@@ -27,6 +27,8 @@
 #include <cstring>
 #include <limits>
 #include <random>
+#include <map>
+#include <tuple>
 #include <vector>
 #include <omp.h>
 
@@ -1189,6 +1191,51 @@ int orc_dvc_run(const orc_run *rc, const void *refv, const void *tarv, const dou
         }
     }
     auto t1 = std::chrono::steady_clock::now();
+    if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
+    return (int)N;
+}
+
+/* The serial loop with one RunControl per point (no reference counterpart): point n is
+ * searched by a Searcher of rc with subvol_geom/size/npts = geom[n], size[n], npp[n]
+ * (one Searcher per distinct triple); start guesses read the shared results as in
+ * orc_dvc_run. */
+int orc_dvc_run_geo(const orc_run *rc, const void *refv, const void *tarv, const double *p, int npts,
+                    const int *order, const int *neigh, int k, int nproc, int nthreads, const int *geom,
+                    const int *size, const int *npp, orc_result *results, double *seconds) {
+    for (int i = 0; i < npts; i++) {
+        results[i].status = -3; results[i].iters = 0; results[i].obj = 0.0;
+        for (int q = 0; q < 12; q++) results[i].par[q] = 0.0;
+    }
+    std::map<std::tuple<int, int, int>, Searcher *> searchers;
+    unsigned int N = (unsigned int)npts;
+    if ((unsigned int)nproc > 1 && (unsigned int)nproc < N) N = (unsigned int)nproc;
+    auto t0 = std::chrono::steady_clock::now();
+    std::vector<const Result *> nres;
+    for (unsigned int i = 0; i < N; i++) {
+        int n = order[i];
+        const auto key = std::make_tuple(geom[n], size[n], npp[n]);
+        auto it = searchers.find(key);
+        if (it == searchers.end()) {
+            orc_run r = *rc;
+            r.geom = geom[n]; r.subvol_size = size[n]; r.subvol_npts = npp[n];
+            it = searchers.emplace(key, new Searcher(r, make_vol(&r, refv), make_vol(&r, tarv), g_sbi(), nthreads)).first;
+        }
+        Searcher &S = *it->second;
+        nres.clear();
+        for (int j = 0; j < k; j++) nres.push_back((const Result *)&results[neigh[(size_t)n * k + j]]);
+        Pt sp(p[3 * n], p[3 * n + 1], p[3 * n + 2]);
+        int st = S.process_point(sp, nres);
+        results[n].iters = S.iters;
+        if (st == 0) {
+            results[n].status = 0;
+            results[n].obj = S.obj_min;
+            for (int q = 0; q < rc->ndof; q++) results[n].par[q] = S.par_min[q];
+        } else {
+            results[n].status = st;
+        }
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    for (auto &kv : searchers) delete kv.second;
     if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
     return (int)N;
 }

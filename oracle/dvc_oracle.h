@@ -81,6 +81,13 @@ int    orc_dvc_run(const orc_run *rc, const void *ref, const void *tar,
                    const double *pts, int npts, const int *order, const int *neigh, int k,
                    int nproc, int nthreads, orc_result *results, double *seconds);
 
+/* The serial loop with per-point subvolume geometry (no reference counterpart): point n
+ * uses rc with subvol_geom = geom[n], subvol_size = size[n], subvol_npts = npp[n]. */
+int    orc_dvc_run_geo(const orc_run *rc, const void *ref, const void *tar,
+                       const double *pts, int npts, const int *order, const int *neigh, int k,
+                       int nproc, int nthreads, const int *geom, const int *size, const int *npp,
+                       orc_result *results, double *seconds);
+
 /* Given-start passes (no reference counterpart): mode 1 searches every point of the
  * first nproc of order[] from start[12n..]; mode 2 adds the judge pass (the reference's
  * start guess over the first pass's results at earlier positions) and keeps the first

@@ -65,6 +65,7 @@ def load() -> C.CDLL:
         "orc_objective": (D, [I, P(D), P(D), I, P(D)]),
         "orc_sort_order_neighbors": (I, [P(D), I, P(D), P(I), P(I), I]),
         "orc_dvc_run": (I, [P(Run), VP, VP, P(D), I, P(I), P(I), I, I, I, VP, P(D)]),
+        "orc_dvc_run_geo": (I, [P(Run), VP, VP, P(D), I, P(I), P(I), I, I, I, P(I), P(I), P(I), VP, P(D)]),
         "orc_dvc_run_start": (I, [P(Run), VP, VP, P(D), I, P(I), P(I), I, I, I, I, P(D), VP, VP, VP, P(I), P(D)]),
         "orc_strain": (I, [P(D), P(D), P(I), P(D), P(I), I, I, I, D, I, I, P(D), P(I), P(D), P(D), P(D), P(D)]),
         "orc_colpiv_solve": (I, [P(D), P(D), I, P(D)]),
@@ -195,6 +196,25 @@ def dvc_run(run: Run, ref: np.ndarray, tar: np.ndarray, points, order, neigh, np
     n = load().orc_dvc_run(C.byref(run), ref.ctypes.data_as(C.c_void_p), tar.ctypes.data_as(C.c_void_p), _dp(p),
                            len(p), _ip(o), _ip(nb), nb.shape[1], nproc, nthreads,
                            res.ctypes.data_as(C.c_void_p), C.byref(sec))
+    return res, sec.value, n
+
+
+def dvc_run_geo(run: Run, ref: np.ndarray, tar: np.ndarray, points, order, neigh, geom, size, npp, nproc=0,
+                nthreads=1):
+    """The serial loop with per-point subvolume geometry; returns (results, seconds, n)."""
+    p = np.ascontiguousarray(points, dtype=np.float64)
+    o = np.ascontiguousarray(order, dtype=np.int32)
+    nb = np.ascontiguousarray(neigh, dtype=np.int32)
+    g = np.ascontiguousarray([GEOM[x] if isinstance(x, str) else int(x) for x in geom], dtype=np.int32)
+    sz = np.ascontiguousarray(size, dtype=np.int32)
+    m = np.ascontiguousarray(npp, dtype=np.int32)
+    ref = np.ascontiguousarray(ref)
+    tar = np.ascontiguousarray(tar)
+    res = np.zeros(len(p), dtype=RESULT_DTYPE)
+    sec = C.c_double()
+    n = load().orc_dvc_run_geo(C.byref(run), ref.ctypes.data_as(C.c_void_p), tar.ctypes.data_as(C.c_void_p), _dp(p),
+                               len(p), _ip(o), _ip(nb), nb.shape[1], nproc, nthreads, _ip(g), _ip(sz), _ip(m),
+                               res.ctypes.data_as(C.c_void_p), C.byref(sec))
     return res, sec.value, n
 
 

@@ -16,7 +16,7 @@ static void run_search(const SearchArgs &a, size_t lds, const std::vector<int> &
         emu::grid_dim.x = cnt;
         for (int b0 = 0; b0 < cnt; b0 += 8) {
             int nb = cnt - b0 < 8 ? cnt - b0 : 8;
-            emu::run_grid(b0, nb, B, lds, [&] { search_kernel<BPV, B>(a, pts); });
+            emu::run_grid(b0, nb, B, lds, [&] { search_kernel<BPV, B, false>(a, pts); });
         }
     }
 }
@@ -63,10 +63,10 @@ extern "C" int emu_search(const dvc_run_params *rc, const void *ref, const void 
         emu::grid_dim.x = N;
         for (int b0 = 0; b0 < N; b0 += 8) {
             int nb = N - b0 < 8 ? N - b0 : 8;
-            if (bpv == 1 && block_threads == 64) emu::run_grid(b0, nb, 64, 0, [&] { reference_kernel<1, 64>(a); });
-            else if (bpv == 1 && block_threads == 128) emu::run_grid(b0, nb, 128, 0, [&] { reference_kernel<1, 128>(a); });
-            else if (bpv == 2 && block_threads == 64) emu::run_grid(b0, nb, 64, 0, [&] { reference_kernel<2, 64>(a); });
-            else if (bpv == 2 && block_threads == 128) emu::run_grid(b0, nb, 128, 0, [&] { reference_kernel<2, 128>(a); });
+            if (bpv == 1 && block_threads == 64) emu::run_grid(b0, nb, 64, 0, [&] { reference_kernel<1, 64, false>(a); });
+            else if (bpv == 1 && block_threads == 128) emu::run_grid(b0, nb, 128, 0, [&] { reference_kernel<1, 128, false>(a); });
+            else if (bpv == 2 && block_threads == 64) emu::run_grid(b0, nb, 64, 0, [&] { reference_kernel<2, 64, false>(a); });
+            else if (bpv == 2 && block_threads == 128) emu::run_grid(b0, nb, 128, 0, [&] { reference_kernel<2, 128, false>(a); });
             else return -1;
         }
     }

@@ -111,3 +111,38 @@ def test_unsearched_points_are_marked(oracle, case):
     rest = order[5:]
     assert (kept[rest] == -1).all() and (res["status"][rest] == -3).all()
     assert (res["status"][order[:5]] == 0).all()
+
+
+def test_per_point_geometry_reduces_to_serial_loop(oracle, case):
+    # every point given the run's own geometry: the per-point-geometry loop is the loop
+    run = _run(oracle, case)
+    order, neigh = oracle.sort_order_neighbors(case.points)
+    chain, _, _ = oracle.dvc_run(run, case.ref, case.tar, case.points, order, neigh)
+    n = len(case.points)
+    p = case.params
+    res, _, _ = oracle.dvc_run_geo(run, case.ref, case.tar, case.points, order, neigh, [p["geom"]] * n,
+                                   [p["subvol_size"]] * n, [p["subvol_npts"]] * n)
+    for f in ("status", "iters", "obj", "par"):
+        assert np.array_equal(res[f], chain[f]), f
+
+
+def test_per_point_geometry_mixes_shapes(oracle, case):
+    # mixed cubes and spheres of several sizes: each point matches the plain loop run
+    # with its own geometry and the same start guess (its neighbours' results)
+    run = _run(oracle, case)
+    order, neigh = oracle.sort_order_neighbors(case.points)
+    rng = np.random.default_rng(16)
+    n = len(case.points)
+    geom = rng.integers(0, 2, n)
+    size = rng.integers(11, 22, n)
+    npp = rng.integers(300, 1200, n)
+    res, _, _ = oracle.dvc_run_geo(run, case.ref, case.tar, case.points, order, neigh, geom, size, npp)
+    assert (res["status"] == 0).all()
+    assert np.abs(res["par"][:, :3] - case.truth).max() < 0.06
+    # the first point in the order has no searched neighbours: a one-point loop with its geometry
+    first = order[0]
+    r1 = _run(oracle, case, geom=["cube", "sphere"][geom[first]], subvol_size=int(size[first]),
+              subvol_npts=int(npp[first]))
+    single, _, _ = oracle.dvc_run(r1, case.ref, case.tar, case.points, order, neigh, nproc=1)
+    assert single["status"][first] == res["status"][first]
+    assert np.array_equal(single["par"][first], res["par"][first])
