@@ -924,22 +924,65 @@ __device__ __forceinline__ double shfl_from(double v, int src) { return __shfl(v
 __device__ __forceinline__ int shfl_from(int v, int src) { return __shfl(v, src, 64); }
 #endif
 
-// One Gauss-Newton step on wavefront 0 (all 64 lanes call it): the normal equations
-// J^T J dx = -J^T e of the objective's residual, assembled from the moments in V
-// (lane j < N builds column j of J^T J, lane N builds -J^T e), then solved by
-// column-pivoted Householder QR (Eigen 3.4 ColPivHouseholderQR::computeInPlace and
-// _solve_impl, as used at Search.cpp:234).  Lane j keeps column j in registers; the
-// pivot scan, the reflector and the back substitution are uniform across lanes, and
-// every column is updated by its own lane with the same operation order as a serial
-// loop.  Adds the step to S.par and stores it in S.upd (Search.cpp:236-240).
+// Cholesky solve of the symmetric normal equations held one column per lane (lane j < N:
+// column j of A, lane N: right-hand side b), on wavefront 0.  Step k takes the pivot
+// A_kk and the column L_ik (i > k) of the updated trailing matrix from lane k, both
+// uniform; lane j > k applies the rank-1 update A_ij -= L_ik L_jk to its column, and
+// lane N runs the forward substitution L y = b alongside.  Only the lower triangle is
+// read.  The back substitution L^T x = y is uniform on every lane.  Returns false,
+// having written nothing, when a pivot is not above 1e-9 of the largest diagonal entry
+// (or is not finite): the caller then solves by column-pivoted QR, whose rank rule
+// covers the (near-)singular systems.
 template <int N>
-__device__ __forceinline__ void gn_step(const SearchArgs &a, PtShared &S, const double *V) {
+__device__ __forceinline__ bool chol_solve(double (&col)[N], int lane, double (&x)[N]) {
+    double dmax = 0.0;
+#pragma unroll
+    for (int k = 0; k < N; k++) dmax = fmax(dmax, rdlane(col[k], k));
+    const double tol = 1e-9 * dmax;
+    double lkk[N];
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+        const double d = rdlane(col[k], k);
+        if (!(d > tol) || !(d < 1.0e300)) return false;
+        const double l = sqrt(d), r = 1.0 / l;
+        lkk[k] = l;
+        double lik[N];
+        double ljk = (lane == N) ? col[k] * r : 0.0;   // lane j > k: L_jk; lane N: y_k
+#pragma unroll
+        for (int i = k + 1; i < N; i++) {
+            lik[i] = rdlane(col[i], k) * r;
+            if (lane == i) ljk = lik[i];
+        }
+        if (lane > k && lane <= N)
+#pragma unroll
+            for (int i = k + 1; i < N; i++) col[i] -= lik[i] * ljk;
+        if (lane == N) col[k] = ljk;                    // y_k
+        if (lane == k) {                                // column k of L
+            col[k] = l;
+#pragma unroll
+            for (int i = k + 1; i < N; i++) col[i] = lik[i];
+        }
+    }
+#pragma unroll
+    for (int i = N - 1; i >= 0; i--) {
+        double sacc = rdlane(col[i], N);
+#pragma unroll
+        for (int q = i + 1; q < N; q++) sacc -= rdlane(col[q], i) * x[q];
+        x[i] = sacc / lkk[i];
+    }
+    return true;
+}
+
+// Column `lane` (lane < N) of the objective's Gauss-Newton normal matrix J^T J, or for
+// lane N the right-hand side -J^T e, from the block's moments V (Search.cpp:234's
+// system, as LM_prep_at assembles it); other lanes get column 0.
+template <int N>
+__device__ __forceinline__ void normal_cols(const SearchArgs &a, const PtShared &S, const double *V, int lane,
+                                            double (&col)[N]) {
 #pragma clang fp contract(off)
-    const int lane = threadIdx.x & 63;
     const double n = (double)a.nsamp;
     const int kind = a.obj;
     const int j = lane < N ? lane : 0;      // column of this lane (lane N: right-hand side)
-    double col[N];
     {
         const double Sgj = V[j * 16 + 14];
         if (kind == DVC_OBJ_SAD || kind == DVC_OBJ_SSD) {
@@ -989,6 +1032,35 @@ __device__ __forceinline__ void gn_step(const SearchArgs &a, PtShared &S, const 
                     col[i] = -(-V[i * 16 + 13] / (Nn * R) + ci * S.Str / (N2 * Nn * R));
                 }
         }
+    }
+}
+
+// One Gauss-Newton step on wavefront 0 (all 64 lanes call it): the normal equations
+// J^T J dx = -J^T e of the objective's residual, assembled from the moments in V
+// (lane j < N builds column j of J^T J, lane N builds -J^T e), then solved by
+// column-pivoted Householder QR (Eigen 3.4 ColPivHouseholderQR::computeInPlace and
+// _solve_impl, as used at Search.cpp:234).  Lane j keeps column j in registers; the
+// pivot scan, the reflector and the back substitution are uniform across lanes, and
+// every column is updated by its own lane with the same operation order as a serial
+// loop.  Adds the step to S.par and stores it in S.upd (Search.cpp:236-240).
+template <int N, bool CHOL = true>
+__device__ __forceinline__ void gn_step(const SearchArgs &a, PtShared &S, const double *V) {
+#pragma clang fp contract(off)
+    const int lane = threadIdx.x & 63;
+    double col[N];
+    normal_cols<N>(a, S, V, lane, col);
+    if (CHOL) {
+        double x[N];
+        if (chol_solve<N>(col, lane, x)) {
+#pragma unroll
+            for (int i = 0; i < N; i++)
+                if (lane == i) {
+                    S.upd[i] = x[i];
+                    S.par[i] += x[i];
+                }
+            return;
+        }
+        normal_cols<N>(a, S, V, lane, col);       // the factorisation overwrote them
     }
     // column norms (exact and updated), the permutation carried with the columns
     double cn = 0.0;

@@ -53,15 +53,17 @@ def search(params, ref, tar, points, order, neigh, nproc=0, threads=64, gather32
     return out
 
 
-def gn_solve(A, b):
-    """The search kernel's Gauss-Newton solve (gn_step) of A x = b on one emulated wave."""
+def gn_solve(A, b, chol=False):
+    """The search kernel's Gauss-Newton solve (gn_step) of A x = b on one emulated wave:
+    the column-pivoted QR alone (chol False), or the Cholesky solve with its QR fallback
+    (chol True, the search kernel's own choice; A symmetric)."""
     lib = load()
     A = np.ascontiguousarray(A, dtype=np.float64)
     b = np.ascontiguousarray(b, dtype=np.float64)
     x = np.zeros(len(b))
     vp = lambda a: a.ctypes.data_as(C.c_void_p)
-    lib.emu_gn_solve.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
-    if lib.emu_gn_solve(len(b), vp(A), vp(b), vp(x)) != 0:
+    lib.emu_gn_solve.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
+    if lib.emu_gn_solve(len(b), vp(A), vp(b), vp(x), int(chol)) != 0:
         raise ValueError("n must be 3, 6 or 12")
     return x
 

@@ -89,7 +89,7 @@ extern "C" int emu_search(const dvc_run_params *rc, const void *ref, const void 
 
 // gn_step<N> alone on one wave: with the SSD moments V[i][j] = A[i][j], V[i][12] = -b[i],
 // V[i][13] = 0 it solves A x = b by the column-pivoted QR of the search kernel.
-extern "C" int emu_gn_solve(int n, const double *A, const double *b, double *x) {
+extern "C" int emu_gn_solve(int n, const double *A, const double *b, double *x, int chol) {
     if (n != 3 && n != 6 && n != 12) return -1;
     SearchArgs a;
     std::memset(&a, 0, sizeof(a));
@@ -106,9 +106,15 @@ extern "C" int emu_gn_solve(int n, const double *A, const double *b, double *x) 
         if (threadIdx.x == 0)
             for (int q = 0; q < 12; q++) { S.par[q] = 0.0; S.upd[q] = 0.0; }
         emu::wave_sync();
-        if (n == 3) gn_step<3>(a, S, V.data());
-        else if (n == 6) gn_step<6>(a, S, V.data());
-        else gn_step<12>(a, S, V.data());
+        if (chol) {
+            if (n == 3) gn_step<3, true>(a, S, V.data());
+            else if (n == 6) gn_step<6, true>(a, S, V.data());
+            else gn_step<12, true>(a, S, V.data());
+        } else {
+            if (n == 3) gn_step<3, false>(a, S, V.data());
+            else if (n == 6) gn_step<6, false>(a, S, V.data());
+            else gn_step<12, false>(a, S, V.data());
+        }
         emu::wave_sync();
         if (threadIdx.x == 0)
             for (int q = 0; q < n; q++) x[q] = S.par[q];

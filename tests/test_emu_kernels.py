@@ -92,6 +92,41 @@ def test_gn_solve_matches_oracle_bitwise(oracle, n):
         assert np.array_equal(got[ok], want[ok]), (A, got, want)
 
 
+@pytest.mark.parametrize("n", [3, 6, 12])
+def test_gn_solve_cholesky(oracle, n):
+    """The search kernel's own solve: Cholesky of the symmetric normal matrix, with the
+    column-pivoted QR taking over when a pivot is not above 1e-9 of the largest diagonal.
+    Well-conditioned and badly scaled systems agree with an LU solve to the conditioning
+    bound; singular, near-singular and all-zero systems give the QR's bits."""
+    rng = np.random.default_rng(70 + n)
+    for scale in (np.ones(n), np.logspace(0, 3, n)):
+        J = rng.normal(size=(40, n)) * scale
+        A = J.T @ J
+        b = rng.normal(size=n)
+        got = emu.gn_solve(A, b, chol=True)
+        want = np.linalg.solve(A, b)
+        assert np.linalg.norm(got - want) <= 1e-13 * np.linalg.cond(A) * np.linalg.norm(want)
+    singular = []
+    J = rng.normal(size=(40, n))
+    J[:, n // 2] = J[:, 0]
+    singular.append(J.T @ J)                        # duplicate column
+    J = rng.normal(size=(40, n))
+    J[:, -1] *= 1e-6
+    singular.append(J.T @ J)                        # pivot ratio ~1e-12
+    Z = J.T @ J
+    Z[:, -1] = 0.0
+    Z[-1, :] = 0.0
+    singular.append(Z)
+    singular.append(np.zeros((n, n)))
+    for A in singular:
+        b = rng.normal(size=n)
+        want = oracle.colpiv_solve(A, b)
+        got = emu.gn_solve(A, b, chol=True)
+        assert np.array_equal(np.isnan(want), np.isnan(got))
+        ok = ~np.isnan(want)
+        assert np.array_equal(got[ok], want[ok]), (A, got, want)
+
+
 def test_strain_kernel(oracle):
     pts = synth.grid_points(5, 9.0, (40.0, 50.0, 60.0))
     rng = np.random.default_rng(3)
