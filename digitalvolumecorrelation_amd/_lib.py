@@ -76,6 +76,7 @@ def load() -> C.CDLL:
         "dvc_search_execute": (I, [VP]),
         "dvc_search_fetch": (I, [VP, VP]),
         "dvc_set_block_threads": (I, [VP, C.c_int]),
+        "dvc_set_round_capacity": (I, [VP, I]),
         "dvc_set_launch_mode": (I, [VP, C.c_int]),
         "dvc_search_info": (I, [VP, C.c_void_p, C.c_void_p, C.c_void_p]),
         "dvc_set_start_mode": (I, [VP, C.c_int]),
@@ -288,6 +289,11 @@ class Context:
         self._check(self.lib.dvc_search_autotune(self.h, reps, C.byref(bt), C.byref(ms)))
         return bt.value, ms.value
 
+    def set_round_capacity(self, points: int):
+        """Points per chain launch for the next prepare: 0 automatic (one round of the
+        chip), -1 dependency levels only, else that many."""
+        self._check(self.lib.dvc_set_round_capacity(self.h, int(points)))
+
     def set_point_geometry(self, geom=None, size=None, nsamp=None):
         """Per-point subvolume shape (0 cube, 1 sphere or names), size and sample count for
         the next prepare; no arguments: one geometry for all points."""
@@ -331,7 +337,8 @@ class Context:
         return [x.value for x in v]
 
     def levels(self) -> int:
-        """Dependency levels of the prepared search (one kernel launch each)."""
+        """Chain launches of the prepared search: dependency levels, or launches of at most
+        the round capacity when a level is wider."""
         return self._info()[0]
 
     def block_threads(self) -> int:

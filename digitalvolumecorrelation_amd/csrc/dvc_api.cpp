@@ -54,6 +54,7 @@ struct dvc_ctx {
     dvc_point_result *d_res = nullptr;
     double off_min[3]{}, off_max[3]{};
     int chosen_threads = 256;
+    int round_cap = 0;               // points per chain launch: 0 automatic, -1 levels only
     std::vector<char> searched;
     std::vector<int> level_off;      // dependency levels (dvc_geometry.h)
     float *d_rvals = nullptr;        // reference samples of every searched point (or null)
@@ -259,6 +260,12 @@ int dvc_last_kernel_ms(dvc_ctx *ctx, float *ms) {
         ctx->timing_pending = false;
     }
     *ms = ctx->last_ms;
+    return DVC_OK;
+}
+
+int dvc_set_round_capacity(dvc_ctx *ctx, int32_t points) {
+    if (!ctx || points < -1) return DVC_ERR_ARG;
+    ctx->round_cap = points;
     return DVC_OK;
 }
 
@@ -534,6 +541,12 @@ int dvc_search_prepare(dvc_ctx *ctx, const dvc_run_params *rc, const double *poi
     const int mode = ctx->start_mode;
     if (mode == DVC_START_CHAIN) {
         dependency_levels(order, pos, neigh, npts, k, N, level_pts, level_off);
+        // levels wider than the chip holds at once: launches of at most one round,
+        // filled with later points whose neighbours are done
+        const int cap = ctx->round_cap > 0 ? ctx->round_cap : ctx->cus * std::max(1, 512 / bt);
+        int widest = 0;
+        for (size_t L = 0; L + 1 < level_off.size(); L++) widest = std::max(widest, level_off[L + 1] - level_off[L]);
+        if (ctx->round_cap >= 0 && widest > cap) capacity_schedule(order, pos, neigh, k, N, cap, level_pts, level_off);
     } else {
         // no point reads a result of the same pass: one launch over the whole order
         level_pts.assign(order, order + N);

@@ -191,4 +191,45 @@ inline int dependency_levels(const int *order, const std::vector<int> &pos, cons
     return depth;
 }
 
+// Launch schedule of the serial chain for a chip that runs `cap` points at once.  In
+// processing order, each point goes into the first launch after every launch that holds
+// one of its earlier neighbours and that still has room (first fit).  As with
+// dependency levels, a point's start guess reads only results of earlier launches, so
+// the results are the same.  A level wider than `cap` then needs no partly empty launch
+// of its own: later points whose neighbours are done fill it.  Same output format as
+// dependency_levels; returns the number of launches.
+inline int capacity_schedule(const int *order, const std::vector<int> &pos, const int *neigh, int k, int N, int cap,
+                             std::vector<int> &level_pts, std::vector<int> &level_off) {
+    std::vector<int> at(N, 0);          // launch of the point at position i
+    std::vector<int> cnt;               // points per launch
+    std::vector<int> nxt;               // union-find: first launch >= t with room
+    auto find = [&](int t) {
+        int r = t;
+        while (r < (int)nxt.size() && nxt[r] != r) r = nxt[r];
+        while (t < (int)nxt.size() && nxt[t] != t) { const int u = nxt[t]; nxt[t] = r; t = u; }
+        return r;
+    };
+    for (int i = 0; i < N; i++) {
+        const int n = order[i];
+        int e = 0;
+        for (int q = 0; q < k; q++) {
+            const int m = neigh[(size_t)n * k + q];
+            const int pm = pos[m];
+            if (pm < i && at[pm] + 1 > e) e = at[pm] + 1;
+        }
+        while ((int)cnt.size() <= e) { nxt.push_back((int)cnt.size()); cnt.push_back(0); }
+        const int t = find(e);
+        if (t == (int)cnt.size()) { nxt.push_back(t); cnt.push_back(0); }
+        at[i] = t;
+        if (++cnt[t] == cap) nxt[t] = t + 1;
+    }
+    const int depth = (int)cnt.size();
+    level_off.assign(depth + 1, 0);
+    for (int t = 0; t < depth; t++) level_off[t + 1] = level_off[t] + cnt[t];
+    std::vector<int> fill(level_off.begin(), level_off.end() - 1);
+    level_pts.assign(N, 0);
+    for (int i = 0; i < N; i++) level_pts[fill[at[i]]++] = order[i];
+    return depth;
+}
+
 }  // namespace dvc

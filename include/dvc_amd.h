@@ -234,8 +234,18 @@ int dvc_search_autotune(dvc_ctx *ctx, int reps, int32_t *best_threads, float *be
 /* 1 (default): the level launches of a prepared search are captured into a HIP graph
  * once and replayed by dvc_search_execute; 0: launched one by one. */
 int dvc_set_launch_mode(dvc_ctx *ctx, int graph);
-/* Shape of the prepared search: dependency levels (one kernel launch each), threads
- * per point, points searched.  Any pointer may be NULL. */
+/* Launches of the serial chain (no reference counterpart), applied by the next
+ * dvc_search_prepare.  The chain runs as one launch per dependency level: a level's
+ * points read only results of earlier levels.  When a level is wider than the chip runs
+ * at once (`points` per launch; 0 = automatic: CUs x workgroups of the chosen size per
+ * CU), the chain instead runs in launches of at most that many points: in processing
+ * order, each point joins the first launch after those holding its earlier neighbours
+ * that has room.  Each point still reads only finished results, so the results are
+ * identical; partly empty rounds at the ends of wide levels fill with later points.
+ * -1: dependency levels only. */
+int dvc_set_round_capacity(dvc_ctx *ctx, int32_t points);
+/* Shape of the prepared search: chain launches (dependency levels, or launches of the
+ * round capacity), threads per point, points searched.  Any pointer may be NULL. */
 int dvc_search_info(dvc_ctx *ctx, int32_t *levels, int32_t *threads_per_point, int32_t *points_searched);
 
 /* -------------------------------------------------------- learned proposal */

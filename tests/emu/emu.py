@@ -83,3 +83,19 @@ def strain(pts, disp, status, objmin, neigh, ndp=25, objt=1.0, refill=False, pla
     lib.emu_strain(vp(pts), vp(disp), vp(st), vp(om), vp(nb), k, n, ndp, float(objt), int(refill), *planar,
                    vp(o["disp"]), vp(o["npts"]), vp(o["swr"]), vp(o["E"]), vp(o["L"]), vp(o["DG"]))
     return o
+
+
+def schedule(order, neigh, cap=0):
+    """The chain's launches: dependency levels (cap 0) or launches of at most cap points;
+    a list of point-index arrays."""
+    lib = load()
+    o = np.ascontiguousarray(order, dtype=np.int32)
+    nb = np.ascontiguousarray(neigh, dtype=np.int32)
+    n = len(o)
+    lp = np.zeros(n, np.int32)
+    lo = np.zeros(n + 1, np.int32)
+    vp = lambda a: a.ctypes.data_as(C.c_void_p)
+    lib.emu_schedule.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
+    d = lib.emu_schedule(vp(o), n, vp(nb), nb.shape[1], int(cap), vp(lp), vp(lo))
+    return [lp[lo[t]:lo[t + 1]].copy() for t in range(d)]
+

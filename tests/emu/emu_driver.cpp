@@ -88,7 +88,8 @@ extern "C" int emu_search(const dvc_run_params *rc, const void *ref, const void 
 }
 
 // gn_step<N> alone on one wave: with the SSD moments V[i][j] = A[i][j], V[i][12] = -b[i],
-// V[i][13] = 0 it solves A x = b by the column-pivoted QR of the search kernel.
+// V[i][13] = 0 it solves A x = b, by the column-pivoted QR of the search kernel (chol 0)
+// or by its Cholesky solve with the QR fallback (chol 1).
 extern "C" int emu_gn_solve(int n, const double *A, const double *b, double *x, int chol) {
     if (n != 3 && n != 6 && n != 12) return -1;
     SearchArgs a;
@@ -136,3 +137,18 @@ extern "C" int emu_strain(const double *pts, const double *disp, const int *stat
     for (int b = 0; b < blocks; b++) emu::run_grid(b, 1, 64 * kStrainWaves, 0, [&] { strain_kernel(a); });
     return 0;
 }
+
+// The chain's launch plans (dvc_geometry.h): dependency levels (cap <= 0) or launches of
+// at most cap points.  level_pts[N], level_off[N + 1]; returns the number of launches.
+extern "C" int emu_schedule(const int *order, int N, const int *neigh, int k, int cap, int *level_pts,
+                            int *level_off) {
+    std::vector<int> pos(N);
+    for (int i = 0; i < N; i++) pos[order[i]] = i;
+    std::vector<int> lp, lo;
+    const int d = cap > 0 ? capacity_schedule(order, pos, neigh, k, N, cap, lp, lo)
+                          : dependency_levels(order, pos, neigh, N, k, N, lp, lo);
+    std::copy(lp.begin(), lp.end(), level_pts);
+    std::copy(lo.begin(), lo.end(), level_off);
+    return d;
+}
+

@@ -127,6 +127,33 @@ def test_gn_solve_cholesky(oracle, n):
         assert np.array_equal(got[ok], want[ok]), (A, got, want)
 
 
+@pytest.mark.parametrize("cap", [1, 7, 40])
+def test_capacity_schedule(cap):
+    """Launches of at most `cap` points: every point once, and every neighbour earlier in
+    the processing order in an earlier launch (so each start guess reads finished
+    results); never more launches than the rounds of the dependency levels, and with
+    cap 1 exactly the processing order."""
+    rng = np.random.default_rng(cap)
+    pts = rng.uniform(0, 100, (3000, 3))
+    order, neigh = D.knn_order_neighbors(pts, start=np.full(3, 50.0))
+    pos = np.empty(len(order), np.int64)
+    pos[order] = np.arange(len(order))
+    launches = emu.schedule(order, neigh, cap)
+    levels = emu.schedule(order, neigh, 0)
+    at = np.full(len(order), -1)
+    for t, L in enumerate(launches):
+        assert 0 < len(L) <= cap
+        assert (at[L] == -1).all()
+        at[L] = t
+    assert (at >= 0).all()
+    earlier = pos[neigh] < pos[:, None]
+    assert (at[neigh][earlier] < np.broadcast_to(at[:, None], neigh.shape)[earlier]).all()
+    assert len(launches) <= sum(-(-len(L) // cap) for L in levels)
+    assert len(launches) >= max(len(levels), -(-len(order) // cap))
+    if cap == 1:
+        assert np.array_equal(np.concatenate(launches), order)
+
+
 def test_strain_kernel(oracle):
     pts = synth.grid_points(5, 9.0, (40.0, 50.0, 60.0))
     rng = np.random.default_rng(3)

@@ -152,3 +152,31 @@ def test_autotune_keeps_parity(ctx, oracle, case3):
         _close(ctx.fetch(), g, "given after autotune")
     finally:
         ctx.set_start_mode(D.START_CHAIN)
+
+
+def test_round_capacity_chain_is_identical(ctx):
+    # launches of at most 3 points instead of dependency levels (216 points, 6 nearest
+    # neighbours: 22 levels of up to 22 points): every point reads the same finished
+    # neighbour results, so every bit of every result is the same
+    case = synth.config1(size=96, n=6, spacing=9)
+    order, neigh = D.knn_order_neighbors(case.points, start=case.start, k=6)
+    ctx.upload(0, case.ref)
+    ctx.upload(1, case.tar)
+    rp = D.make_params(**case.params)
+    out = {}
+    try:
+        for cap in (-1, 3, 0):
+            ctx.set_round_capacity(cap)
+            ctx.prepare(rp, case.points, order, neigh)
+            ctx.execute()
+            out[cap] = (ctx.levels(), ctx.fetch())
+    finally:
+        ctx.set_round_capacity(0)
+    levels, ref = out[-1]
+    print(f"  {levels} levels, {out[3][0]} launches of at most 3 points")
+    assert out[3][0] > levels and out[0][0] == levels
+    assert (ref["status"] == 0).all()
+    for cap in (3, 0):
+        for f in ("status", "iters", "obj", "par"):
+            assert np.array_equal(out[cap][1][f], ref[f]), (cap, f)
+
