@@ -336,7 +336,7 @@ def full_scale(a, D, synth, workload):
     ctx.prepare(params, case.points, order, neigh)
     t_prep = time.time() - t1
     levels = ctx.levels()
-    log(f"prepared in {t_prep:.1f} s: {levels} dependency levels")
+    log(f"prepared in {t_prep:.1f} s: {levels} chain launches")
     ctx.execute()
     ms_first = ctx.kernel_ms()
     dev = []

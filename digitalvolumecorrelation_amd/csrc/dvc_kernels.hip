@@ -592,8 +592,28 @@ __device__ __forceinline__ bool range_ok(const SearchArgs &a, const PtShared &S,
 
 // Rotation matrix of SearchParams::set_param_vect (SearchParams.cpp:77-87) and its
 // derivatives with respect to (phi, the, psi).
-__device__ void rotation(double phi, double the, double psi, double A[9], double dA[3][9]) {
-    double cf = cos(phi), sf = sin(phi), ct = cos(the), st = sin(the), cp = cos(psi), sp = sin(psi);
+// Cross-lane moves of one wavefront.  rdlane: value of lane l (l uniform);
+// shfl_from: value of lane src (src per lane).
+#ifdef DVC_EMU
+__device__ __forceinline__ double rdlane(double v, int l) { return emu::shfl_idx(v, l); }
+__device__ __forceinline__ double shfl_from(double v, int src) { return emu::shfl_idx(v, src); }
+__device__ __forceinline__ int shfl_from(int v, int src) { return (int)emu::shfl_idx((double)v, src); }
+__device__ __forceinline__ int rdlane_i(int v, int l) { return (int)emu::shfl_idx((double)v, l); }
+#else
+__device__ __forceinline__ int rdlane_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ __forceinline__ double rdlane(double v, int l) {
+    long long b = __double_as_longlong(v);
+    int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
+    int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+__device__ __forceinline__ double shfl_from(double v, int src) { return __shfl(v, src, 64); }
+__device__ __forceinline__ int shfl_from(int v, int src) { return __shfl(v, src, 64); }
+#endif
+
+// trig: cos/sin of phi, the, psi (cf sf ct st cp sp)
+__device__ void rotation(const double trig[6], double A[9], double dA[3][9]) {
+    const double cf = trig[0], sf = trig[1], ct = trig[2], st = trig[3], cp = trig[4], sp = trig[5];
     A[0] = ct * cf; A[1] = ct * sf; A[2] = -st;
     A[3] = sp * st * cf - cp * sf; A[4] = sp * st * sf + cp * cf; A[5] = ct * sp;
     A[6] = cp * st * cf + sp * sf; A[7] = cp * st * sf - sp * cf; A[8] = ct * cp;
@@ -611,8 +631,9 @@ __device__ void rotation(double phi, double the, double psi, double A[9], double
     Q[6] = -sp * st * cf + cp * sf; Q[7] = -sp * st * sf - cp * cf; Q[8] = -ct * sp;
 }
 
-// Warp state for parameter vector evalpar (thread 0).  SearchParams.cpp:53-107.
-__device__ __forceinline__ void set_warp(PtShared &S, int ndof) {
+// Warp state for parameter vector evalpar (thread 0), given the cosines and sines of
+// its three angles (trig).  SearchParams.cpp:53-107.
+__device__ __forceinline__ void set_warp(PtShared &S, int ndof, const double trig[6]) {
 #pragma clang fp contract(off)
     for (int q = 0; q < 3; q++) {
         S.u[q] = S.evalpar[q];
@@ -620,7 +641,7 @@ __device__ __forceinline__ void set_warp(PtShared &S, int ndof) {
     }
     double dA[3][9];
     if (ndof > 3) {
-        rotation(S.evalpar[3], S.evalpar[4], S.evalpar[5], S.A, dA);
+        rotation(trig, S.A, dA);
     } else {
         for (int q = 0; q < 9; q++) S.A[q] = (q % 4 == 0) ? 1.0 : 0.0;
         for (int r = 0; r < 3; r++) for (int q = 0; q < 9; q++) dA[r][q] = 0.0;
@@ -654,8 +675,15 @@ __device__ __forceinline__ bool evaluate(const SearchArgs &a, PtShared &S, int n
     constexpr int NW = B / 64;
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int n = a.nsamp;
+    if (tid < 64) {
+        // cosine and sine of the three angles, one angle per lane of wavefront 0
+        double sv = 0.0, cv = 1.0;
+        if (ndof > 3 && lane < 3) sincos(S.evalpar[3 + lane], &sv, &cv);
+        const double trig[6] = {rdlane(cv, 0), rdlane(sv, 0), rdlane(cv, 1), rdlane(sv, 1), rdlane(cv, 2),
+                                rdlane(sv, 2)};
+        if (tid == 0) set_warp(S, ndof, trig);
+    }
     if (tid == 0) {
-        set_warp(S, ndof);
         // non-finite parameters (singular normal matrix) are a range failure; a flag of
         // its own, so that thread 0 writing S.fail below cannot race with this read
         S.pfail = 0;
@@ -905,24 +933,6 @@ __device__ __forceinline__ bool evaluate(const SearchArgs &a, PtShared &S, int n
 
 // ------------------------------------------------------------- step solve
 
-// Cross-lane moves of one wavefront.  rdlane: value of lane l (l uniform);
-// shfl_from: value of lane src (src per lane).
-#ifdef DVC_EMU
-__device__ __forceinline__ double rdlane(double v, int l) { return emu::shfl_idx(v, l); }
-__device__ __forceinline__ double shfl_from(double v, int src) { return emu::shfl_idx(v, src); }
-__device__ __forceinline__ int shfl_from(int v, int src) { return (int)emu::shfl_idx((double)v, src); }
-__device__ __forceinline__ int rdlane_i(int v, int l) { return (int)emu::shfl_idx((double)v, l); }
-#else
-__device__ __forceinline__ int rdlane_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
-__device__ __forceinline__ double rdlane(double v, int l) {
-    long long b = __double_as_longlong(v);
-    int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
-    int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
-    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
-}
-__device__ __forceinline__ double shfl_from(double v, int src) { return __shfl(v, src, 64); }
-__device__ __forceinline__ int shfl_from(int v, int src) { return __shfl(v, src, 64); }
-#endif
 
 // Cholesky solve of the symmetric normal equations held one column per lane (lane j < N:
 // column j of A, lane N: right-hand side b), on wavefront 0.  Step k takes the pivot
