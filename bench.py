@@ -41,6 +41,9 @@ def parse():
     ap.add_argument("--cpu-points", type=int, default=384, help="oracle sample: first M points of the order")
     ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (0 = min(16, cores))")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-sweep", type=int, default=0, metavar="M",
+                    help="only time the oracle on the first M points of the default workload at 1, 2, 4, 8 and "
+                         "16 threads (the reference's thread sweep) and print one JSON line")
     ap.add_argument("--bytes-sample", type=int, default=400, help="points sampled for the algorithmic byte count")
     ap.add_argument("--cache", default="", help="directory to keep the generated volumes in between runs")
     ap.add_argument("--no-proposal", action="store_true", help="skip the learned-proposal leg")
@@ -241,6 +244,29 @@ def uneven(a, D, synth):
     print(json.dumps(out), flush=True)
 
 
+def cpu_sweep(a, case, order, neigh):
+    """The oracle's points/s on a prefix of the default workload's processing order at
+    1..P threads (OpenMP inside interpolation, as the reference parallelises; its thread
+    sweep is tests.cpp:215-314)."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as O
+    cores = os.cpu_count() or 1
+    M = min(a.cpu_sweep, len(case.points))
+    run = O.make_run(shape=case.ref.shape, bytes_per=case.ref.dtype.itemsize, jacobian=0, **case.params)
+    rows = []
+    for thr in (1, 2, 4, 8, 16):
+        if thr > cores:
+            break
+        _, sec, n = O.dvc_run(run, case.ref, case.tar, case.points, order, neigh, nproc=M, nthreads=thr)
+        rows.append({"threads": thr, "points_per_s": n / sec, "seconds": round(sec, 2)})
+        log(f"oracle, {thr} threads: {n / sec:.2f} points/s ({sec:.1f} s for {n} points)")
+    base = rows[0]["points_per_s"]
+    for r in rows:
+        r["speedup"] = round(r["points_per_s"] / base, 2)
+    print(json.dumps({"cpu_sweep": rows, "sample": f"first {M} points of the processing order", "cores": cores,
+                      "workload": "plain 12-DOF affine, 256^3 u16, sphere d=41 / 5000 samples"}), flush=True)
+
+
 def learned_eval(a, D, synth):
     """The learned proposal's evaluation on a 1024^3 uint8 pair (translation up to 10
     voxels per axis plus a sinusoidal field, 10^5 random points): the plain serial
@@ -374,7 +400,7 @@ def full_scale(a, D, synth, workload):
         ref = ctx.download(0, case.shape, np.uint16)
         tar = ctx.download(1, case.shape, np.uint16)
         thr = a.cpu_threads or min(16, os.cpu_count() or 1)
-        M = min(a.cpu_points, npts)
+        M = min(max(a.cpu_points, 2000), npts)       # a prefix of at least 2000 points at this size
         run = O.make_run(shape=case.shape, bytes_per=2, jacobian=0, **case.params)
         ores, sec, m = O.dvc_run(run, ref, tar, case.points, order, neigh, nproc=M, nthreads=thr)
         idx = order[:m]
@@ -433,6 +459,8 @@ def main():
     case = load_case(a.cache, synth)
     p = case.params
     order, neigh = D.sort_order_neighbors(case.points, case.start)
+    if a.cpu_sweep:
+        return cpu_sweep(a, case, order, neigh)
     log(f"workload built in {time.time() - t0:.1f} s: {len(case.points)} points, volumes {case.ref.shape} "
         f"{case.ref.dtype}")
 
