@@ -61,6 +61,7 @@ struct dvc_ctx {
     // per-point geometry for the next prepare (empty: the run parameters' for all points)
     std::vector<int32_t> pg_geom, pg_size, pg_npts;
     PtGeo *d_pgeo = nullptr;
+    int *d_ref_order = nullptr;      // reference_kernel's workgroup order (spatial, per XCD)
     RefStats *d_rstat = nullptr;
     // start modes (DVC_START_*): the mode set for the next prepare and the prepared one
     int start_mode = DVC_START_CHAIN;
@@ -110,13 +111,16 @@ void drop_graph(dvc_ctx *c) {
 
 // HBM for the reference sub-volumes computed ahead of the level chain
 constexpr size_t kRefBudgetBytes = (size_t)48 << 30;
+// accelerator complex dies of an MI355X (gfx950), each with its own L2
+constexpr int kXcds = 8;
 
 void free_plan(dvc_ctx *c) {
     drop_graph(c);
     hipFree(c->d_pts); hipFree(c->d_sph); hipFree(c->d_order); hipFree(c->d_pos);
     hipFree(c->d_neigh); hipFree(c->d_levels); hipFree(c->d_res);
-    hipFree(c->d_rvals); hipFree(c->d_rstat); hipFree(c->d_pgeo);
+    hipFree(c->d_rvals); hipFree(c->d_rstat); hipFree(c->d_pgeo); hipFree(c->d_ref_order);
     c->d_pgeo = nullptr;
+    c->d_ref_order = nullptr;
     hipFree(c->d_start); hipFree(c->d_res_prop); hipFree(c->d_res_judge); hipFree(c->d_kept);
     c->d_start = nullptr; c->d_res_prop = c->d_res_judge = nullptr; c->d_kept = nullptr;
     c->starts_set = false;
@@ -552,6 +556,18 @@ int dvc_search_prepare(dvc_ctx *ctx, const dvc_run_params *rc, const double *poi
         level_pts.assign(order, order + N);
         level_off = {0, N};
     }
+    // within each launch, workgroups in the XCD-aware spatial order (the points of one
+    // launch do not read each other's results, so any order gives the same results)
+    {
+        std::vector<int> seg, idx;
+        for (size_t L = 0; L + 1 < level_off.size(); L++) {
+            const int lo = level_off[L], cnt = level_off[L + 1] - lo;
+            if (cnt < 2) continue;
+            spatial_block_order(points, level_pts.data() + lo, cnt, kXcds, idx);
+            seg.assign(level_pts.begin() + lo, level_pts.begin() + lo + cnt);
+            for (int b = 0; b < cnt; b++) level_pts[lo + b] = seg[idx[b]];
+        }
+    }
     if (int e = dev_alloc(ctx, &ctx->d_levels, level_pts.size())) return e;
     if (int e = dev_alloc(ctx, &ctx->d_res, (size_t)npts)) return e;
     ctx->starts_set = false;
@@ -565,9 +581,17 @@ int dvc_search_prepare(dvc_ctx *ctx, const dvc_run_params *rc, const double *poi
         if (int e = dev_alloc(ctx, &ctx->d_kept, (size_t)npts)) return e;
         HIPCHK(hipMemsetAsync(ctx->d_kept, 0, sizeof(int) * (size_t)npts, ctx->stream));
     }
+    std::vector<int> ref_order;
     if (table) {
         if (int e = dev_alloc(ctx, &ctx->d_rvals, table_floats)) return e;
         if (int e = dev_alloc(ctx, &ctx->d_rstat, (size_t)N)) return e;
+        spatial_block_order(points, order, N, kXcds, ref_order);
+        if (int e = dev_alloc(ctx, &ctx->d_ref_order, (size_t)N)) return e;
+        HIPCHK(hipMemcpyAsync(ctx->d_ref_order, ref_order.data(), sizeof(int) * (size_t)N, hipMemcpyHostToDevice,
+                              ctx->stream));
+    } else if (ctx->d_ref_order) {
+        hipFree(ctx->d_ref_order);
+        ctx->d_ref_order = nullptr;
     }
     if (uneven) {
         if (int e = dev_alloc(ctx, &ctx->d_pgeo, (size_t)npts)) return e;
@@ -619,8 +643,10 @@ static int enqueue_search(dvc_ctx *ctx, hipStream_t st) {
     const long long res_words = (long long)ctx->npts * (long long)(sizeof(dvc_point_result) / 4);
     if (launch_zero_int((int *)ctx->d_res, res_words, st) != 0)
         return fail(ctx, DVC_ERR_HIP, "zero launch");
+    a.ref_order = ctx->d_ref_order;
     if (a.rvals && launch_reference(a, bt, (size_t)a.nsamp_pad * sizeof(float), st) != 0)
         return fail(ctx, DVC_ERR_HIP, "reference kernel launch failed");
+    a.ref_order = nullptr;
     if (ctx->plan_mode == DVC_START_CHAIN) {
         const int depth = (int)ctx->level_off.size() - 1;
         for (int L = 0; L < depth; L++) {

@@ -232,4 +232,40 @@ inline int capacity_schedule(const int *order, const std::vector<int> &pos, cons
     return depth;
 }
 
+// Workgroup order of the reference kernel: block b computes processing position
+// out[b].  The positions are sorted by the Morton code of their point's 16-voxel cell,
+// and the chips' L2 caches are per XCD, with workgroups dealt to the XCDs in turn
+// (block b to XCD b % xcds); so XCD x gets one contiguous run of the spatial order, and
+// points whose sub-volumes overlap share an L2.  Any order gives the same results: each
+// workgroup fills only its own point's row of the table.
+inline void spatial_block_order(const double *points, const int *order, int N, int xcds, std::vector<int> &out) {
+    auto spread = [](uint64_t v) {                 // 21 bits -> every third bit
+        v &= 0x1fffff;
+        v = (v | v << 32) & 0x1f00000000ffffull;
+        v = (v | v << 16) & 0x1f0000ff0000ffull;
+        v = (v | v << 8) & 0x100f00f00f00f00full;
+        v = (v | v << 4) & 0x10c30c30c30c30c3ull;
+        v = (v | v << 2) & 0x1249249249249249ull;
+        return v;
+    };
+    std::vector<std::pair<uint64_t, int>> key(N);
+    for (int i = 0; i < N; i++) {
+        const double *q = points + 3 * (size_t)order[i];
+        uint64_t c[3];
+        for (int d = 0; d < 3; d++) {
+            const double v = std::isfinite(q[d]) ? q[d] / 16.0 : 0.0;
+            c[d] = v <= 0.0 ? 0 : (v >= 2097151.0 ? 2097151 : (uint64_t)v);
+        }
+        key[i] = {spread(c[0]) | spread(c[1]) << 1 | spread(c[2]) << 2, i};
+    }
+    std::sort(key.begin(), key.end());
+    out.assign(N, 0);
+    size_t start = 0;
+    for (int x = 0; x < xcds; x++) {
+        const int cnt = x < N ? (N - x + xcds - 1) / xcds : 0;   // blocks x, x + xcds, ...
+        for (int j = 0; j < cnt; j++) out[(size_t)x + (size_t)j * xcds] = key[start + j].second;
+        start += cnt;
+    }
+}
+
 }  // namespace dvc

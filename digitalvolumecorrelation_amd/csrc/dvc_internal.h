@@ -53,6 +53,7 @@ struct SearchArgs {
     float *rvals;                    // [nproc * nsamp_pad] reference samples by position, or null
     RefStats *rstat;                 // [nproc]
     const PtGeo *pgeo;               // [npts] per-point geometry, or null (one geometry for all)
+    const int *ref_order;            // [nproc] processing position of each reference_kernel workgroup, or null
 };
 
 struct StrainArgs {

@@ -1507,7 +1507,7 @@ template <int BPV, int B, bool UNEVEN>
 __global__ void __launch_bounds__(B) reference_kernel(SearchArgs a) {
     DVC_SHARED(PtShared, S);
     DVC_SHARED(double[(B / 64) * 9], red);
-    const int p = blockIdx.x;
+    const int p = a.ref_order ? a.ref_order[blockIdx.x] : (int)blockIdx.x;
     const int n = a.order[p];
     if constexpr (UNEVEN) {
         DVC_SHARED(SearchArgs, la);

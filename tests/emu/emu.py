@@ -99,3 +99,15 @@ def schedule(order, neigh, cap=0):
     d = lib.emu_schedule(vp(o), n, vp(nb), nb.shape[1], int(cap), vp(lp), vp(lo))
     return [lp[lo[t]:lo[t + 1]].copy() for t in range(d)]
 
+
+def spatial_order(points, order, xcds=8):
+    """Workgroup order of the reference kernel: the processing position of each block."""
+    lib = load()
+    p = np.ascontiguousarray(points, dtype=np.float64)
+    o = np.ascontiguousarray(order, dtype=np.int32)
+    out = np.zeros(len(o), np.int32)
+    vp = lambda a: a.ctypes.data_as(C.c_void_p)
+    lib.emu_spatial_order.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p]
+    lib.emu_spatial_order(vp(p), vp(o), len(o), int(xcds), vp(out))
+    return out
+

@@ -152,3 +152,10 @@ extern "C" int emu_schedule(const int *order, int N, const int *neigh, int k, in
     return d;
 }
 
+// The reference kernel's workgroup order (dvc_geometry.h): out[N].
+extern "C" void emu_spatial_order(const double *points, const int *order, int N, int xcds, int *out) {
+    std::vector<int> v;
+    spatial_block_order(points, order, N, xcds, v);
+    std::copy(v.begin(), v.end(), out);
+}
+

@@ -154,6 +154,32 @@ def test_capacity_schedule(cap):
         assert np.array_equal(np.concatenate(launches), order)
 
 
+@pytest.mark.parametrize("n", [1, 7, 1000])
+def test_reference_block_order(n):
+    """The reference kernel's workgroup order is a permutation of the processing
+    positions; the blocks one XCD runs (b, b + 8, ...) hold one contiguous run of the
+    points sorted by Morton code of their 16-voxel cell, so neighbouring points share
+    that XCD's L2."""
+    rng = np.random.default_rng(n)
+    pts = rng.uniform(0, 500, (n, 3))
+    order = rng.permutation(n).astype(np.int32)
+    out = emu.spatial_order(pts, order)
+    assert np.array_equal(np.sort(out), np.arange(n))
+
+    def morton(q):
+        c = np.floor(q / 16.0).astype(np.int64)
+        k = 0
+        for bit in range(21):
+            for d in range(3):
+                k |= ((c[d] >> bit) & 1) << (3 * bit + d)
+        return k
+
+    keys = np.array([morton(pts[order[i]]) for i in range(n)])
+    runs = [out[x::8] for x in range(8)]
+    flat = np.concatenate(runs)
+    assert (np.diff(keys[flat]) >= 0).all()       # XCD 0's run, then XCD 1's, ... in key order
+
+
 def test_strain_kernel(oracle):
     pts = synth.grid_points(5, 9.0, (40.0, 50.0, 60.0))
     rng = np.random.default_rng(3)
