@@ -91,7 +91,7 @@ def proposal_leg(ctx, a, case, params, order, neigh, plain, D):
     if not wpath.exists():
         log(f"no proposal weights at {wpath}: NN-proposal leg skipped")
         return None
-    net = P.ProposalNet(ctx, max_batch=1024, params=P.ProposalNet.load_params(wpath))
+    net = P.ProposalNet(ctx, max_batch=4096, params=P.ProposalNet.load_params(wpath))
     ctx.set_start_mode(D.START_JUDGED)
     ctx.prepare(params, case.points, order, neigh)
     ctx.set_start_mode(D.START_CHAIN)
@@ -145,7 +145,7 @@ def uneven(a, D, synth):
     log(f"volumes made on the device in {time.time() - t0:.1f} s; {n} points, samples per point "
         f"{nsamp.min()}..{nsamp.max()} (cubes rounded up), {len(set(zip(geom, sub, nsamp)))} geometries")
     order, neigh = D.knn_order_neighbors(case.points, start=case.start)
-    net = P.ProposalNet(ctx, max_batch=1024, params=P.ProposalNet.load_params(Path(a.weights) if a.weights else P.WEIGHTS))
+    net = P.ProposalNet(ctx, max_batch=4096, params=P.ProposalNet.load_params(Path(a.weights) if a.weights else P.WEIGHTS))
     out = {"workload": f"batched uneven: {case.shape[0]}^3 u8 pair made on the device (sinusoids of amplitude 3, "
                        f"wavelength 128), {n} random points, each a cube or sphere of size 21..61 with 1000..10000 "
                        f"samples, tricubic, 6-DOF, disp_max 6", "points": n}
@@ -308,7 +308,7 @@ def learned_eval(a, D, synth):
     ms_plain, plain = timed(D.START_CHAIN)
     levels = ctx.levels()
     log(f"plain chain: {ms_plain:.1f} ms per step, {levels} levels")
-    net = P.ProposalNet(ctx, max_batch=1024, params=P.ProposalNet.load_params(Path(a.weights) if a.weights else P.WEIGHTS))
+    net = P.ProposalNet(ctx, max_batch=4096, params=P.ProposalNet.load_params(Path(a.weights) if a.weights else P.WEIGHTS))
     ms_j, judged = timed(D.START_JUDGED, propose=net.propose)
     prop, judge, kept = ctx.fetch_passes()
     log(f"judged proposals: {ms_j:.1f} ms per step")
