@@ -174,9 +174,10 @@ int dvc_search_points(dvc_ctx *ctx, const dvc_run_params *rc, const double *poin
                       const int32_t *neigh, int32_t k, dvc_point_result *results);
 
 /* The same loop split for callers that keep inputs resident: prepare uploads the
- * point cloud and its order/neighbour tables and derives the dependency levels;
- * execute enqueues one launch per level on the context stream without synchronising;
- * fetch waits and copies results back. */
+ * point cloud and its order/neighbour tables and derives the chain's launches (the
+ * dependency levels, or launches of the round capacity, dvc_set_round_capacity);
+ * execute enqueues them on the context stream without synchronising; fetch waits and
+ * copies results back. */
 int dvc_search_prepare(dvc_ctx *ctx, const dvc_run_params *rc, const double *points,
                        int32_t npts, const int32_t *order, int32_t nproc,
                        const int32_t *neigh, int32_t k);
