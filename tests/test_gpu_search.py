@@ -104,3 +104,30 @@ def test_cloud_larger_than_lds(ctx, oracle):
     p = dict(case.params, subvol_size=41, disp_max=4)
     g, o = _compare(ctx, oracle, case, p, truth_tol=0.05)
     assert np.all(g["status"] == 0)
+
+
+def test_reference_maximum_sizes(ctx, oracle):
+    """The largest inputs the reference accepts: subvol_npts 50,000 (a cube of 37^3 =
+    50,653 samples, InputRead.cpp:560-571) and 128 neighbours per point (the library's
+    limit; the reference uses 75).  All 216 points run on the GPU; the oracle checks the
+    first 16 of the processing order, and the rest must converge to the imposed shift."""
+    case = synth.config1(size=112, n=6, spacing=8, npts=50000)
+    p = dict(case.params, subvol_size=41, disp_max=4)
+    order, neigh = D.knn_order_neighbors(case.points, start=case.start, k=128)
+    assert neigh.shape[1] == 128
+    run = oracle.make_run(shape=case.ref.shape, bytes_per=1, jacobian=0, **p)
+    M = 16
+    ores, _, m = oracle.dvc_run(run, case.ref, case.tar, case.points, order, neigh, nproc=M, nthreads=8)
+    ctx.upload(0, case.ref)
+    ctx.upload(1, case.tar)
+    g = ctx.search(D.make_params(**p), case.points, order, neigh)
+    idx = order[:m]
+    assert np.array_equal(g["status"][idx], ores["status"][idx])
+    assert np.array_equal(g["iters"][idx], ores["iters"][idx])
+    du = np.abs(g["par"][idx, :3] - ores["par"][idx, :3]).max()
+    rel = (np.abs(g["obj"][idx] - ores["obj"][idx]) / np.abs(ores["obj"][idx])).max()
+    print(f"  {m} points against the oracle: max |du| {du:.2e}, max obj rel {rel:.2e}")
+    assert du <= 1e-3 and rel <= 1e-5
+    assert (g["status"] == 0).all()
+    assert np.abs(g["par"][:, :3] - case.truth).max() < 0.05
+
