@@ -54,3 +54,22 @@ def test_rejected_inputs_then_a_good_run(ctx):
     r = ctx.fetch()
     assert (r["status"] == 0).all()
     assert np.abs(r["par"][:, :3] - case.truth).max() < 0.05
+
+
+def test_single_point(ctx, oracle):
+    """The smallest input the reference runs: one point, its own neighbour list of one
+    (itself, never earlier in the order), so the start is the rigid translation."""
+    case = synth.config1(size=72, n=3, spacing=8)
+    pt = case.points[13:14]
+    order = np.zeros(1, np.int32)
+    neigh = np.zeros((1, 1), np.int32)
+    ctx.upload(0, case.ref)
+    ctx.upload(1, case.tar)
+    g = ctx.search(D.make_params(**case.params), pt, order, neigh)
+    run = oracle.make_run(shape=case.ref.shape, bytes_per=1, jacobian=0, **case.params)
+    o, _, _ = oracle.dvc_run(run, case.ref, case.tar, pt, order, neigh, nthreads=4)
+    assert g["status"][0] == o["status"][0] == 0
+    assert g["iters"][0] == o["iters"][0]
+    assert np.abs(g["par"][0, :3] - o["par"][0, :3]).max() <= 1e-3
+    assert np.abs(g["par"][0, :3] - case.truth[13]).max() < 0.05
+
