@@ -68,13 +68,13 @@ def log(msg):
 def load_case(cache: str, synth):
     """The synthetic workload; its two volumes optionally kept as .npy under `cache`."""
     if not cache:
-        return synth.config2()
+        return synth.affine_u16()
     d = Path(cache)
     r, t = d / "affine_u16_256_ref.npy", d / "affine_u16_256_tar.npy"
     if r.exists() and t.exists():
-        case = synth.config2(volumes=(np.load(r), np.load(t)))
+        case = synth.affine_u16(volumes=(np.load(r), np.load(t)))
     else:
-        case = synth.config2()
+        case = synth.affine_u16()
         d.mkdir(parents=True, exist_ok=True)
         np.save(r, case.ref)
         np.save(t, case.tar)
@@ -140,7 +140,7 @@ def uneven(a, D, synth):
     from digitalvolumecorrelation_amd import proposal as P
     t0 = time.time()
     ctx = D.Context(0)
-    case, geom, sub, nsamp = synth.config3(ctx)
+    case, geom, sub, nsamp = synth.uneven_u8(ctx)
     n = len(case.points)
     log(f"volumes made on the device in {time.time() - t0:.1f} s; {n} points, samples per point "
         f"{nsamp.min()}..{nsamp.max()} (cubes rounded up), {len(set(zip(geom, sub, nsamp)))} geometries")
@@ -277,7 +277,7 @@ def learned_eval(a, D, synth):
     from digitalvolumecorrelation_amd import proposal as P
     t0 = time.time()
     ctx = D.Context(0)
-    case = synth.config4_eval(ctx)
+    case = synth.learned_eval_u8(ctx)
     log(f"volumes made on the device in {time.time() - t0:.1f} s: {case.shape} uint8")
     order, neigh = D.knn_order_neighbors(case.points, start=case.start)
     params = D.make_params(**case.params)
@@ -350,7 +350,7 @@ def full_scale(a, D, synth, workload):
     t0 = time.time()
     ctx = D.Context(0)
     size, n = a.full_size, a.full_grid
-    case = synth.config5(ctx, size=size, n=n, spacing=14.0 * 100 / n if size == 1536 else (size - 150) / (n - 1))
+    case = synth.full_scale_u16(ctx, size=size, n=n, spacing=14.0 * 100 / n if size == 1536 else (size - 150) / (n - 1))
     t_gen = time.time() - t0
     log(f"volumes made on the device in {t_gen:.1f} s: {case.shape} uint16")
     t1 = time.time()

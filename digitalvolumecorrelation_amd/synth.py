@@ -136,7 +136,7 @@ class Case:
     start: np.ndarray | None = None        # starting_point (None: points[0])
 
 
-def config1(size: int = 128, n: int = 10, spacing: float = 8.0, seed: int = 1,
+def translation_u8(size: int = 128, n: int = 10, spacing: float = 8.0, seed: int = 1,
             t=(1.3, -0.7, 2.1), npts: int = 2000) -> Case:
     """Plain 3-DOF translation case: uint8 size^3, sigma 1.5, cube 31 / 2000 samples,
     ZNSSD, disp_max 5, rigid_trans the nearest integer translation."""
@@ -150,7 +150,7 @@ def config1(size: int = 128, n: int = 10, spacing: float = 8.0, seed: int = 1,
     return Case("translation-u8", ref, tar, pts, truth, params)
 
 
-def config2(size: int = 256, n: int = 15, spacing: float = 12.0, seed: int = 2,
+def affine_u16(size: int = 256, n: int = 15, spacing: float = 12.0, seed: int = 2,
             t=(2.4, 0.3, -1.1), rot_deg: float = 1.0, strain_z: float = 0.005, npts: int = 5000,
             dof: int = 12, diameter: int = 41, volumes=None) -> Case:
     """Plain 12-DOF affine case: uint16 size^3, sigma 1.5, rotation about x through the
@@ -205,7 +205,7 @@ def bump_field(X, bumps, strain, center, waves=None):
     return u
 
 
-def config5(ctx, size: int = 1536, n: int = 100, spacing: float = 14.0, seed: int = 5, nbumps: int = 8,
+def full_scale_u16(ctx, size: int = 1536, n: int = 100, spacing: float = 14.0, seed: int = 5, nbumps: int = 8,
             amplitude: float = 4.0, width: float = 200.0, volumetric: float = 0.01, npts: int = 8000,
             diameter: int = 41) -> DeviceCase:
     """Full-scale 12-DOF case made on the device: uint16 size^3 smoothed noise (sigma 2),
@@ -231,7 +231,7 @@ def config5(ctx, size: int = 1536, n: int = 100, spacing: float = 14.0, seed: in
     return DeviceCase("full-u16", shape, np.uint16, pts, truth, params, start, bumps, strain)
 
 
-def config4_eval(ctx, size: int = 1024, npoints: int = 100_000, seed: int = 4, trans: float = 10.0,
+def learned_eval_u8(ctx, size: int = 1024, npoints: int = 100_000, seed: int = 4, trans: float = 10.0,
                  amplitude: float = 2.0, wavelength: float = 256.0, margin: float = 48.0) -> DeviceCase:
     """Evaluation pair of the learned proposal, made on the device: uint8 size^3 smoothed
     noise (sigma 2); the target under a uniform translation (each axis up to `trans`
@@ -265,7 +265,7 @@ def config4_eval(ctx, size: int = 1024, npoints: int = 100_000, seed: int = 4, t
     return case
 
 
-def config3(ctx, size: int = 512, npoints: int = 50_000, seed: int = 3, amplitude: float = 3.0,
+def uneven_u8(ctx, size: int = 512, npoints: int = 50_000, seed: int = 3, amplitude: float = 3.0,
             wavelength: float = 128.0, margin: float = 48.0):
     """Batched uneven workload made on the device: uint8 size^3 smoothed noise (sigma 2),
     the target under three plane sinusoids (amplitude 3 voxels along x, y and z in turn,

@@ -6,7 +6,7 @@
  * (Core/Search.cpp, Core/Interpolate.cpp, Core/Matrix_4d.*, Core/FloatingCloud.cpp,
  * Core/SearchParams.cpp, Core/ObjectiveFunctions.cpp, Core/DataCloud.cpp,
  * Core/strain.cpp).  Only tests/, __graft_entry__.smoke() and bench.py's
- * cpu_baseline leg load it, as the checker and as the timed CPU baseline.  The
+ * cpu_baseline leg load it, for the results to compare against and as the timed CPU baseline.  The
  * product (libdvc_amd.so) never links or calls it.
  */
 #ifndef DVC_ORACLE_H

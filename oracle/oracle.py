@@ -1,7 +1,7 @@
 """ctypes wrapper of the FP64 CPU oracle (oracle/_build/liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, as the checker and as the timed CPU restatement of
+cpu_baseline leg of bench.py, as the reference results to compare against and as the timed CPU restatement of
 the reference.  The product (digitalvolumecorrelation_amd) never imports it.
 """
 from __future__ import annotations

@@ -1,6 +1,6 @@
 """FP64 numpy restatement of the learned start-warp proposal network.
 
-TEST INFRASTRUCTURE ONLY: the checker of the network kernels in
+TEST INFRASTRUCTURE ONLY: the expected values for the network kernels in
 digitalvolumecorrelation_amd/csrc/dvc_net.hip (crops, forward pass, loss and every
 parameter gradient).  The reference has no counterpart of this network; what fixes
 the expected values is the mathematics of the layers themselves, pinned in

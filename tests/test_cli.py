@@ -2,7 +2,7 @@
 ROI reading, the .sort.csv and .stat files, and the reference's error messages
 (Core/InputRead.cpp:477-716, 1033-1106; Core/DataCloud.cpp:43-67, 146-158;
 Core/strain.cpp:136-330).  Without a GPU the programs stop with a device error after
-writing these files; on a GPU box they run to the end."""
+writing these files; with a gfx950 GPU they run to the end."""
 import math
 
 import numpy as np
@@ -19,7 +19,7 @@ def bins():
 
 @pytest.fixture(scope="module")
 def case():
-    return synth.config1(size=48, n=2, spacing=8, npts=500)
+    return synth.translation_u8(size=48, n=2, spacing=8, npts=500)
 
 
 def _params(case):
@@ -89,7 +89,7 @@ def test_roi_file_rules(tmp_path, bins, case):
 
 
 def test_header_and_big_endian_volumes_accepted(tmp_path, bins):
-    case = synth.config2(size=48, n=2, spacing=8, npts=400, diameter=13)
+    case = synth.affine_u16(size=48, n=2, spacing=8, npts=400, diameter=13)
     p = dict(case.params, disp_max=3)
     d = U.write_run(tmp_path, case.ref, case.tar, case.points, p, header=b"\0" * 24, big_endian=True)
     r = U.run([bins / "dvc", "dvc_in"], tmp_path)

@@ -37,13 +37,13 @@ def _cmp(oracle, case, params, threads=64, nproc=0, gather32=True, precompute=Fa
 
 @pytest.mark.parametrize("gather32", [True, False])
 def test_translation_cube_u8(oracle, gather32):
-    case = synth.config1(size=56, n=2, spacing=8, npts=1000)
+    case = synth.translation_u8(size=56, n=2, spacing=8, npts=1000)
     _cmp(oracle, case, dict(case.params, subvol_size=21, disp_max=4), gather32=gather32)
 
 
 def test_odd_row_width_u8(oracle):
     """A row width that is not a whole number of dwords takes the 64-bit row addressing."""
-    case = synth.config1(size=58, n=2, spacing=8, npts=1000)
+    case = synth.translation_u8(size=58, n=2, spacing=8, npts=1000)
     ref, tar = case.ref[:, :, :57].copy(), case.tar[:, :, :57].copy()
     case = synth.Case(case.name, ref, tar, case.points, case.truth, case.params)
     _cmp(oracle, case, dict(case.params, subvol_size=21, disp_max=4))
@@ -51,14 +51,14 @@ def test_odd_row_width_u8(oracle):
 
 @pytest.mark.parametrize("precompute", [False, True])
 def test_affine_sphere_u16_12dof(oracle, precompute):
-    case = synth.config2(size=72, n=2, spacing=10, npts=600, diameter=21)
+    case = synth.affine_u16(size=72, n=2, spacing=10, npts=600, diameter=21)
     g, o = _cmp(oracle, case, dict(case.params, disp_max=5), threads=128, precompute=precompute)
     assert np.all(o["status"] == 0)
 
 
 @pytest.mark.parametrize("obj", ["ssd", "zssd", "nssd", "sad"])
 def test_6dof_objectives_and_partial(oracle, obj):
-    case = synth.config2(size=72, n=2, spacing=10, npts=600, diameter=21)
+    case = synth.affine_u16(size=72, n=2, spacing=10, npts=600, diameter=21)
     _cmp(oracle, case, dict(case.params, disp_max=5, ndof=6, obj=obj), nproc=5)
 
 
@@ -199,7 +199,7 @@ def test_12dof_range_failures(oracle):
     """Range_Fail at set-up (boxes leaving the volume) and inside the Gauss-Newton loop
     (the moving cloud leaving the active box with disp_max 1), where the bounding box
     comes from the speculative sample pass."""
-    case = synth.config2(size=72, n=3, spacing=20, npts=600, diameter=21)
+    case = synth.affine_u16(size=72, n=3, spacing=20, npts=600, diameter=21)
     g, o = _cmp(oracle, case, dict(case.params, disp_max=5), threads=128, precompute=True)
     assert (o["status"] == D.RANGE_FAIL).any() and (o["status"] == D.POINT_GOOD).any()
     g, o = _cmp(oracle, case, dict(case.params, disp_max=1, rigid_trans=(0.0, 0.0, 0.0)), threads=128)

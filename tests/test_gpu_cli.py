@@ -19,7 +19,7 @@ def bins():
 
 
 def test_dvc_and_strain_programs(tmp_path, bins, oracle):
-    case = synth.config1(size=72, n=4, spacing=8, npts=1000)
+    case = synth.translation_u8(size=72, n=4, spacing=8, npts=1000)
     p = dict(case.params, subvol_size=21, disp_max=4)
     start = (36.0, 36.0, 36.0)
     d = U.write_run(tmp_path, case.ref, case.tar, case.points, p, extra={"starting_point": "36.0 36.0 36.0"})

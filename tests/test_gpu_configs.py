@@ -28,7 +28,7 @@ def _check(g, o):
 
 
 def test_translation_u8_128_full(ctx, oracle):
-    case = synth.config1()
+    case = synth.translation_u8()
     p = case.params
     order, neigh = D.sort_order_neighbors(case.points, case.start)
     ctx.upload(0, case.ref)
@@ -49,7 +49,7 @@ def test_translation_u8_128_full(ctx, oracle):
 
 
 def test_affine_u16_256_prefix_and_properties(ctx, oracle):
-    case = synth.config2()
+    case = synth.affine_u16()
     p = case.params
     order, neigh = D.sort_order_neighbors(case.points, case.start)
     ctx.upload(0, case.ref)

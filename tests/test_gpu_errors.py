@@ -19,7 +19,7 @@ def ctx():
 
 
 def test_rejected_inputs_then_a_good_run(ctx):
-    case = synth.config1(size=72, n=3, spacing=8)
+    case = synth.translation_u8(size=72, n=3, spacing=8)
     order, neigh = D.sort_order_neighbors(case.points)
     ctx.upload(0, case.ref)
     ctx.upload(1, case.tar)
@@ -59,7 +59,7 @@ def test_rejected_inputs_then_a_good_run(ctx):
 def test_single_point(ctx, oracle):
     """The smallest input the reference runs: one point, its own neighbour list of one
     (itself, never earlier in the order), so the start is the rigid translation."""
-    case = synth.config1(size=72, n=3, spacing=8)
+    case = synth.translation_u8(size=72, n=3, spacing=8)
     pt = case.points[13:14]
     order = np.zeros(1, np.int32)
     neigh = np.zeros((1, 1), np.int32)

@@ -22,7 +22,7 @@ pytestmark = pytest.mark.gpu
 
 @pytest.fixture(scope="module")
 def pair():
-    return synth.config1(size=64, n=2, spacing=8)
+    return synth.translation_u8(size=64, n=2, spacing=8)
 
 
 def _bits(d):
@@ -101,7 +101,7 @@ def test_training_fits_a_fixed_batch(ctx, pair):
 
 
 def test_propose_equals_forward(ctx):
-    case = synth.config1(size=80, n=3, spacing=8)
+    case = synth.translation_u8(size=80, n=3, spacing=8)
     ctx.upload(0, case.ref)
     ctx.upload(1, case.tar)
     net = P.ProposalNet(ctx, max_batch=8, params=P.init_params(11))

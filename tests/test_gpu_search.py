@@ -45,7 +45,7 @@ def _compare(ctx, oracle, case, params, start=None, nproc=0, threads=None, truth
 
 @pytest.fixture(scope="module")
 def case1():
-    return synth.config1(size=80, n=4, spacing=8)
+    return synth.translation_u8(size=80, n=4, spacing=8)
 
 
 def test_translation_cube_u8(ctx, oracle, case1):
@@ -59,7 +59,7 @@ def test_block_sizes_agree(ctx, oracle, case1, threads):
 
 @pytest.fixture(scope="module")
 def case12():
-    return synth.config2(size=96, n=3, spacing=12, npts=2000, diameter=25)
+    return synth.affine_u16(size=96, n=3, spacing=12, npts=2000, diameter=25)
 
 
 def test_affine_sphere_u16_12dof(ctx, oracle, case12):
@@ -92,7 +92,7 @@ def test_range_fail_partial_and_grid(ctx, oracle, case1):
 
 
 def test_points_near_volume_edge(ctx, oracle):
-    case = synth.config1(size=64, n=4, spacing=14)   # outer points' boxes leave the volume
+    case = synth.translation_u8(size=64, n=4, spacing=14)   # outer points' boxes leave the volume
     g, o = _compare(ctx, oracle, case, case.params)
     assert (g["status"] == D.RANGE_FAIL).any() and (g["status"] == D.POINT_GOOD).any()
 
@@ -100,7 +100,7 @@ def test_points_near_volume_edge(ctx, oracle):
 def test_cloud_larger_than_lds(ctx, oracle):
     """A cube cloud of 35^3 = 42,875 samples (subvol_npts 40,000; the reference accepts up
     to 50,000): the reference samples live in the HBM table, not in LDS."""
-    case = synth.config1(size=96, n=2, spacing=10, npts=40000)
+    case = synth.translation_u8(size=96, n=2, spacing=10, npts=40000)
     p = dict(case.params, subvol_size=41, disp_max=4)
     g, o = _compare(ctx, oracle, case, p, truth_tol=0.05)
     assert np.all(g["status"] == 0)
@@ -111,7 +111,7 @@ def test_reference_maximum_sizes(ctx, oracle):
     50,653 samples, InputRead.cpp:560-571) and 128 neighbours per point (the library's
     limit; the reference uses 75).  All 216 points run on the GPU; the oracle checks the
     first 16 of the processing order, and the rest must converge to the imposed shift."""
-    case = synth.config1(size=112, n=6, spacing=8, npts=50000)
+    case = synth.translation_u8(size=112, n=6, spacing=8, npts=50000)
     p = dict(case.params, subvol_size=41, disp_max=4)
     order, neigh = D.knn_order_neighbors(case.points, start=case.start, k=128)
     assert neigh.shape[1] == 128

@@ -68,12 +68,12 @@ def _run(ctx, oracle, case, params, starts, mode, nproc=0):
 
 @pytest.fixture(scope="module")
 def case12():
-    return synth.config2(size=96, n=3, spacing=12, npts=2000, diameter=25)
+    return synth.affine_u16(size=96, n=3, spacing=12, npts=2000, diameter=25)
 
 
 @pytest.fixture(scope="module")
 def case3():
-    return synth.config1(size=80, n=4, spacing=8)
+    return synth.translation_u8(size=80, n=4, spacing=8)
 
 
 def test_given_starts_12dof(ctx, oracle, case12):
@@ -158,7 +158,7 @@ def test_round_capacity_chain_is_identical(ctx):
     # launches of at most 3 points instead of dependency levels (216 points, 6 nearest
     # neighbours: 22 levels of up to 22 points): every point reads the same finished
     # neighbour results, so every bit of every result is the same
-    case = synth.config1(size=96, n=6, spacing=9)
+    case = synth.translation_u8(size=96, n=6, spacing=9)
     order, neigh = D.knn_order_neighbors(case.points, start=case.start, k=6)
     ctx.upload(0, case.ref)
     ctx.upload(1, case.tar)

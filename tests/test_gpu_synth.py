@@ -52,7 +52,7 @@ def test_identity_and_uniform_translation(ctx):
 
 
 def test_full_scale_field_matches_host_formula(ctx):
-    case = synth.config5(ctx, size=96, n=3, spacing=10.0)
+    case = synth.full_scale_u16(ctx, size=96, n=3, spacing=10.0)
     u = synth.bump_field(case.points, case.bumps, case.strain, np.full(3, 47.5))
     assert np.allclose(u, case.truth)
     assert abs((1 + case.strain) ** 3 - 1.01) < 1e-12

@@ -31,7 +31,7 @@ def _close(g, o, label):
 
 @pytest.fixture(scope="module")
 def case():
-    return synth.config1(size=80, n=4, spacing=8)
+    return synth.translation_u8(size=80, n=4, spacing=8)
 
 
 @pytest.mark.parametrize("obj", ["znssd", "ssd"])

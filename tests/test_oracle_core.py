@@ -181,7 +181,7 @@ def test_dense_solvers(oracle):
 
 @pytest.fixture(scope="module")
 def small_case():
-    return synth.config1(size=80, n=3, spacing=9)
+    return synth.translation_u8(size=80, n=3, spacing=9)
 
 
 def _run1(oracle, case, **kw):

@@ -34,7 +34,7 @@ def chain_starts(res, order, neigh, ndof, rigid):
 
 @pytest.fixture(scope="module")
 def case():
-    return synth.config1(size=64, n=3, spacing=8)
+    return synth.translation_u8(size=64, n=3, spacing=8)
 
 
 def _run(oracle, case, **kw):
