@@ -54,6 +54,7 @@ struct dvc_ctx {
     dvc_point_result *d_res = nullptr;
     double off_min[3]{}, off_max[3]{};
     int chosen_threads = 256;
+    int ref_threads = 256;           // threads per point of reference_kernel (one launch over all points)
     int round_cap = 0;               // points per chain launch: 0 automatic, -1 levels only
     std::vector<char> searched;
     std::vector<int> level_off;      // dependency levels (dvc_geometry.h)
@@ -613,6 +614,7 @@ int dvc_search_prepare(dvc_ctx *ctx, const dvc_run_params *rc, const double *poi
     ctx->npts = npts; ctx->nproc = N; ctx->k = k;
     for (int q = 0; q < 3; q++) { ctx->off_min[q] = omn[q]; ctx->off_max[q] = omx[q]; }
     ctx->chosen_threads = bt;
+    ctx->ref_threads = 256;
     ctx->level_off = level_off;
     ctx->plan_mode = mode;
     ctx->searched.assign(npts, 0);
@@ -644,7 +646,7 @@ static int enqueue_search(dvc_ctx *ctx, hipStream_t st) {
     if (launch_zero_int((int *)ctx->d_res, res_words, st) != 0)
         return fail(ctx, DVC_ERR_HIP, "zero launch");
     a.ref_order = ctx->d_ref_order;
-    if (a.rvals && launch_reference(a, bt, (size_t)a.nsamp_pad * sizeof(float), st) != 0)
+    if (a.rvals && launch_reference(a, ctx->ref_threads, (size_t)a.nsamp_pad * sizeof(float), st) != 0)
         return fail(ctx, DVC_ERR_HIP, "reference kernel launch failed");
     a.ref_order = nullptr;
     if (ctx->plan_mode == DVC_START_CHAIN) {
@@ -837,6 +839,29 @@ int dvc_search_autotune(dvc_ctx *ctx, int reps, int32_t *best_threads, float *be
     }
     if (best == 0) return fail(ctx, DVC_ERR_STATE, "no workgroup size fits");
     ctx->chosen_threads = best;
+    // the reference kernel is one launch over every point, throughput-bound: its own
+    // workgroup size, timed with the chosen search size
+    if (ctx->d_rvals) {
+        int best_ref = ctx->ref_threads;
+        for (int c : {128, 512, 256}) {
+            if (c == best_ref) continue;
+            const int keep = ctx->ref_threads;
+            ctx->ref_threads = c;
+            drop_graph(ctx);
+            if (int e = dvc_search_execute(ctx)) return e;
+            float tot = 0.0f;
+            for (int r = 0; r < reps; r++) {
+                if (int e = dvc_search_execute(ctx)) return e;
+                float ms = 0.0f;
+                if (int e = dvc_last_kernel_ms(ctx, &ms)) return e;
+                tot += ms;
+            }
+            const float avg = tot / (float)reps;
+            if (avg < bms) { bms = avg; best_ref = c; }
+            else ctx->ref_threads = keep;
+        }
+        ctx->ref_threads = best_ref;
+    }
     drop_graph(ctx);
     if (best_threads) *best_threads = best;
     if (best_ms) *best_ms = bms;

@@ -227,7 +227,9 @@ int dvc_search_fetch_passes(dvc_ctx *ctx, dvc_point_result *proposal, dvc_point_
 int dvc_set_block_threads(dvc_ctx *ctx, int threads);
 /* Autotuner of a prepared search (no reference counterpart): times reps executes at
  * each threads-per-point count whose LDS fits (64, 128, 256, 512, 1024; one point per
- * workgroup) and keeps the fastest for later executes (until the next prepare).  The
+ * workgroup) and keeps the fastest for later executes (until the next prepare); then
+ * times 128, 256 and 512 threads per point for the launch that computes every point's
+ * reference samples (256 by default) and keeps the fastest of those too.  The
  * given starts must be set first in the given-start and judged modes.  Results change
  * only in the summation order of the per-point sums.  best_threads / best_ms (device
  * milliseconds per execute) may be NULL. */
