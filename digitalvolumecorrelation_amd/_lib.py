@@ -79,6 +79,8 @@ def load() -> C.CDLL:
         "dvc_set_round_capacity": (I, [VP, I]),
         "dvc_set_launch_mode": (I, [VP, C.c_int]),
         "dvc_search_info": (I, [VP, C.c_void_p, C.c_void_p, C.c_void_p]),
+        "dvc_set_slice_launches": (I, [VP, I, I, I]),
+        "dvc_search_slice_info": (I, [VP, C.c_void_p, C.c_void_p, C.c_void_p]),
         "dvc_set_start_mode": (I, [VP, C.c_int]),
         "dvc_set_point_geometry": (I, [VP, P(I), P(I), P(I), I]),
         "dvc_search_autotune": (I, [VP, C.c_int, P(I), P(C.c_float)]),
@@ -288,6 +290,19 @@ class Context:
         bt, ms = C.c_int32(), C.c_float()
         self._check(self.lib.dvc_search_autotune(self.h, reps, C.byref(bt), C.byref(ms)))
         return bt.value, ms.value
+
+    def set_slice_launches(self, max_workgroups: int = 0, evaluations: int = 0, samples_per_workgroup: int = 0):
+        """Slice launches of the chain (one launch per evaluation, a point's samples split
+        over workgroups) for launches of at most max_workgroups workgroups (0 automatic,
+        -1 never); `evaluations` slice launches before the one-workgroup search resumes."""
+        self._check(self.lib.dvc_set_slice_launches(self.h, int(max_workgroups), int(evaluations),
+                                                    int(samples_per_workgroup)))
+
+    def slice_info(self):
+        """(sliced chain launches, workgroups per point, slice launches per chain launch)."""
+        v = [C.c_int32(), C.c_int32(), C.c_int32()]
+        self._check(self.lib.dvc_search_slice_info(self.h, *[C.byref(x) for x in v]))
+        return tuple(x.value for x in v)
 
     def set_round_capacity(self, points: int):
         """Points per chain launch for the next prepare: 0 automatic (one round of the

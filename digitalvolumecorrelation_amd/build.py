@@ -11,6 +11,7 @@ from __future__ import annotations
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
@@ -36,14 +37,35 @@ def _run(cmd: list[str]) -> None:
     subprocess.run(cmd, check=True)
 
 
+# dvc_kernels.hip is compiled as KERNEL_PARTS translation units at once: part 0 holds
+# the launch dispatch and the small kernels, parts 1.. the search-kernel instantiations.
+KERNEL_PARTS = 6
+
+
 def build_library() -> Path:
-    srcs = [CSRC / "dvc_kernels.hip", CSRC / "dvc_api.cpp", CSRC / "dvc_net.hip", CSRC / "dvc_synth.hip"]
-    objs = []
-    for s in srcs:
-        o = CSRC / (s.stem + ".o")
-        lang = ["-x", "hip"] if s.suffix in (".hip", ".cpp") else []
-        _run([HIPCC, *HIP_FLAGS, *lang, "-c", str(s), "-o", str(o)])
-        objs.append(o)
+    jobs = []
+    for k in range(KERNEL_PARTS):
+        jobs.append((CSRC / "dvc_kernels.hip", CSRC / f"dvc_kernels_p{k}.o", [f"-DDVC_PART={k}"]))
+    for s in (CSRC / "dvc_api.cpp", CSRC / "dvc_net.hip", CSRC / "dvc_synth.hip"):
+        jobs.append((s, CSRC / (s.stem + ".o"), []))
+
+    def compile_one(job):
+        src, obj, defs = job
+        cmd = [HIPCC, *HIP_FLAGS, *defs, "-x", "hip", "-c", str(src), "-o", str(obj)]
+        print("+", " ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        return cmd, r
+
+    workers = max(1, min(len(jobs), int(os.environ.get("MAX_JOBS", "0") or 0) or (os.cpu_count() or 1)))
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        results = list(ex.map(compile_one, jobs))
+    for cmd, r in results:
+        msg = "\n".join(l for l in (r.stdout + r.stderr).splitlines() if "packed-fp32-ops" not in l)
+        if msg.strip():
+            print(msg, flush=True)
+        if r.returncode != 0:
+            raise subprocess.CalledProcessError(r.returncode, cmd)
+    objs = [o for _, o, _ in jobs]
     tmp = LIB.with_suffix(".so.tmp")
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp)])
     os.replace(tmp, LIB)

@@ -72,6 +72,14 @@ struct dvc_ctx {
     dvc_point_result *d_res_prop = nullptr;    // proposal pass (DVC_START_JUDGED)
     dvc_point_result *d_res_judge = nullptr;   // judge pass (DVC_START_JUDGED)
     int *d_kept = nullptr;                     // 1 where the proposal's result was kept
+    // slice launches of the chain (SliceArgs): largest workgroup count of a sliced launch
+    // (0 automatic, -1 off), slice launches per chain launch before the one-workgroup
+    // search resumes the points still searching (0 automatic), samples per workgroup
+    int slice_wgs = 0, slice_evals = 0, slice_per = 512;
+    std::vector<char> sliced;                  // chain launches run as slice launches
+    int slice_cnt8 = 0, slice_n = 0;           // buffer shape of the current plan
+    unsigned char *d_state = nullptr;          // [2][cnt8][kStateStride]
+    double *d_part = nullptr;                  // [2][cnt8][nslice][kPartStride]
     // the level launches of a prepared search, captured once and replayed
     int use_graph = 1;
     hipGraph_t graph = nullptr;
@@ -122,6 +130,10 @@ void free_plan(dvc_ctx *c) {
     hipFree(c->d_rvals); hipFree(c->d_rstat); hipFree(c->d_pgeo); hipFree(c->d_ref_order);
     c->d_pgeo = nullptr;
     c->d_ref_order = nullptr;
+    hipFree(c->d_state); hipFree(c->d_part);
+    c->d_state = nullptr; c->d_part = nullptr;
+    c->slice_cnt8 = c->slice_n = 0;
+    c->sliced.clear();
     hipFree(c->d_start); hipFree(c->d_res_prop); hipFree(c->d_res_judge); hipFree(c->d_kept);
     c->d_start = nullptr; c->d_res_prop = c->d_res_judge = nullptr; c->d_kept = nullptr;
     c->starts_set = false;
@@ -623,8 +635,43 @@ int dvc_search_prepare(dvc_ctx *ctx, const dvc_run_params *rc, const double *poi
     return DVC_OK;
 }
 
+// Slice launches per chain launch before the one-workgroup search resumes.
+static int slice_evals_of(const dvc_ctx *ctx) { return ctx->slice_evals > 0 ? ctx->slice_evals : 4; }
+
+// Samples per point split into workgroups of ctx->slice_per samples.
+static int slices_of(const dvc_ctx *ctx) {
+    return (ctx->geo.subv_num + ctx->slice_per - 1) / ctx->slice_per;
+}
+
+// Threads per slice workgroup: one sample per thread up to 512, more above.
+static int slice_threads_of(const dvc_ctx *ctx) { return ctx->slice_per >= kSliceThreads[1] ? kSliceThreads[1] : kSliceThreads[0]; }
+
+// Which chain launches of the prepared search run as slice launches, and their buffers.
+// Slice launches serve the serial chain with one geometry for all points, no translation
+// grid and the HBM table of reference samples.  Called outside stream capture.
+static int plan_slices(dvc_ctx *ctx) {
+    ctx->sliced.assign(ctx->level_off.empty() ? 0 : ctx->level_off.size() - 1, 0);
+    const bool eligible = ctx->plan_mode == DVC_START_CHAIN && ctx->d_rvals && !ctx->d_pgeo &&
+                          ctx->rc.basin_radius == 0.0 && ctx->slice_wgs >= 0;
+    if (!eligible) return DVC_OK;
+    const int nslice = slices_of(ctx);
+    const int max_wgs = ctx->slice_wgs > 0 ? ctx->slice_wgs : 4 * ctx->cus;
+    const int cnt8 = slice_plan(ctx->level_off, nslice, max_wgs, kXcds, ctx->sliced);
+    if (cnt8 == 0) return DVC_OK;
+    if (cnt8 > ctx->slice_cnt8 || nslice > ctx->slice_n || !ctx->d_state) {
+        const int c8 = std::max(cnt8, ctx->slice_cnt8), ns = std::max(nslice, ctx->slice_n);
+        if (int e = dev_alloc(ctx, &ctx->d_state, (size_t)2 * c8 * kStateStride)) return e;
+        if (int e = dev_alloc(ctx, &ctx->d_part, (size_t)2 * c8 * ns * kPartStride)) return e;
+        ctx->slice_cnt8 = c8;
+        ctx->slice_n = ns;
+    }
+    return DVC_OK;
+}
+
 // Enqueue the whole search on `st`: clear the results, then one launch per dependency
-// level (a level's points only read results of earlier levels).
+// level (a level's points only read results of earlier levels).  A level run as slice
+// launches takes slice_evals_of(ctx) launches of the slice kernel, then the
+// one-workgroup search resumes the points that are still searching.
 static int enqueue_search(dvc_ctx *ctx, hipStream_t st) {
     const Geometry &G = ctx->geo;
     const dvc_run_params &rc = ctx->rc;
@@ -651,10 +698,30 @@ static int enqueue_search(dvc_ctx *ctx, hipStream_t st) {
     a.ref_order = nullptr;
     if (ctx->plan_mode == DVC_START_CHAIN) {
         const int depth = (int)ctx->level_off.size() - 1;
+        SliceArgs sa{};
+        sa.nslice = slices_of(ctx);
+        sa.per = ctx->slice_per;
+        for (int q = 0; q < 2; q++) {
+            sa.state[q] = ctx->d_state ? ctx->d_state + (size_t)q * ctx->slice_cnt8 * kStateStride : nullptr;
+            sa.part[q] = ctx->d_part ? ctx->d_part + (size_t)q * ctx->slice_cnt8 * ctx->slice_n * kPartStride : nullptr;
+        }
+        const int sthreads = slice_threads_of(ctx);
         for (int L = 0; L < depth; L++) {
-            int cnt = ctx->level_off[L + 1] - ctx->level_off[L];
-            if (launch_search(a, ctx->d_levels + ctx->level_off[L], bt, cnt, lds, st) != 0)
+            const int cnt = ctx->level_off[L + 1] - ctx->level_off[L];
+            const int *lp = ctx->d_levels + ctx->level_off[L];
+            if (L < (int)ctx->sliced.size() && ctx->sliced[L]) {
+                sa.level_pts = lp;
+                sa.cnt = cnt;
+                sa.cnt8 = (cnt + kXcds - 1) / kXcds * kXcds;
+                for (int t = 0; t < slice_evals_of(ctx); t++) {
+                    sa.t = t;
+                    if (launch_slice(a, sa, sthreads, st) != 0) return fail(ctx, DVC_ERR_HIP, "slice kernel launch failed");
+                }
+                sa.t = slice_evals_of(ctx);
+                if (launch_resume(a, sa, st) != 0) return fail(ctx, DVC_ERR_HIP, "resume kernel launch failed");
+            } else if (launch_search(a, lp, bt, cnt, lds, st) != 0) {
                 return fail(ctx, DVC_ERR_HIP, "search kernel launch failed");
+            }
         }
         return DVC_OK;
     }
@@ -787,6 +854,8 @@ int dvc_search_execute(dvc_ctx *ctx) {
         return fail(ctx, DVC_ERR_STATE, "dvc_search_set_starts first");
     if (int e = check_volumes(ctx)) return e;
     HIPCHK(hipSetDevice(ctx->device));
+    if (!ctx->gexec)
+        if (int e = plan_slices(ctx)) return e;
     if (ctx->use_graph && !ctx->gexec) {
         if (hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
             return fail(ctx, DVC_ERR_HIP, "stream capture could not start");
@@ -862,9 +931,65 @@ int dvc_search_autotune(dvc_ctx *ctx, int reps, int32_t *best_threads, float *be
         }
         ctx->ref_threads = best_ref;
     }
+    // slice launches of the chain (unless switched off): off, and a few widths and
+    // launch counts, the fastest kept
+    if (ctx->plan_mode == DVC_START_CHAIN && ctx->slice_wgs >= 0 && ctx->d_rvals && !ctx->d_pgeo &&
+        ctx->rc.basin_radius == 0.0) {
+        const int keep_w = ctx->slice_wgs, keep_e = ctx->slice_evals;
+        const int C = ctx->cus;
+        const int cands[][2] = {{-1, 0}, {2 * C, 4}, {4 * C, 3}, {4 * C, 4}, {4 * C, 6}, {8 * C, 4}};
+        int bw = keep_w, be = keep_e;
+        float bsl = -1.0f;
+        for (const auto &c : cands) {
+            ctx->slice_wgs = c[0];
+            ctx->slice_evals = c[1];
+            drop_graph(ctx);
+            if (int e = dvc_search_execute(ctx)) return e;
+            float tot = 0.0f;
+            for (int r = 0; r < reps; r++) {
+                if (int e = dvc_search_execute(ctx)) return e;
+                float ms = 0.0f;
+                if (int e = dvc_last_kernel_ms(ctx, &ms)) return e;
+                tot += ms;
+            }
+            const float avg = tot / (float)reps;
+            if (bsl < 0.0f || avg < bsl) { bsl = avg; bw = c[0]; be = c[1]; }
+        }
+        ctx->slice_wgs = bw;
+        ctx->slice_evals = be;
+        bms = bsl;
+    }
     drop_graph(ctx);
     if (best_threads) *best_threads = best;
     if (best_ms) *best_ms = bms;
+    return DVC_OK;
+}
+
+int dvc_set_slice_launches(dvc_ctx *ctx, int32_t max_workgroups, int32_t evaluations, int32_t samples_per_workgroup) {
+    if (!ctx) return DVC_ERR_ARG;
+    if (max_workgroups < -1) return fail(ctx, DVC_ERR_ARG, "max_workgroups must be -1, 0 or positive");
+    if (evaluations < 0 || evaluations > 20) return fail(ctx, DVC_ERR_ARG, "evaluations must be in 0..20");
+    if (samples_per_workgroup != 0 && samples_per_workgroup != 256 &&
+        (samples_per_workgroup < 512 || samples_per_workgroup > 8192 || samples_per_workgroup % 512 != 0))
+        return fail(ctx, DVC_ERR_ARG, "samples_per_workgroup must be 0, 256 or a multiple of 512 up to 8192");
+    drop_graph(ctx);
+    ctx->slice_wgs = max_workgroups;
+    ctx->slice_evals = evaluations;
+    ctx->slice_per = samples_per_workgroup ? samples_per_workgroup : 512;
+    return DVC_OK;
+}
+
+int dvc_search_slice_info(dvc_ctx *ctx, int32_t *sliced_launches, int32_t *slices_per_point, int32_t *evaluations) {
+    if (!ctx) return DVC_ERR_ARG;
+    if (!ctx->plan_ready) return fail(ctx, DVC_ERR_STATE, "dvc_search_prepare first");
+    HIPCHK(hipSetDevice(ctx->device));
+    if (!ctx->gexec)
+        if (int e = plan_slices(ctx)) return e;
+    int n = 0;
+    for (char c : ctx->sliced) n += c ? 1 : 0;
+    if (sliced_launches) *sliced_launches = n;
+    if (slices_per_point) *slices_per_point = n ? slices_of(ctx) : 0;
+    if (evaluations) *evaluations = n ? slice_evals_of(ctx) : 0;
     return DVC_OK;
 }
 

@@ -149,6 +149,29 @@ inline void fill_search_args(SearchArgs &a, const dvc_run_params &rc, const Geom
     a.gather32 = (wb % 4 == 0) && ((long long)(G.box[2] + 1) * whb + 64 < (1ll << 31)) ? 1 : 0;
 }
 
+// Slice launches of the serial chain (SliceArgs): which chain launches split their points'
+// samples over several workgroups.  A launch of cnt points is sliced when its cnt8 *
+// nslice workgroups (cnt8: cnt rounded up to a multiple of xcds) are at most max_wgs and
+// there are at least two slices; narrow launches leave most of the chip idle with one
+// workgroup per point.  sliced[L] marks launch L; returns the largest cnt8 among them
+// (0: none).
+inline int slice_plan(const std::vector<int> &level_off, int nslice, int max_wgs, int xcds,
+                      std::vector<char> &sliced) {
+    const int nl = (int)level_off.size() - 1;
+    sliced.assign(nl > 0 ? nl : 0, 0);
+    int widest = 0;
+    if (nslice < 2 || max_wgs <= 0) return 0;
+    for (int L = 0; L < nl; L++) {
+        const int cnt = level_off[L + 1] - level_off[L];
+        const int cnt8 = (cnt + xcds - 1) / xcds * xcds;
+        if (cnt > 0 && (long long)cnt8 * nslice <= max_wgs) {
+            sliced[L] = 1;
+            widest = std::max(widest, cnt8);
+        }
+    }
+    return widest;
+}
+
 // Position of every point in order[] (npts when it is never searched, dvc.cpp:229-232).
 inline bool order_positions(const int *order, int npts, int N, std::vector<int> &pos) {
     pos.assign(npts, npts);

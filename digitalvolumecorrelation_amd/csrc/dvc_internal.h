@@ -56,6 +56,25 @@ struct SearchArgs {
     const int *ref_order;            // [nproc] processing position of each reference_kernel workgroup, or null
 };
 
+// Slice launches of the serial chain: the points of one chain launch run one launch per
+// Gauss-Newton evaluation, each point's samples split over nslice workgroups.  Block b
+// works on point slot b % cnt8 (level_pts[slot]) and samples [slice*per, slice*per + per)
+// with slice = b / cnt8; cnt8 is cnt rounded up to a multiple of the XCD count, so the
+// slices of one point share an XCD (and its L2).  No workgroup communicates with another
+// inside a launch: launch t writes every slice's sums and, from slice 0, the point's state
+// into buffers t & 1; launch t + 1 reads them (every workgroup of a point adds the
+// slices up, in slice order, and takes the same step) and writes buffers (t + 1) & 1.
+constexpr int kStateStride = 2048;   // bytes of one point's carried state
+constexpr int kPartStride = 272;     // doubles of one slice's partial sums: 3 sums, 6 bbox, 7 pad, 256 V
+struct SliceArgs {
+    const int *level_pts;            // points of the chain launch
+    int cnt, cnt8;                   // points; cnt rounded up to a multiple of 8
+    int nslice, per;                 // workgroups per point; samples per workgroup
+    int t;                           // launch index within the chain launch (0: set-up and start guess)
+    unsigned char *state[2];         // [cnt8][kStateStride] carried point state, by launch parity
+    double *part[2];                 // [cnt8][nslice][kPartStride] slice sums, by launch parity
+};
+
 struct StrainArgs {
     const double *pts, *disp, *objmin;
     const int *status, *neigh;
@@ -69,6 +88,11 @@ struct StrainArgs {
 // launchers implemented in dvc_kernels.hip
 int launch_search(const SearchArgs &a, const int *level_pts, int block_threads, int grid_blocks, size_t lds_bytes,
                   void *stream);
+// one slice launch with `threads` (kSliceThreads[]) per workgroup
+constexpr int kSliceThreads[2] = {256, 512};
+int launch_slice(const SearchArgs &a, const SliceArgs &sa, int threads, void *stream);
+// the points of rs.level_pts after rs.t slice launches, one 512-thread workgroup each
+int launch_resume(const SearchArgs &a, const SliceArgs &rs, void *stream);
 // the geometry a point's search uses (pgeo[n] applied, or a itself)
 int search_set_lds_limit(size_t lds_total);
 int search_static_lds(int bpv, int block_threads, size_t *bytes);

@@ -1,7 +1,9 @@
 // dvc_kernels.hip -- gfx950 kernels of the DVC per-point correlation path.
 //
-// Search kernel (one workgroup per ROI point, one launch per dependency level of the
-// serial point order):
+// Search kernels: one workgroup per ROI point and one launch per dependency level of the
+// serial point order (search_kernel), or, for narrow levels, one launch per Gauss-Newton
+// evaluation with every point's samples split over several workgroups (slice_kernel,
+// then resume_kernel):
 //   Search::process_point            Core/Search.cpp:95-192
 //   Search::search_pt_setup          Core/Search.cpp:248-275
 //   Search::starting_param           Core/Search.cpp:277-377
@@ -26,11 +28,12 @@
 //     v_j = [dt_j/dp (12), t_j - rbar, r_j - rbar, 1, 0]; each wave forms it on the
 //     matrix cores with v_mfma_f32_16x16x4_f32 (A = B = the same staged register),
 //     flushing the FP32 accumulator into FP64 every 64 samples.
-//   * Objective values are FP64 sums over the FP32 sample values kept in LDS; the
-//     12x12 solve (column-pivoted Householder QR, as Eigen's colPivHouseholderQr) and
-//     all point geometry (cloud, warp, bounding boxes, range checks) are FP64 with
-//     the reference's operation order and no fused multiply-add, so statuses follow
-//     the reference's comparisons.
+//   * Objective values are FP64 sums over the FP32 sample values; the 12x12 step solve
+//     (Cholesky or LDL^T of the normal equations, with column-pivoted Householder QR as
+//     Eigen's colPivHouseholderQr for (near-)singular systems) and all point geometry
+//     (cloud, warp, bounding boxes, range checks) are FP64 with the reference's
+//     operation order and no fused multiply-add, so statuses follow the reference's
+//     comparisons.
 #ifdef DVC_EMU
 #include "dvc_emu.h"
 #else
@@ -41,6 +44,7 @@
     __shared__ name##_t name
 #endif
 #include <math.h>
+#include <stddef.h>
 #include "dvc_internal.h"
 
 namespace dvc {
@@ -457,11 +461,13 @@ __device__ __forceinline__ float tap_eval(const VolView &v, const BoxView &bx, c
 
 // ------------------------------------------------------------- point state
 
+// State of one point's search, in LDS.  Everything before nb_sum is the point's state
+// between evaluations: slice launches keep it in HBM from one launch to the next.
 struct PtShared {
     int status, iters, fail, stop, pfail;
     int b0[3], box_ok;
-    int nb_status[kMaxNeigh];
-    double nb_par[kMaxNeigh][3];
+    int it, done, last, pad_;        // next Gauss-Newton iteration; search finished; last slice of a launch
+    double obj_old;                  // objective of the previous iteration (Search.cpp:217-230)
     double sp[3], sref[3], smin[3], inc[3], sbb_mn[3], sbb_mx[3];
     double act_mn[3], act_mx[3];
     double par[12], upd[12], evalpar[12];
@@ -475,7 +481,13 @@ struct PtShared {
     double obj, omin;
     int gmin[3];
     double sums[8];
+    double bbr[6];                   // moving-cloud bounding box of a sample pass (min xyz, max xyz)
+    // start guess inputs, not part of the carried state
+    double nb_sum[3];                // translations of the good earlier neighbours, summed
+    int nb_good;                     // their number
 };
+constexpr int kStateBytes = (int)offsetof(PtShared, nb_sum);
+static_assert(kStateBytes % 8 == 0, "carried point state is copied in 8-byte words");
 
 // Stable cloud sample j (FloatingCloud.cpp:84-128 cube grid, z outer / x inner;
 // FloatingCloud.cpp:71-76 sphere = centre + aspect*rel).
@@ -666,15 +678,18 @@ __device__ __forceinline__ void set_warp(PtShared &S, int ndof, const double tri
 
 // ------------------------------------------------------------- evaluation
 
-// One objective evaluation of the moving cloud at S.evalpar (first ndof entries):
-// Search::obj_val_at (Search.cpp:448-488).  With need_j the analytic-Jacobian moments
-// are left in V (16x16, FP64).  Returns false on Range_Fail.
-template <int BPV, int B>
-__device__ __forceinline__ bool evaluate(const SearchArgs &a, PtShared &S, int ndof, bool need_j,
-                         const float *rsrc, float *stage, double *red, double *V) {
-    constexpr int NW = B / 64;
-    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    const int n = a.nsamp;
+// One objective evaluation of the moving cloud at S.evalpar (first ndof entries),
+// Search::obj_val_at (Search.cpp:448-488), in three parts: eval_begin (warp and the
+// checks that need no samples), eval_samples (the sums over a range of samples, block
+// totals in S.sums[0..2], S.bbr and, with need_j, the Jacobian moments V) and eval_close
+// (the range rule of the moving bounding box, then the objective).  A workgroup runs all
+// three over every sample; slice launches split eval_samples over several workgroups.
+
+// Warp of S.evalpar and the checks before any sample: non-finite parameters and, with
+// 3 DOF, the range rule of the shifted stable box.  False (uniformly) on Range_Fail.
+template <int B>
+__device__ __forceinline__ bool eval_begin(const SearchArgs &a, PtShared &S, int ndof) {
+    const int tid = threadIdx.x, lane = tid & 63;
     if (tid < 64) {
         // cosine and sine of the three angles, one angle per lane of wavefront 0
         double sv = 0.0, cv = 1.0;
@@ -693,13 +708,12 @@ __device__ __forceinline__ bool evaluate(const SearchArgs &a, PtShared &S, int n
     __syncthreads();
     if (S.pfail) return false;
     // Range rule.  With 3 DOF the moving bounding box is the stable one shifted; with
-    // more it is accumulated in the sample pass below, which then runs speculatively:
-    // every sample is clamped into the interpolation box (inside the volume), and a
-    // failed range check discards the sums.
-    const bool merged = ndof > 3;
+    // more it is accumulated in the sample pass, which then runs speculatively: every
+    // sample is clamped into the interpolation box (inside the volume), and a failed
+    // range check discards the sums.
     if (tid == 0) {
 #pragma clang fp contract(off)
-        if (!merged) {
+        if (ndof <= 3) {
             for (int q = 0; q < 3; q++) { S.bb[q] = S.sbb_mn[q] + S.u[q]; S.bb[3 + q] = S.sbb_mx[q] + S.u[q]; }
             S.fail = range_ok(a, S, S.bb) ? 0 : 1;
         } else {
@@ -707,8 +721,22 @@ __device__ __forceinline__ bool evaluate(const SearchArgs &a, PtShared &S, int n
         }
     }
     __syncthreads();
-    if (S.fail) return false;
+    return S.fail == 0;
+}
 
+// Sums over a workgroup's samples of the moving cloud at the warp of S: block totals of
+// sum t, and of t^2 and t r (centred objectives) or (t - r)^2 and |t - r| (SSD, SAD) in
+// S.sums[0..2]; with more than 3 DOF the samples' bounding box in S.bbr; with need_j the
+// moments V = sum v v^T (16x16) of the block's samples.  Samples go to wavefronts in
+// chunks of 64 consecutive samples (one per lane): the workgroup takes chunks c0,
+// c0 + cstride, c0 + 2 cstride, ... (c0 = 0, cstride = 1: every sample, j = tid + it*B).
+template <int BPV, int B, int U = kInflight>
+__device__ __forceinline__ void eval_samples(const SearchArgs &a, PtShared &S, int ndof, bool need_j,
+                                             const float *rsrc, float *stage, double *red, double *V, int c0,
+                                             int cstride) {
+    constexpr int NW = B / 64;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const bool merged = ndof > 3;
     const VolView vv = {a.vol[1], a.W, (long long)a.W * a.H};
     const BoxView bx = {{S.b0[0], S.b0[1], S.b0[2]}, {a.box[0], a.box[1], a.box[2]}};
     const Gather gv = make_gather<BPV>(vv, bx, a.gather32 != 0);
@@ -735,13 +763,13 @@ __device__ __forceinline__ bool evaluate(const SearchArgs &a, PtShared &S, int n
         }
     };
     // accumulate one sample (every lane of the wave calls it; ok = false adds nothing)
-    auto accumulate = [&](bool ok, int j, float t, float gx, float gy, float gz, double xs, double ys, double zs) {
+    auto accumulate = [&](bool ok, float rc, float t, float gx, float gy, float gz, double xs, double ys,
+                          double zs) {
         float v[16];
 #pragma unroll
         for (int q = 0; q < 16; q++) v[q] = 0.0f;
         if (ok) {
             float tc = (float)((double)t - rbmean);
-            float rc = rsrc[j];
             const double tcd = (double)tc, rcd = (double)rc;
             s_tc += tcd;
             if (centred) {
@@ -805,13 +833,22 @@ __device__ __forceinline__ bool evaluate(const SearchArgs &a, PtShared &S, int n
         }
     };
 
-    const int steps = (n + B - 1) / B;
+    constexpr int NWv = B / 64;
+    const int n = a.nsamp;
+    const int nchunk = (n + 63) / 64;
+    const int mych = c0 < nchunk ? (nchunk - c0 + cstride - 1) / cstride : 0;   // this workgroup's chunks
+    const int steps = (mych + NWv - 1) / NWv;
+    // sample of this thread at step it (n or beyond: none)
+    auto sample_of = [&](int it) {
+        const int k = it * NWv + wid;
+        return k < mych ? (c0 + k * cstride) * 64 + lane : n;
+    };
+    const int j1 = n;
     if (method == DVC_INTERP_TRICUBIC) {
-        // kInflight samples per thread: all their row loads are issued before any is used
-        constexpr int U = kInflight;
+        // U samples per thread: all their row loads are issued before any is used
         SphOff pre[U];
 #pragma unroll
-        for (int u = 0; u < U; u++) pre[u] = sph_off(a, tid + u * B, tid + u * B < n);
+        for (int u = 0; u < U; u++) pre[u] = sph_off(a, sample_of(u), sample_of(u) < j1);
         for (int it = 0; it < steps; it += U) {
             double xs[U], ys[U], zs[U];
             Tap<BPV> t[U];
@@ -819,44 +856,52 @@ __device__ __forceinline__ bool evaluate(const SearchArgs &a, PtShared &S, int n
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 cur[u] = pre[u];
-                const int jn = tid + (it + U + u) * B;
-                pre[u] = sph_off(a, jn, jn < n);
+                const int jn = sample_of(it + U + u);
+                pre[u] = sph_off(a, jn, jn < j1);
             }
+            float rc[U];
 #pragma unroll
             for (int u = 0; u < U; u++) {
-                const int j = tid + (it + u) * B;
+                const int j = sample_of(it + u);
                 double x = 0.0, y = 0.0, z = 0.0;
                 xs[u] = ys[u] = zs[u] = 0.0;
-                if (j < n) position(j, cur[u], xs[u], ys[u], zs[u], x, y, z);
+                rc[u] = 0.0f;
+                if (j < j1) {
+                    rc[u] = rsrc[j];
+                    position(j, cur[u], xs[u], ys[u], zs[u], x, y, z);
+                }
                 tap_locate<BPV>(bx, x, y, z, t[u]);
+                // a lane without a sample reads interior rows and skips the face-cell
+                // path (its value is not used)
+                if (!(j < j1)) t[u].face = false;
             }
 #pragma unroll
             for (int u = 0; u < U; u++) tap_load<BPV>(vv, gv, bx, t[u]);
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 if (it + u < steps) {
-                    const int j = tid + (it + u) * B;
+                    const int j = sample_of(it + u);
                     float gx, gy, gz;
                     float tv = tap_eval<BPV>(vv, bx, t[u], fbias, jrow, gx, gy, gz);
-                    accumulate(j < n, j, tv, gx, gy, gz, xs[u], ys[u], zs[u]);
+                    accumulate(j < j1, rc[u], tv, gx, gy, gz, xs[u], ys[u], zs[u]);
                 }
             }
         }
     } else {
         for (int it = 0; it < steps; it++) {
-            const int j = tid + it * B;
-            const bool ok = j < n;
+            const int j = sample_of(it);
+            const bool ok = j < j1;
             double xs = 0.0, ys = 0.0, zs = 0.0, x = 0.0, y = 0.0, z = 0.0;
-            float t = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f;
+            float t = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f, rc = 0.0f;
             if (ok) {
+                rc = rsrc[j];
                 position(j, sph_off(a, j, true), xs, ys, zs, x, y, z);
                 t = interp_at<BPV>(vv, bx, x, y, z, method, fbias, jrow, gx, gy, gz);
             }
-            accumulate(ok, j, t, gx, gy, gz, xs, ys, zs);
+            accumulate(ok, rc, t, gx, gy, gz, xs, ys, zs);
         }
     }
-    // block reduction of the sums and (merged) of the moving bounding box, then the
-    // range rule (Interpolate.cpp:474-481 on the box of FloatingCloud::resize_bbox)
+    // block totals of the sums and of the moving bounding box
     {
         double v3[3] = {wave_sum(s_tc), wave_sum(s_x1), wave_sum(s_x2)};
         if (merged)
@@ -870,22 +915,16 @@ __device__ __forceinline__ bool evaluate(const SearchArgs &a, PtShared &S, int n
             double sacc = 0.0;
             for (int w = 0; w < NW; w++) sacc += red[w * 9 + tid];
             S.sums[tid] = sacc;
-        }
-        if (merged && tid == 0) {
-            for (int q = 0; q < 3; q++) {
-                double lo = red[3 + q], hi = red[6 + q];
-                for (int w = 1; w < NW; w++) { lo = fmin(lo, red[w * 9 + 3 + q]); hi = fmax(hi, red[w * 9 + 6 + q]); }
-                // resize_bbox sentinels (FloatingCloud.cpp:208-214)
-                S.bb[q] = fmin(lo, 9999.9);
-                S.bb[3 + q] = fmax(hi, -9999.9);
-            }
-            S.fail = range_ok(a, S, S.bb) ? 0 : 1;
+        } else if (tid < 9) {
+            const int q = tid - 3;
+            double v = red[3 + q];
+            for (int w = 1; w < NW; w++) v = q < 3 ? fmin(v, red[w * 9 + 3 + q]) : fmax(v, red[w * 9 + 3 + q]);
+            S.bbr[q] = v;
         }
     }
-    __syncthreads();
-    if (S.fail) return false;
     if (jrow) {
         double *st64 = (double *)stage;
+        __syncthreads();            // every wave is done with its staging tile
         // D row = 4*(lane/16) + r, column = lane%16 (symmetric, so either reading works)
 #pragma unroll
         for (int r = 0; r < 4; r++) st64[wid * 256 + (4 * (lane >> 4) + r) * 16 + (lane & 15)] = acc64[r];
@@ -895,15 +934,32 @@ __device__ __forceinline__ bool evaluate(const SearchArgs &a, PtShared &S, int n
             for (int w = 0; w < NW; w++) s += st64[w * 256 + e];
             V[e] = s;
         }
-        __syncthreads();
     }
-    // objective value and the moments the step needs (ObjectiveFunctions.cpp:24-263):
-    // the centred sums of the reference's second and third passes follow from the one
-    // pass above, e.g. sum (t - tm)^2 = sum t^2 - tm sum t, and the normalised
-    // objectives are (2 - 2 rho)/4 and (2 - 2 rho)/2 with rho the normalised correlation
+    __syncthreads();
+}
+
+// The range rule on the moving bounding box of the whole cloud (S.bbr; with more than
+// 3 DOF, Interpolate.cpp:474-481 on the box of FloatingCloud::resize_bbox), then the
+// objective value and the moments the step needs (ObjectiveFunctions.cpp:24-263) from
+// the sums in S.sums.  The centred sums of the reference's second and third passes
+// follow from the one pass, e.g. sum (t - tm)^2 = sum t^2 - tm sum t, and the normalised
+// objectives are (2 - 2 rho)/4 and (2 - 2 rho)/2 with rho the normalised correlation.
+// False (uniformly) on Range_Fail.
+__device__ __forceinline__ bool eval_close(const SearchArgs &a, PtShared &S, int ndof) {
+    const int tid = threadIdx.x;
+    if (ndof > 3 && tid == 0) {
+        for (int q = 0; q < 3; q++) {
+            // resize_bbox sentinels (FloatingCloud.cpp:208-214)
+            S.bb[q] = fmin(S.bbr[q], 9999.9);
+            S.bb[3 + q] = fmax(S.bbr[3 + q], -9999.9);
+        }
+        S.fail = range_ok(a, S, S.bb) ? 0 : 1;
+    }
+    __syncthreads();
+    if (S.fail) return false;
     if (tid == 0) {
         const int kind = a.obj;
-        const double nn = (double)n;
+        const double nn = (double)a.nsamp;
         const double st = S.sums[0], sx1 = S.sums[1], sx2 = S.sums[2];
         const double tm = st / nn;
         S.tmean = tm;
@@ -931,8 +987,60 @@ __device__ __forceinline__ bool evaluate(const SearchArgs &a, PtShared &S, int n
     return true;
 }
 
+// The whole evaluation on one workgroup.  With need_j the analytic-Jacobian moments are
+// left in V (16x16, FP64).  Returns false on Range_Fail.
+template <int BPV, int B>
+__device__ __forceinline__ bool evaluate(const SearchArgs &a, PtShared &S, int ndof, bool need_j,
+                                         const float *rsrc, float *stage, double *red, double *V) {
+    if (!eval_begin<B>(a, S, ndof)) return false;
+    eval_samples<BPV, B>(a, S, ndof, need_j, rsrc, stage, red, V, 0, 1);
+    return eval_close(a, S, ndof);
+}
+
 // ------------------------------------------------------------- step solve
 
+
+// One entry of the objective's Gauss-Newton normal equations (Search.cpp:234's system,
+// as LM_prep_at assembles it) from the block's moments V: (J^T J)_ij for j < N, the
+// right-hand side (-J^T e)_i for j == N.  The same arithmetic as normal_cols.
+__device__ __forceinline__ double normal_entry(const SearchArgs &a, const PtShared &S, const double *V, int i, int j,
+                                               int N) {
+#pragma clang fp contract(off)
+    const double n = (double)a.nsamp;
+    const int kind = a.obj;
+    const bool rhs = j == N;
+    const int jj = rhs ? 0 : j;
+    const double Sgj = V[jj * 16 + 14];
+    if (kind == DVC_OBJ_SAD || kind == DVC_OBJ_SSD) {
+        if (!rhs) return V[i * 16 + jj];
+        return -(V[i * 16 + 12] - V[i * 16 + 13]);
+    } else if (kind == DVC_OBJ_ZSSD) {
+        const double tm = S.tmean, Src = V[13 * 16 + 14];
+        if (!rhs) return V[i * 16 + jj] - V[i * 16 + 14] * Sgj / n;
+        const double Sg = V[i * 16 + 14];
+        return -((V[i * 16 + 12] - tm * Sg) - V[i * 16 + 13] + (Sg / n) * Src);
+    } else if (kind == DVC_OBJ_NSSD) {
+        const double T2 = S.T2, T = sqrt(T2), Rn = sqrt(S.Rn2), rm = S.rmean;
+        const double ci = V[i * 16 + 12] + rm * V[i * 16 + 14];
+        if (!rhs) {
+            const double cj = V[jj * 16 + 12] + rm * Sgj;
+            return V[i * 16 + jj] / T2 - ci * cj / (T2 * T2);
+        }
+        const double Sgr = V[i * 16 + 13] + rm * V[i * 16 + 14];
+        return -(-Sgr / (T * Rn) + ci * S.Strr / (T2 * T * Rn));
+    }
+    const double tm = S.tmean, N2 = S.N2, Nn = sqrt(N2), R = sqrt(S.R2);
+    const double ci = V[i * 16 + 12] - tm * V[i * 16 + 14];
+    if (!rhs) {
+        const double cj = V[jj * 16 + 12] - tm * Sgj;
+        return (V[i * 16 + jj] - V[i * 16 + 14] * Sgj / n) / N2 - ci * cj / (N2 * N2);
+    }
+    // sum_j (g_ij - mean g_i) (r_j - mean r): the FP32-centred reference samples sum to
+    // Src, not exactly zero
+    const double Sg = V[i * 16 + 14];
+    const double gr = V[i * 16 + 13] - Sg * S.Src / n;
+    return -(-gr / (Nn * R) + ci * S.Str / (N2 * Nn * R));
+}
 
 // Cholesky solve of the symmetric normal equations held one column per lane (lane j < N:
 // column j of A, lane N: right-hand side b), on wavefront 0.  Step k takes the pivot
@@ -980,6 +1088,112 @@ __device__ __forceinline__ bool chol_solve(double (&col)[N], int lane, double (&
         for (int q = i + 1; q < N; q++) sacc -= rdlane(col[q], i) * x[q];
         x[i] = sacc / lkk[i];
     }
+    return true;
+}
+
+// Reciprocals of the scale factors of the normal equations (normal_entry), once per step
+// for the uniform solve: its entries are then products, not quotients.
+struct NormalScale {
+    double inv_n, inv_a, inv_b, inv_c, inv_d;
+};
+
+__device__ __forceinline__ NormalScale normal_scale(const SearchArgs &a, const PtShared &S) {
+#pragma clang fp contract(off)
+    NormalScale f = {1.0 / (double)a.nsamp, 0.0, 0.0, 0.0, 0.0};
+    if (a.obj == DVC_OBJ_NSSD) {
+        const double T2 = S.T2, T = sqrt(T2), Rn = sqrt(S.Rn2);
+        f.inv_a = 1.0 / T2; f.inv_b = 1.0 / (T2 * T2); f.inv_c = 1.0 / (T * Rn); f.inv_d = 1.0 / (T2 * T * Rn);
+    } else if (a.obj == DVC_OBJ_ZNSSD) {
+        const double N2 = S.N2, Nn = sqrt(N2), R = sqrt(S.R2);
+        f.inv_a = 1.0 / N2; f.inv_b = 1.0 / (N2 * N2); f.inv_c = 1.0 / (Nn * R); f.inv_d = 1.0 / (N2 * Nn * R);
+    }
+    return f;
+}
+
+// normal_entry with the quotients replaced by products with the reciprocals in f.
+__device__ __forceinline__ double normal_entry_scaled(const SearchArgs &a, const PtShared &S, const double *V, int i,
+                                                      int j, int N, const NormalScale &f) {
+#pragma clang fp contract(off)
+    const int kind = a.obj;
+    const bool rhs = j == N;
+    const int jj = rhs ? 0 : j;
+    const double Sgj = V[jj * 16 + 14], Sg = V[i * 16 + 14];
+    if (kind == DVC_OBJ_SAD || kind == DVC_OBJ_SSD) {
+        if (!rhs) return V[i * 16 + jj];
+        return -(V[i * 16 + 12] - V[i * 16 + 13]);
+    } else if (kind == DVC_OBJ_ZSSD) {
+        const double tm = S.tmean, Src = V[13 * 16 + 14];
+        if (!rhs) return V[i * 16 + jj] - Sg * Sgj * f.inv_n;
+        return -((V[i * 16 + 12] - tm * Sg) - V[i * 16 + 13] + (Sg * f.inv_n) * Src);
+    } else if (kind == DVC_OBJ_NSSD) {
+        const double rm = S.rmean;
+        const double ci = V[i * 16 + 12] + rm * Sg;
+        if (!rhs) return V[i * 16 + jj] * f.inv_a - ci * (V[jj * 16 + 12] + rm * Sgj) * f.inv_b;
+        return -(-(V[i * 16 + 13] + rm * Sg) * f.inv_c + ci * S.Strr * f.inv_d);
+    }
+    const double tm = S.tmean;
+    const double ci = V[i * 16 + 12] - tm * Sg;
+    if (!rhs) return (V[i * 16 + jj] - Sg * Sgj * f.inv_n) * f.inv_a - ci * (V[jj * 16 + 12] - tm * Sgj) * f.inv_b;
+    const double gr = V[i * 16 + 13] - Sg * S.Src * f.inv_n;
+    return -(-gr * f.inv_c + ci * S.Str * f.inv_d);
+}
+
+// Solve of the symmetric normal equations A x = b by an LDL^T factorisation, the same on
+// every lane of the wave (A's lower triangle row by row, then b, in `e`: every lane holds
+// the whole system in registers, so no step exchanges values between lanes).  Step k
+// takes the pivot D_k = A_kk of the updated trailing matrix and its reciprocal, the
+// multipliers L_ik = A_ik / D_k and the update A_ij -= L_ik A_jk; the forward
+// substitution runs alongside, then x = L^-T (D^-1 y).  Each step has one division on its
+// dependency chain and no square root.  Returns false, having computed nothing usable,
+// when a pivot is not above 1e-9 of the largest diagonal entry (or is not finite): the
+// caller then solves by column-pivoted QR, whose rank rule covers the (near-)singular
+// systems.  (D_k is the square of the Cholesky pivot, so the test is the same.)
+// 1 / d for a positive finite d: the hardware reciprocal estimate and two Newton steps
+// r' = r + r (1 - d r), which carry its error below double rounding (within an ulp of
+// the quotient).  Shorter on the dependency chain than the division.
+__device__ __forceinline__ double fast_rcp(double d) {
+    double r = __builtin_amdgcn_rcp(d);
+    r = __builtin_fma(__builtin_fma(-d, r, 1.0), r, r);
+    r = __builtin_fma(__builtin_fma(-d, r, 1.0), r, r);
+    return r;
+}
+
+template <int N>
+__device__ __forceinline__ bool ldl_uniform(double (&e)[N * (N + 1) / 2 + N]) {
+#pragma clang fp contract(off)
+#define A_(i, j) e[(i) * ((i) + 1) / 2 + (j)]
+#define B_(i) e[N * (N + 1) / 2 + (i)]
+    double dmax = 0.0;
+#pragma unroll
+    for (int k = 0; k < N; k++) dmax = fmax(dmax, A_(k, k));
+    const double tol = 1e-9 * dmax;
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+        const double d = A_(k, k);
+        if (!(d > tol) || !(d < 1.0e300)) return false;
+        const double r = fast_rcp(d);
+        // rows from the bottom up: row i reads column k only at rows j <= i, which still
+        // hold A_jk when row i replaces its own entry by L_ik
+#pragma unroll
+        for (int i = N - 1; i > k; i--) {
+            const double l = A_(i, k) * r;
+#pragma unroll
+            for (int j = k + 1; j <= i; j++) A_(i, j) -= l * A_(j, k);
+            B_(i) -= l * B_(k);
+            A_(i, k) = l;
+        }
+        A_(k, k) = r;                 // D_k is not needed again: keep its reciprocal
+    }
+    // x = L^-T (D^-1 y), x_i over y_i in place
+#pragma unroll
+    for (int i = N - 1; i >= 0; i--) {
+        double sacc = B_(i) * A_(i, i);
+#pragma unroll
+        for (int q = i + 1; q < N; q++) sacc -= A_(q, i) * B_(q);
+        B_(i) = sacc;
+    }
+#undef A_
+#undef B_
     return true;
 }
 
@@ -1038,28 +1252,72 @@ __device__ __forceinline__ void normal_cols(const SearchArgs &a, const PtShared 
             if (lane == N)
 #pragma unroll
                 for (int i = 0; i < N; i++) {
-                    const double ci = V[i * 16 + 12] - tm * V[i * 16 + 14];
-                    col[i] = -(-V[i * 16 + 13] / (Nn * R) + ci * S.Str / (N2 * Nn * R));
+                    // sum_j (g_ij - mean g_i) (r_j - mean r): the FP32-centred reference
+                    // samples sum to Src, not exactly zero
+                    const double Sg = V[i * 16 + 14];
+                    const double ci = V[i * 16 + 12] - tm * Sg;
+                    const double gr = V[i * 16 + 13] - Sg * S.Src / n;
+                    col[i] = -(-gr / (Nn * R) + ci * S.Str / (N2 * Nn * R));
                 }
         }
     }
 }
 
 // One Gauss-Newton step on wavefront 0 (all 64 lanes call it): the normal equations
-// J^T J dx = -J^T e of the objective's residual, assembled from the moments in V
-// (lane j < N builds column j of J^T J, lane N builds -J^T e), then solved by
-// column-pivoted Householder QR (Eigen 3.4 ColPivHouseholderQR::computeInPlace and
-// _solve_impl, as used at Search.cpp:234).  Lane j keeps column j in registers; the
-// pivot scan, the reflector and the back substitution are uniform across lanes, and
-// every column is updated by its own lane with the same operation order as a serial
-// loop.  Adds the step to S.par and stores it in S.upd (Search.cpp:236-240).
-template <int N, bool CHOL = true>
-__device__ __forceinline__ void gn_step(const SearchArgs &a, PtShared &S, const double *V) {
+// J^T J dx = -J^T e of the objective's residual, assembled from the moments in V, then
+// solved by Cholesky with one column per lane (chol_solve), or with UNIFORM by LDL^T on
+// every lane at once (ldl_uniform; the lanes first build one entry each into the LDS
+// scratch `ne`, N (N + 1) / 2 + N doubles, then every lane reads the whole system): fewer
+// cross-lane steps, more registers.  When the factorisation rejects a pivot the step is
+// solved by column-pivoted Householder QR
+// (Eigen 3.4 ColPivHouseholderQR::computeInPlace and _solve_impl, as used at
+// Search.cpp:234): lane j keeps column j of J^T J in registers (lane N the right-hand
+// side); the pivot scan, the reflector and the back substitution are uniform across
+// lanes, and every column is updated by its own lane with the same operation order as a
+// serial loop.  Adds the step to S.par and stores it in S.upd (Search.cpp:236-240).
+template <int N, bool CHOL = true, bool UNIFORM = false>
+__device__ __forceinline__ void gn_step(const SearchArgs &a, PtShared &S, const double *V, double *ne) {
 #pragma clang fp contract(off)
     const int lane = threadIdx.x & 63;
+    if (CHOL && UNIFORM) {
+        constexpr int NE = N * (N + 1) / 2 + N;
+        const NormalScale f = normal_scale(a, S);
+        for (int q = lane; q < NE; q += 64) {
+            int i, j;
+            if (q < N * (N + 1) / 2) {
+                i = 0;
+                while ((i + 1) * (i + 2) / 2 <= q) i++;
+                j = q - i * (i + 1) / 2;
+            } else {
+                i = q - N * (N + 1) / 2;
+                j = N;
+            }
+            ne[q] = normal_entry_scaled(a, S, V, i, j, N, f);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        double e[NE];
+#pragma unroll
+        for (int q = 0; q < NE; q++) e[q] = ne[q];
+        const bool ok = ldl_uniform<N>(e);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (ok) {
+            if (lane == 0)
+#pragma unroll
+                for (int i = 0; i < N; i++) {
+                    const double x = e[N * (N + 1) / 2 + i];
+                    S.upd[i] = x;
+                    S.par[i] += x;
+                }
+            return;
+        }
+    }
     double col[N];
     normal_cols<N>(a, S, V, lane, col);
-    if (CHOL) {
+    if (CHOL && !UNIFORM) {
         double x[N];
         if (chol_solve<N>(col, lane, x)) {
 #pragma unroll
@@ -1190,12 +1448,11 @@ __device__ __forceinline__ void gn_step(const SearchArgs &a, PtShared &S, const 
 
 // Point set-up (Search.cpp:98-119, 252-258): cloud geometry, the reference box on the
 // point, its range rule and the integer bias.  Returns false (uniformly) on Range_Fail.
-template <int BPV, int B>
-__device__ __forceinline__ bool point_setup(const SearchArgs &a, int n, PtShared &S) {
+template <int BPV>
+__device__ __forceinline__ void point_setup_t0(const SearchArgs &a, int n, PtShared &S) {
 #pragma clang fp contract(off)
-    const int tid = threadIdx.x;
     // ---- cloud geometry and reference box (Search.cpp:98-119, 252-258)
-    if (tid == 0) {
+    {
 #pragma clang fp contract(off)
         S.status = DVC_POINT_GOOD; S.iters = 0; S.fail = 0; S.stop = 0;
         for (int q = 0; q < 12; q++) { S.par[q] = 0.0; S.upd[q] = 0.0; }
@@ -1230,8 +1487,32 @@ __device__ __forceinline__ bool point_setup(const SearchArgs &a, int n, PtShared
             S.fbias = voxel<BPV>(vr, (int)(S.sp[0] / 1), (int)(S.sp[1] / 1), (int)(S.sp[2] / 1));
         }
     }
+}
+
+template <int BPV, int B>
+__device__ __forceinline__ bool point_setup(const SearchArgs &a, int n, PtShared &S) {
+    if (threadIdx.x == 0) point_setup_t0<BPV>(a, n, S);
     __syncthreads();
     return S.fail == 0;
+}
+
+// The start guess's inputs (Search.cpp:277-377), by one wavefront: the translations of
+// the neighbours of point n (at position p) that come earlier in the order and are
+// good, summed (wavefront sums) into S.nb_sum, and their number into S.nb_good.
+__device__ __forceinline__ void gather_neighbours(const SearchArgs &a, int p, int n, PtShared &S) {
+    double sx = 0.0, sy = 0.0, sz = 0.0, good = 0.0;
+    for (int q = threadIdx.x & 63; q < a.k; q += 64) {
+        const int m = a.neigh[(long long)n * a.k + q];
+        if (m >= 0 && m < a.npts && a.pos[m] < p && a.nbres[m].status == DVC_POINT_GOOD) {
+            sx += a.nbres[m].par[0]; sy += a.nbres[m].par[1]; sz += a.nbres[m].par[2];
+            good += 1.0;
+        }
+    }
+    sx = wave_sum(sx); sy = wave_sum(sy); sz = wave_sum(sz); good = wave_sum(good);
+    if ((threadIdx.x & 63) == 0) {
+        S.nb_sum[0] = sx; S.nb_sum[1] = sy; S.nb_sum[2] = sz;
+        S.nb_good = (int)good;
+    }
 }
 
 // The reference sub-volume (Search.cpp:252-258): sample values at the stable cloud,
@@ -1312,29 +1593,36 @@ __device__ __forceinline__ void reference_pass(const SearchArgs &a, PtShared &S,
     }
 }
 
-template <int BPV, int B>
-__device__ __forceinline__ void point_search(const SearchArgs &a, int p, int n, PtShared &S, float *rcb, float *stage,
-                             double *red, double *V) {
+// Search::process_point up to its Gauss-Newton loop (Search.cpp:98-119, 248-377,
+// 934-1043): set-up, reference samples, start guess and (GRID) the translation grid.
+// rsrc receives the point's reference samples.  False (uniformly) when the point
+// failed; S.status then holds its status.
+template <int BPV, int B, bool GRID>
+__device__ __forceinline__ bool point_start(const SearchArgs &a, int p, int n, PtShared &S, float *rcb, float *stage,
+                                            double *red, double *V, const float *&rsrc) {
 #pragma clang fp contract(off)
-    const int tid = threadIdx.x, lane = tid & 63;
-    const int ns = a.nsamp;
-    if (!point_setup<BPV, B>(a, n, S)) {
+    const int tid = threadIdx.x, wid = tid >> 6;
+    // independent loads side by side: the point's set-up (thread 0), the neighbours'
+    // results for the start guess (one wavefront) and the reference-sample sums
+    constexpr int kGatherWave = B >= 128 ? 1 : 0;
+    constexpr int kStatThread = B >= 192 ? 128 : 0;
+    if (tid == 0) point_setup_t0<BPV>(a, n, S);
+    if (!a.start && wid == kGatherWave) gather_neighbours(a, p, n, S);
+    if (a.rvals && tid == kStatThread) {
+        const RefStats &st = a.rstat[p];
+        S.rmean = st.rmean; S.rbmean = st.rbmean;
+        S.R2 = st.R2; S.Src = st.Src; S.Rn2 = st.Rn2;
+    }
+    __syncthreads();
+    if (S.fail) {
         if (tid == 0) S.status = DVC_RANGE_FAIL;
-        return;
+        return false;
     }
     // reference samples: computed for every point before the levels run
     // (reference_kernel) and read from HBM, or computed here into LDS
-    const float *rsrc = rcb;
-    if (a.rvals) {
-        rsrc = a.rvals + (size_t)p * a.nsamp_pad;
-        if (tid == 0) {
-            const RefStats &st = a.rstat[p];
-            S.rmean = st.rmean; S.rbmean = st.rbmean;
-            S.R2 = st.R2; S.Src = st.Src; S.Rn2 = st.Rn2;
-        }
-    } else {
-        reference_pass<BPV, B>(a, S, rcb, red);
-    }
+    rsrc = rcb;
+    if (a.rvals) rsrc = a.rvals + (size_t)p * a.nsamp_pad;
+    else reference_pass<BPV, B>(a, S, rcb, red);
     if (a.start) {
         // ---- a given start warp (proposal pass): all ndof parameters, no neighbours
         if (tid == 0) {
@@ -1350,35 +1638,17 @@ __device__ __forceinline__ void point_search(const SearchArgs &a, int p, int n, 
         __syncthreads();
         if (S.fail) {
             if (tid == 0) S.status = DVC_RANGE_FAIL;
-            return;
+            return false;
         }
     } else {
-        // ---- starting_param: good neighbours at earlier positions of the order.  In
-        // the serial chain (nbres == res) launches run the dependency levels in order, so
-        // every such neighbour's result is already in memory; a judge pass reads them
-        // from the finished table of a proposal pass.
-        if (tid < 64) {
-            for (int q = lane; q < a.k; q += 64) {
-                int m = a.neigh[(long long)n * a.k + q];
-                int st = DVC_NOT_SEARCH;
-                double px = 0.0, py = 0.0, pz = 0.0;
-                if (m >= 0 && m < a.npts && a.pos[m] < p) {
-                    st = a.nbres[m].status;
-                    px = a.nbres[m].par[0]; py = a.nbres[m].par[1]; pz = a.nbres[m].par[2];
-                }
-                S.nb_status[q] = st;
-                S.nb_par[q][0] = px; S.nb_par[q][1] = py; S.nb_par[q][2] = pz;
-            }
-        }
-        __syncthreads();
+        // ---- starting_param: good neighbours at earlier positions of the order
+        // (gathered above).  In the serial chain (nbres == res) launches run the
+        // dependency levels in order, so every such neighbour's result is already in
+        // memory; a judge pass reads them from the finished table of a proposal pass.
         if (tid == 0) {
 #pragma clang fp contract(off)
-            double sx = 0.0, sy = 0.0, sz = 0.0;
-            int good = 0;
-            for (int q = 0; q < a.k; q++)
-                if (S.nb_status[q] == DVC_POINT_GOOD) {
-                    sx += S.nb_par[q][0]; sy += S.nb_par[q][1]; sz += S.nb_par[q][2]; good += 1;
-                }
+            double sx = S.nb_sum[0], sy = S.nb_sum[1], sz = S.nb_sum[2];
+            const int good = S.nb_good;
             if (good != 0) { sx /= good; sy /= good; sz /= good; }
             if (good != 0) { S.par[0] = sx; S.par[1] = sy; S.par[2] = sz; }
             else { S.par[0] = a.rigid[0]; S.par[1] = a.rigid[1]; S.par[2] = a.rigid[2]; }
@@ -1389,7 +1659,7 @@ __device__ __forceinline__ void point_search(const SearchArgs &a, int p, int n, 
         __syncthreads();
     }
     // ---- translation grid (Search.cpp:934-1043)
-    if (a.basin != 0.0) {
+    if (GRID && a.basin != 0.0) {
         const int ni = (int)(a.disp_max / a.basin);
         const double inc = a.disp_max / ni;
         if (tid == 0) { S.omin = 1.7976931348623157e308; S.gmin[0] = S.gmin[1] = S.gmin[2] = 0; }
@@ -1405,7 +1675,7 @@ __device__ __forceinline__ void point_search(const SearchArgs &a, int p, int n, 
                     __syncthreads();
                     if (!evaluate<BPV, B>(a, S, 3, false, rsrc, stage, red, V)) {
                         if (tid == 0) S.status = DVC_RANGE_FAIL;
-                        return;
+                        return false;
                     }
                     if (tid == 0 && S.obj < S.omin) { S.omin = S.obj; S.gmin[0] = x; S.gmin[1] = y; S.gmin[2] = z; }
                     __syncthreads();
@@ -1418,11 +1688,20 @@ __device__ __forceinline__ void point_search(const SearchArgs &a, int p, int n, 
         }
         __syncthreads();
     }
-    // ---- Gauss-Newton (Search.cpp:194-246)
+    return true;
+}
+
+// Gauss-Newton from iteration i0 (Search.cpp:194-246; obj_old: the objective of
+// iteration i0 - 1), then the final objective when the iteration limit ends the loop
+// (Search.cpp:187).  S.status is set on a range failure.
+template <int BPV, int B>
+__device__ __forceinline__ void gn_iterations(const SearchArgs &a, PtShared &S, const float *rsrc, float *stage,
+                                              double *red, double *V, int i0, double obj_old) {
+#pragma clang fp contract(off)
+    const int tid = threadIdx.x;
     const int nd = a.ndof;
-    double obj_old = 0.0;
     bool converged = false;
-    for (int i = 0; i < 20; i++) {
+    for (int i = i0; i < 20; i++) {
         if (tid == 0) {
             for (int q = 0; q < 12; q++) S.evalpar[q] = S.par[q];
             S.iters += 1;                 // LM_prep_at calls, counted when attempted
@@ -1444,9 +1723,10 @@ __device__ __forceinline__ void point_search(const SearchArgs &a, int p, int n, 
                 if ((del_obj < 0.000001) || small_step) stop = true;
             }
             if (!stop) {
-                if (nd == 3) gn_step<3>(a, S, V);
-                else if (nd == 6) gn_step<6>(a, S, V);
-                else gn_step<12>(a, S, V);
+                double *ne = (double *)stage;
+                if (nd == 3) gn_step<3>(a, S, V, ne);
+                else if (nd == 6) gn_step<6>(a, S, V, ne);
+                else gn_step<12>(a, S, V, ne);
             }
             if (tid == 0) S.stop = stop ? 1 : 0;
         }
@@ -1464,6 +1744,14 @@ __device__ __forceinline__ void point_search(const SearchArgs &a, int p, int n, 
             return;
         }
     }
+}
+
+template <int BPV, int B>
+__device__ __forceinline__ void point_search(const SearchArgs &a, int p, int n, PtShared &S, float *rcb, float *stage,
+                                             double *red, double *V) {
+    const float *rsrc = nullptr;
+    if (point_start<BPV, B, true>(a, p, n, S, rcb, stage, red, V, rsrc))
+        gn_iterations<BPV, B>(a, S, rsrc, stage, red, V, 0, 0.0);
 }
 
 // Per-point geometry (PtGeo of point n) applied over the launch's arguments, by thread 0
@@ -1519,6 +1807,114 @@ __global__ void __launch_bounds__(B) reference_kernel(SearchArgs a) {
     }
 }
 
+// Ends evaluation S.it of a point whose samples ran as slices: adds the slices' sums up
+// (sp: the point's nslice partial records in HBM, in slice order), applies the range rule
+// and the objective (eval_close), the stopping test of Search.cpp:217-230 and, when it
+// does not stop, the step.  Every workgroup of the point runs it on the same inputs, so
+// all agree.  Returns 0 after a step (S.it and S.obj_old advanced), 1 when the search
+// stopped (S holds its result), -1 on Range_Fail (S.status set).
+// Slice sums prefetched per thread: entry tid of the first kPrefetch slices' records.
+constexpr int kPrefetch = 16;
+template <int B>
+__device__ __forceinline__ void prefetch_slices(const double *sp, int nslice, double (&pre)[kPrefetch]) {
+    const int e = threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < kPrefetch; q++) pre[q] = (e < kPartStride && q < nslice) ? sp[(size_t)q * kPartStride + e] : 0.0;
+}
+
+template <int B, bool UNIFORM>
+__device__ __forceinline__ int finish_sliced(const SearchArgs &a, PtShared &S, const double *sp, int nslice,
+                                             double *V, float *stage, const double (&pre)[kPrefetch]) {
+#pragma clang fp contract(off)
+    const int tid = threadIdx.x;
+    const int i = S.it, nd = a.ndof;
+    const double del_mag = sqrt(S.upd[0] * S.upd[0] + S.upd[1] * S.upd[1] + S.upd[2] * S.upd[2]);
+    const bool small_step = i > 0 && del_mag < 0.01;
+    // entry e of every slice record, added up in slice order (min / max for the bounding
+    // box); thread e's own entry of the first kPrefetch slices is already in `pre`
+    for (int e = tid; e < kPartStride; e += B) {
+        if ((e >= 9 && e < 16) || (e >= 16 && small_step)) continue;
+        const int op = e < 3 || e >= 16 ? 0 : (e < 6 ? -1 : 1);
+        auto fold = [&](double acc, double w, int q) {
+            if (q == 0) return w;
+            return op == 0 ? acc + w : (op < 0 ? fmin(acc, w) : fmax(acc, w));
+        };
+        double acc = 0.0;
+        int q0 = 0;
+        if (e == tid) {
+#pragma unroll
+            for (int q = 0; q < kPrefetch; q++)
+                if (q < nslice) acc = fold(acc, pre[q], q);
+            q0 = kPrefetch;
+        }
+        for (; q0 < nslice; q0 += 8) {
+            double w[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) w[u] = q0 + u < nslice ? sp[(size_t)(q0 + u) * kPartStride + e] : 0.0;
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+                if (q0 + u < nslice) acc = fold(acc, w[u], q0 + u);
+        }
+        if (e < 3) S.sums[e] = acc;
+        else if (e < 9) S.bbr[e - 3] = acc;
+        else V[e - 16] = acc;
+    }
+    __syncthreads();
+    if (!eval_close(a, S, nd)) {
+        if (tid == 0) S.status = DVC_RANGE_FAIL;
+        return -1;
+    }
+    if (tid < 64) {
+        bool stop = false;
+        if (i > 0) {
+            const double del_obj = fabs(S.obj - S.obj_old);
+            if ((del_obj < 0.000001) || small_step) stop = true;
+        }
+        if (!stop) {
+            double *ne = (double *)stage;
+            if (nd == 3) gn_step<3, true, UNIFORM>(a, S, V, ne);
+            else if (nd == 6) gn_step<6, true, UNIFORM>(a, S, V, ne);
+            else gn_step<12, true, UNIFORM>(a, S, V, ne);
+        }
+        if (tid == 0) {
+            S.stop = stop ? 1 : 0;
+            if (!stop) { S.obj_old = S.obj; S.it = i + 1; }
+        }
+    }
+    __syncthreads();
+    return S.stop ? 1 : 0;
+}
+
+// Result record of a finished point (dvc.cpp:263-283 records only the status of a failed
+// point).  Thread 0.
+__device__ __forceinline__ void write_result(const SearchArgs &a, const PtShared &S, int n) {
+    dvc_point_result *r = &a.res[n];
+    r->status = S.status;
+    r->iters = S.iters;
+    if (S.status == DVC_POINT_GOOD) {
+        r->obj = S.obj;
+        for (int q = 0; q < 12; q++) r->par[q] = (q < a.ndof) ? S.par[q] : 0.0;
+    } else {
+        r->obj = 0.0;
+        for (int q = 0; q < 12; q++) r->par[q] = 0.0;
+    }
+}
+
+// Carried point state between launches: the head of PtShared (kStateBytes), copied by
+// every thread of the workgroup in 8-byte words.  Callers synchronise around the copies.
+template <int B>
+__device__ __forceinline__ void state_load(PtShared &S, const unsigned char *g) {
+    const unsigned long long *src = (const unsigned long long *)g;
+    unsigned long long *dst = (unsigned long long *)&S;
+    for (int w = threadIdx.x; w < kStateBytes / 8; w += B) dst[w] = src[w];
+}
+template <int B>
+__device__ __forceinline__ void state_store(const PtShared &S, unsigned char *g) {
+    const unsigned long long *src = (const unsigned long long *)&S;
+    unsigned long long *dst = (unsigned long long *)g;
+    for (int w = threadIdx.x; w < kStateBytes / 8; w += B) dst[w] = src[w];
+}
+
 // One dependency level of the point loop: workgroup b searches point level_pts[b].
 // All neighbours a point depends on belong to earlier levels (earlier launches).
 template <int BPV, int B, bool UNEVEN>
@@ -1542,29 +1938,139 @@ __global__ void __launch_bounds__(B) search_kernel(SearchArgs a, const int *leve
         point_search<BPV, B>(a, p, n, S, rcb, stage, red, V);
     }
     __syncthreads();
-    if (tid == 0) {
-        dvc_point_result *r = &a.res[n];
-        r->status = S.status;
-        r->iters = S.iters;
-        if (S.status == DVC_POINT_GOOD) {
-            r->obj = S.obj;
-            for (int q = 0; q < 12; q++) r->par[q] = (q < a.ndof) ? S.par[q] : 0.0;
-        } else {
-            // dvc.cpp:268-283 records only the status of a failed point
-            r->obj = 0.0;
-            for (int q = 0; q < 12; q++) r->par[q] = 0.0;
+    if (tid == 0) write_result(a, S, n);
+}
+
+// The points of a chain launch after its slice launches (rs.t of them): workgroup b
+// continues point level_pts[b] from the state and slice sums that slice launch rs.t - 1
+// left.  Finished points are skipped; the others end their sliced evaluation and run
+// their remaining Gauss-Newton iterations here, one workgroup per point.
+template <int BPV, int B>
+__global__ void __launch_bounds__(B) resume_kernel(SearchArgs a, const int *level_pts, SliceArgs rs) {
+    DVC_DYN_LDS(float, dyn);
+    float *stage = dyn;
+    DVC_SHARED(PtShared, S);
+    DVC_SHARED(double[(B / 64) * 9], red);
+    DVC_SHARED(double[256], V);
+    const int n = level_pts[blockIdx.x];
+    const int p = a.pos[n];
+    const int par = (rs.t - 1) & 1;
+    const double *sp = rs.part[par] + (size_t)blockIdx.x * rs.nslice * kPartStride;
+    double pre[kPrefetch];
+    prefetch_slices<B>(sp, rs.nslice, pre);
+    state_load<B>(S, rs.state[par] + (size_t)blockIdx.x * kStateStride);
+    __syncthreads();
+    if (S.done) return;
+    const int r = finish_sliced<B, false>(a, S, sp, rs.nslice, V, stage, pre);
+    if (r == 0) gn_iterations<BPV, B>(a, S, a.rvals + (size_t)p * a.nsamp_pad, stage, red, V, S.it, S.obj_old);
+    __syncthreads();
+    if (threadIdx.x == 0) write_result(a, S, n);
+}
+
+// One slice launch (SliceArgs) of the points of a chain launch.  Launch t ends evaluation
+// t - 1 of every point still searching (finish_sliced, on the slice sums and state that
+// launch t - 1 left; at t = 0 instead the point's set-up and start guess), then runs the
+// set-up of evaluation t and its own slice of the samples, and writes the slice's sums;
+// slice 0 also writes the point's state.  A point whose search ended writes its result
+// (slice 0) and carries its finished state forward.  Every workgroup of a point computes
+// the same set-up, sums and step from the same inputs, so all agree.
+template <int BPV, int BS>
+__global__ void __launch_bounds__(BS) slice_kernel(SearchArgs a, SliceArgs sa) {
+    DVC_DYN_LDS(float, dyn);
+    float *stage = dyn;
+    DVC_SHARED(PtShared, S);
+    DVC_SHARED(double[(BS / 64) * 9], red);
+    DVC_SHARED(double[256], V);
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int slot = (int)blockIdx.x % sa.cnt8, slice = (int)blockIdx.x / sa.cnt8;
+    if (slot >= sa.cnt) return;
+    const int n = sa.level_pts[slot];
+    const int p = a.pos[n];
+    const int cur = sa.t & 1, prev = cur ^ 1;
+    unsigned char *gst = sa.state[cur] + (size_t)slot * kStateStride;
+    const float *rsrc = a.rvals + (size_t)p * a.nsamp_pad;
+    // the point's search has ended: its result, and its state for the later launches
+    auto finished = [&]() {
+        if (slice != 0) return;
+        if (tid == 0) { S.done = 1; write_result(a, S, n); }
+        __syncthreads();
+        state_store<BS>(S, gst);
+    };
+    if (sa.t == 0) {
+        const float *rs = nullptr;
+        if (!point_start<BPV, BS, false>(a, p, n, S, nullptr, stage, red, V, rs)) {
+            finished();
+            return;
+        }
+        if (tid == 0) { S.it = 0; S.obj_old = 0.0; S.done = 0; }
+    } else {
+        // the slice sums are loaded while the state is: both come from launch t - 1
+        const double *sp = sa.part[prev] + (size_t)slot * sa.nslice * kPartStride;
+        double pre[kPrefetch];
+        prefetch_slices<BS>(sp, sa.nslice, pre);
+        state_load<BS>(S, sa.state[prev] + (size_t)slot * kStateStride);
+        __syncthreads();
+        if (S.done) {
+            if (slice == 0) state_store<BS>(S, gst);
+            return;
+        }
+        const int r = finish_sliced<BS, true>(a, S, sp, sa.nslice, V, stage, pre);
+        if (r != 0) {
+            finished();
+            return;
         }
     }
+    // ---- evaluation S.it (Search.cpp:194-246): set-up, then this workgroup's samples
+    if (tid == 0) {
+        for (int q = 0; q < 12; q++) S.evalpar[q] = S.par[q];
+        S.iters += 1;
+    }
+    __syncthreads();
+    const int i = S.it, nd = a.ndof;
+    const double del_mag = sqrt(S.upd[0] * S.upd[0] + S.upd[1] * S.upd[1] + S.upd[2] * S.upd[2]);
+    const bool need_j = !(i > 0 && del_mag < 0.01);
+    if (!eval_begin<BS>(a, S, nd)) {
+        if (tid == 0) S.status = DVC_RANGE_FAIL;
+        finished();
+        return;
+    }
+    // chunks of 64 samples dealt to the slices in turn: every slice gets parts of the
+    // whole cloud, so slices cost the same where rows of the cloud are short (its caps)
+    eval_samples<BPV, BS, 1>(a, S, nd, need_j, rsrc, stage, red, V, slice, sa.nslice);
+    if (tid < 64) {
+        double *gp = sa.part[cur] + ((size_t)slot * sa.nslice + slice) * kPartStride;
+        const int ne = need_j ? kPartStride : 16;
+        for (int e = lane; e < ne; e += 64)
+            gp[e] = e < 3 ? S.sums[e] : (e < 9 ? S.bbr[e - 3] : (e < 16 ? 0.0 : V[e - 16]));
+    }
+    if (slice == 0) state_store<BS>(S, gst);
 }
 
 #ifndef DVC_EMU
-size_t search_lds_bytes(int nsamp_pad, int block_threads, bool samples_in_lds) {
-    return (samples_in_lds ? (size_t)nsamp_pad * sizeof(float) : 0) +
-           (size_t)(block_threads / 64) * 64 * 17 * sizeof(float);
-}
+// ---------------------------------------------------- launchers per instantiation
+
+// The search and reference kernels of one (bytes per voxel, workgroup size) pair.  The
+// library compiles this file in several translation units at once: with DVC_PART = k > 0
+// a unit instantiates only the pairs of part k (below); DVC_PART = 0 holds everything
+// else and refers to those instantiations.
+template <int BPV, int B>
+struct Inst {
+    static int launch_search(const SearchArgs &a, const int *level_pts, int grid, size_t lds, hipStream_t st);
+    static int lds_limit(size_t lds_total);
+    static int launch_reference(const SearchArgs &a, hipStream_t st);
+    static int occupancy(size_t lds, int *bpc);
+    static int static_lds(size_t *bytes);
+};
+
+template <int BPV, int BS>
+struct SliceInst {
+    static int launch(const SearchArgs &a, const SliceArgs &sa, hipStream_t st);
+    // resume_kernel of BS threads per point
+    static int resume(const SearchArgs &a, const SliceArgs &rs, hipStream_t st);
+};
 
 template <int BPV, int B>
-static int launch_search_t(const SearchArgs &a, const int *level_pts, int grid, size_t lds, hipStream_t st) {
+int Inst<BPV, B>::launch_search(const SearchArgs &a, const int *level_pts, int grid, size_t lds, hipStream_t st) {
     if (a.pgeo) hipLaunchKernelGGL((search_kernel<BPV, B, true>), dim3(grid), dim3(B), lds, st, a, level_pts);
     else hipLaunchKernelGGL((search_kernel<BPV, B, false>), dim3(grid), dim3(B), lds, st, a, level_pts);
     return hipGetLastError() == hipSuccess ? 0 : -1;
@@ -1574,7 +2080,7 @@ static int launch_search_t(const SearchArgs &a, const int *level_pts, int grid, 
 // of `lds_total`.  Done once per context, outside any stream capture, so that launches
 // (direct or replayed from a graph) never change function attributes.
 template <int BPV, int B>
-static int lds_limit_t(size_t lds_total, int *) {
+int Inst<BPV, B>::lds_limit(size_t lds_total) {
     const void *ks[4] = {(const void *)search_kernel<BPV, B, false>, (const void *)reference_kernel<BPV, B, false>,
                          (const void *)search_kernel<BPV, B, true>, (const void *)reference_kernel<BPV, B, true>};
     for (const void *k : ks) {
@@ -1587,41 +2093,22 @@ static int lds_limit_t(size_t lds_total, int *) {
     return 0;
 }
 
+// reference_kernel uses static LDS only
 template <int BPV, int B>
-static int launch_reference_t(const SearchArgs &a, size_t lds, hipStream_t st) {
-    (void)lds;
+int Inst<BPV, B>::launch_reference(const SearchArgs &a, hipStream_t st) {
     if (a.pgeo) hipLaunchKernelGGL((reference_kernel<BPV, B, true>), dim3(a.nproc), dim3(B), 0, st, a);
     else hipLaunchKernelGGL((reference_kernel<BPV, B, false>), dim3(a.nproc), dim3(B), 0, st, a);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 template <int BPV, int B>
-static int occ_t(size_t lds, int *bpc) {
+int Inst<BPV, B>::occupancy(size_t lds, int *bpc) {
     auto k = search_kernel<BPV, B, false>;
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(bpc, (const void *)k, B, lds) == hipSuccess ? 0 : -1;
 }
 
-#define DVC_DISPATCH(BPV_, B_, CALL)                                  \
-    if (bpv == BPV_ && block_threads == B_) return CALL<BPV_, B_>
-
-int launch_search(const SearchArgs &a, const int *level_pts, int block_threads, int grid, size_t lds, void *stream) {
-    hipStream_t st = (hipStream_t)stream;
-    int bpv = a.bpv;
-    DVC_DISPATCH(1, 64, launch_search_t)(a, level_pts, grid, lds, st);
-    DVC_DISPATCH(1, 128, launch_search_t)(a, level_pts, grid, lds, st);
-    DVC_DISPATCH(1, 256, launch_search_t)(a, level_pts, grid, lds, st);
-    DVC_DISPATCH(1, 512, launch_search_t)(a, level_pts, grid, lds, st);
-    DVC_DISPATCH(1, 1024, launch_search_t)(a, level_pts, grid, lds, st);
-    DVC_DISPATCH(2, 64, launch_search_t)(a, level_pts, grid, lds, st);
-    DVC_DISPATCH(2, 128, launch_search_t)(a, level_pts, grid, lds, st);
-    DVC_DISPATCH(2, 256, launch_search_t)(a, level_pts, grid, lds, st);
-    DVC_DISPATCH(2, 512, launch_search_t)(a, level_pts, grid, lds, st);
-    DVC_DISPATCH(2, 1024, launch_search_t)(a, level_pts, grid, lds, st);
-    return -1;
-}
-
 template <int BPV, int B>
-static int static_lds_t(size_t *bytes, int *) {
+int Inst<BPV, B>::static_lds(size_t *bytes) {
     hipFuncAttributes fa;
     // the per-point-geometry instantiation holds one more shared SearchArgs: the larger
     if (hipFuncGetAttributes(&fa, (const void *)search_kernel<BPV, B, true>) != hipSuccess) return -1;
@@ -1629,73 +2116,134 @@ static int static_lds_t(size_t *bytes, int *) {
     return 0;
 }
 
+template <int BPV, int BS>
+int SliceInst<BPV, BS>::launch(const SearchArgs &a, const SliceArgs &sa, hipStream_t st) {
+    const size_t lds = (size_t)(BS / 64) * 64 * 17 * sizeof(float);
+    const unsigned grid = (unsigned)sa.cnt8 * (unsigned)sa.nslice;
+    hipLaunchKernelGGL((slice_kernel<BPV, BS>), dim3(grid), dim3(BS), lds, st, a, sa);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+template <int BPV, int BS>
+int SliceInst<BPV, BS>::resume(const SearchArgs &a, const SliceArgs &rs, hipStream_t st) {
+    const size_t lds = (size_t)(BS / 64) * 64 * 17 * sizeof(float);
+    hipLaunchKernelGGL((resume_kernel<BPV, BS>), dim3(rs.cnt), dim3(BS), lds, st, a, rs.level_pts, rs);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+#define DVC_INST_LIST(X) X(1, 64) X(1, 128) X(1, 256) X(1, 512) X(1, 1024) X(2, 64) X(2, 128) X(2, 256) X(2, 512) X(2, 1024)
+
+#if defined(DVC_PART) && DVC_PART > 0
+#if DVC_PART == 1
+template struct Inst<1, 64>;
+template struct Inst<1, 128>;
+template struct Inst<1, 256>;
+#elif DVC_PART == 2
+template struct Inst<1, 512>;
+template struct Inst<1, 1024>;
+#elif DVC_PART == 3
+template struct Inst<2, 64>;
+template struct Inst<2, 128>;
+template struct Inst<2, 256>;
+#elif DVC_PART == 4
+template struct Inst<2, 512>;
+template struct Inst<2, 1024>;
+#elif DVC_PART == 5
+template struct SliceInst<1, 256>;
+template struct SliceInst<2, 256>;
+template struct SliceInst<1, 512>;
+template struct SliceInst<2, 512>;
+#endif
+#else  // the main unit
+#ifdef DVC_PART
+#define DVC_EXTERN(BPV_, B_) extern template struct Inst<BPV_, B_>;
+DVC_INST_LIST(DVC_EXTERN)
+extern template struct SliceInst<1, 256>;
+extern template struct SliceInst<2, 256>;
+extern template struct SliceInst<1, 512>;
+extern template struct SliceInst<2, 512>;
+#undef DVC_EXTERN
+#endif
+
+size_t search_lds_bytes(int nsamp_pad, int block_threads, bool samples_in_lds) {
+    return (samples_in_lds ? (size_t)nsamp_pad * sizeof(float) : 0) +
+           (size_t)(block_threads / 64) * 64 * 17 * sizeof(float);
+}
+
+#define DVC_DISPATCH(BPV_, B_, FN, ...) \
+    if (bpv == BPV_ && block_threads == B_) return Inst<BPV_, B_>::FN(__VA_ARGS__);
+
+int launch_search(const SearchArgs &a, const int *level_pts, int block_threads, int grid, size_t lds, void *stream) {
+    hipStream_t st = (hipStream_t)stream;
+    const int bpv = a.bpv;
+#define X(BPV_, B_) DVC_DISPATCH(BPV_, B_, launch_search, a, level_pts, grid, lds, st)
+    DVC_INST_LIST(X)
+#undef X
+    return -1;
+}
+
+int launch_slice(const SearchArgs &a, const SliceArgs &sa, int threads, void *stream) {
+    if (sa.cnt <= 0) return 0;
+    hipStream_t st = (hipStream_t)stream;
+    if (a.bpv == 1 && threads == 256) return SliceInst<1, 256>::launch(a, sa, st);
+    if (a.bpv == 2 && threads == 256) return SliceInst<2, 256>::launch(a, sa, st);
+    if (a.bpv == 1 && threads == 512) return SliceInst<1, 512>::launch(a, sa, st);
+    if (a.bpv == 2 && threads == 512) return SliceInst<2, 512>::launch(a, sa, st);
+    return -1;
+}
+
+int launch_resume(const SearchArgs &a, const SliceArgs &rs, void *stream) {
+    if (rs.cnt <= 0) return 0;
+    hipStream_t st = (hipStream_t)stream;
+    if (a.bpv == 1) return SliceInst<1, 512>::resume(a, rs, st);
+    if (a.bpv == 2) return SliceInst<2, 512>::resume(a, rs, st);
+    return -1;
+}
+
 int search_static_lds(int bpv, int block_threads, size_t *bytes) {
-    DVC_DISPATCH(1, 64, static_lds_t)(bytes, nullptr);
-    DVC_DISPATCH(1, 128, static_lds_t)(bytes, nullptr);
-    DVC_DISPATCH(1, 256, static_lds_t)(bytes, nullptr);
-    DVC_DISPATCH(1, 512, static_lds_t)(bytes, nullptr);
-    DVC_DISPATCH(1, 1024, static_lds_t)(bytes, nullptr);
-    DVC_DISPATCH(2, 64, static_lds_t)(bytes, nullptr);
-    DVC_DISPATCH(2, 128, static_lds_t)(bytes, nullptr);
-    DVC_DISPATCH(2, 256, static_lds_t)(bytes, nullptr);
-    DVC_DISPATCH(2, 512, static_lds_t)(bytes, nullptr);
-    DVC_DISPATCH(2, 1024, static_lds_t)(bytes, nullptr);
+#define X(BPV_, B_) DVC_DISPATCH(BPV_, B_, static_lds, bytes)
+    DVC_INST_LIST(X)
+#undef X
     return -1;
 }
 
 int launch_reference(const SearchArgs &a, int block_threads, size_t lds, void *stream) {
+    (void)lds;
     if (a.nproc <= 0) return 0;
     hipStream_t st = (hipStream_t)stream;
-    int bpv = a.bpv;
-    DVC_DISPATCH(1, 64, launch_reference_t)(a, lds, st);
-    DVC_DISPATCH(1, 128, launch_reference_t)(a, lds, st);
-    DVC_DISPATCH(1, 256, launch_reference_t)(a, lds, st);
-    DVC_DISPATCH(1, 512, launch_reference_t)(a, lds, st);
-    DVC_DISPATCH(1, 1024, launch_reference_t)(a, lds, st);
-    DVC_DISPATCH(2, 64, launch_reference_t)(a, lds, st);
-    DVC_DISPATCH(2, 128, launch_reference_t)(a, lds, st);
-    DVC_DISPATCH(2, 256, launch_reference_t)(a, lds, st);
-    DVC_DISPATCH(2, 512, launch_reference_t)(a, lds, st);
-    DVC_DISPATCH(2, 1024, launch_reference_t)(a, lds, st);
-    return -1;
-}
-
-static int lds_limit_one(int bpv, int block_threads, size_t lds_total) {
-    DVC_DISPATCH(1, 64, lds_limit_t)(lds_total, nullptr);
-    DVC_DISPATCH(1, 128, lds_limit_t)(lds_total, nullptr);
-    DVC_DISPATCH(1, 256, lds_limit_t)(lds_total, nullptr);
-    DVC_DISPATCH(1, 512, lds_limit_t)(lds_total, nullptr);
-    DVC_DISPATCH(1, 1024, lds_limit_t)(lds_total, nullptr);
-    DVC_DISPATCH(2, 64, lds_limit_t)(lds_total, nullptr);
-    DVC_DISPATCH(2, 128, lds_limit_t)(lds_total, nullptr);
-    DVC_DISPATCH(2, 256, lds_limit_t)(lds_total, nullptr);
-    DVC_DISPATCH(2, 512, lds_limit_t)(lds_total, nullptr);
-    DVC_DISPATCH(2, 1024, lds_limit_t)(lds_total, nullptr);
+    const int bpv = a.bpv;
+#define X(BPV_, B_) DVC_DISPATCH(BPV_, B_, launch_reference, a, st)
+    DVC_INST_LIST(X)
+#undef X
     return -1;
 }
 
 int search_set_lds_limit(size_t lds_total) {
     for (int bpv = 1; bpv <= 2; bpv++)
-        for (int bt = 64; bt <= 1024; bt *= 2)
-            if (lds_limit_one(bpv, bt, lds_total) != 0) return -1;
+        for (int block_threads = 64; block_threads <= 1024; block_threads *= 2) {
+            int r = -1;
+            [&] {
+#define X(BPV_, B_) \
+    if (bpv == BPV_ && block_threads == B_) { r = Inst<BPV_, B_>::lds_limit(lds_total); return; }
+                DVC_INST_LIST(X)
+#undef X
+            }();
+            if (r != 0) return -1;
+        }
     return 0;
 }
 
 int search_max_blocks(int bpv, int block_threads, size_t lds, int *bpc) {
-    DVC_DISPATCH(1, 64, occ_t)(lds, bpc);
-    DVC_DISPATCH(1, 128, occ_t)(lds, bpc);
-    DVC_DISPATCH(1, 256, occ_t)(lds, bpc);
-    DVC_DISPATCH(1, 512, occ_t)(lds, bpc);
-    DVC_DISPATCH(1, 1024, occ_t)(lds, bpc);
-    DVC_DISPATCH(2, 64, occ_t)(lds, bpc);
-    DVC_DISPATCH(2, 128, occ_t)(lds, bpc);
-    DVC_DISPATCH(2, 256, occ_t)(lds, bpc);
-    DVC_DISPATCH(2, 512, occ_t)(lds, bpc);
-    DVC_DISPATCH(2, 1024, occ_t)(lds, bpc);
+#define X(BPV_, B_) DVC_DISPATCH(BPV_, B_, occupancy, lds, bpc)
+    DVC_INST_LIST(X)
+#undef X
     return -1;
 }
-
+#endif  // the main unit
 #endif  // DVC_EMU
+
+// The remaining kernels and their launchers belong to the main unit.
+#if !defined(DVC_PART) || DVC_PART == 0
 
 // ------------------------------------------------------ interpolation kernel
 
@@ -2267,5 +2815,7 @@ int launch_strain(const StrainArgs &a, void *stream) {
 }
 
 #endif  // DVC_EMU
+
+#endif  // the main unit
 
 }  // namespace dvc

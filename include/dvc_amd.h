@@ -247,6 +247,21 @@ int dvc_set_launch_mode(dvc_ctx *ctx, int graph);
  * identical; partly empty rounds at the ends of wide levels fill with later points.
  * -1: dependency levels only. */
 int dvc_set_round_capacity(dvc_ctx *ctx, int32_t points);
+/* Slice launches of the serial chain (no reference counterpart).  A chain launch of few
+ * points leaves most of the chip idle with one workgroup per point.  Such a launch can
+ * instead run as slice launches: one launch per Gauss-Newton evaluation, each point's
+ * samples split over workgroups of samples_per_workgroup samples (0: 256), the last
+ * workgroup of a point to finish adding the slices up in slice order and taking the step
+ * of Search.cpp:194-246.  After `evaluations` such launches (0: 4) the one-workgroup
+ * search resumes the points still searching.  A launch is sliced when its points times
+ * their slices are at most max_workgroups (0: 4 x the chip's CUs; -1: never).  Applies to
+ * the serial chain with one geometry for all points, no translation grid and the HBM
+ * table of reference samples; results differ from the one-workgroup search only in the
+ * summation order of the per-point sums.  Takes effect at the next execute. */
+int dvc_set_slice_launches(dvc_ctx *ctx, int32_t max_workgroups, int32_t evaluations, int32_t samples_per_workgroup);
+/* Slice launches of the prepared search: chain launches run that way, workgroups per
+ * point and slice launches per chain launch (all 0 when none).  Any pointer may be NULL. */
+int dvc_search_slice_info(dvc_ctx *ctx, int32_t *sliced_launches, int32_t *slices_per_point, int32_t *evaluations);
 /* Shape of the prepared search: chain launches (dependency levels, or launches of the
  * round capacity), threads per point, points searched.  Any pointer may be NULL. */
 int dvc_search_info(dvc_ctx *ctx, int32_t *levels, int32_t *threads_per_point, int32_t *points_searched);

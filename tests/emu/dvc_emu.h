@@ -160,6 +160,12 @@ inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
 #define __builtin_amdgcn_fence(order, scope) std::atomic_thread_fence(std::memory_order_seq_cst)
 #define __builtin_amdgcn_s_sleep(n) std::this_thread::yield()
 #define __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, x, y, z) emu::mfma16x16x4((a), (b), (c))
+// a reciprocal estimate of float precision over the whole double range, as the hardware's
+inline double __builtin_amdgcn_rcp(double x) {
+    int e = 0;
+    const double m = std::frexp(x, &e);
+    return std::ldexp((double)(1.0f / (float)m), -e);
+}
 inline uint32_t __builtin_amdgcn_alignbyte(uint32_t hi, uint32_t lo, uint32_t s) {
     return (uint32_t)((((uint64_t)hi << 32) | lo) >> (8 * (s & 3)));
 }

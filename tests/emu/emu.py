@@ -34,7 +34,11 @@ def load():
     return _lib
 
 
-def search(params, ref, tar, points, order, neigh, nproc=0, threads=64, gather32=True, precompute=False):
+def search(params, ref, tar, points, order, neigh, nproc=0, threads=64, gather32=True, precompute=False,
+           slices=None):
+    """slices: None (one workgroup per point), or (max_workgroups, evaluations,
+    samples_per_slice) of the slice launches (max_workgroups 0: no limit); then the
+    number of sliced chain launches is returned with the results."""
     from digitalvolumecorrelation_amd._lib import RESULT_DTYPE
     lib = load()
     ref = np.ascontiguousarray(ref)
@@ -44,16 +48,19 @@ def search(params, ref, tar, points, order, neigh, nproc=0, threads=64, gather32
     o = np.ascontiguousarray(order, dtype=np.int32)
     nb = np.ascontiguousarray(neigh, dtype=np.int32)
     out = np.zeros(len(p), dtype=RESULT_DTYPE)
-    rc = lib.emu_search(C.byref(params), ref.ctypes.data_as(C.c_void_p), tar.ctypes.data_as(C.c_void_p),
-                        ref.dtype.itemsize, W, H, T, p.ctypes.data_as(C.c_void_p), len(p), o.ctypes.data_as(C.c_void_p),
-                        nproc, nb.ctypes.data_as(C.c_void_p), nb.shape[1], threads, -1 if gather32 else 0, int(precompute),
-                        out.ctypes.data_as(C.c_void_p))
+    sw, se, sp = slices if slices is not None else (-1, 0, 0)
+    ns = C.c_int(0)
+    rc = lib.emu_search_sliced(C.byref(params), ref.ctypes.data_as(C.c_void_p), tar.ctypes.data_as(C.c_void_p),
+                               ref.dtype.itemsize, W, H, T, p.ctypes.data_as(C.c_void_p), len(p),
+                               o.ctypes.data_as(C.c_void_p), nproc, nb.ctypes.data_as(C.c_void_p), nb.shape[1], threads,
+                               -1 if gather32 else 0, int(precompute), sw, se, sp, out.ctypes.data_as(C.c_void_p),
+                               C.byref(ns))
     if rc != 0:
         raise RuntimeError(f"emulated search error flags {rc}")
-    return out
+    return (out, ns.value) if slices is not None else out
 
 
-def gn_solve(A, b, chol=False):
+def gn_solve(A, b, chol=False, uniform=False):
     """The search kernel's Gauss-Newton solve (gn_step) of A x = b on one emulated wave:
     the column-pivoted QR alone (chol False), or the Cholesky solve with its QR fallback
     (chol True, the search kernel's own choice; A symmetric)."""
@@ -63,7 +70,7 @@ def gn_solve(A, b, chol=False):
     x = np.zeros(len(b))
     vp = lambda a: a.ctypes.data_as(C.c_void_p)
     lib.emu_gn_solve.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
-    if lib.emu_gn_solve(len(b), vp(A), vp(b), vp(x), int(chol)) != 0:
+    if lib.emu_gn_solve(len(b), vp(A), vp(b), vp(x), 2 if uniform else int(chol)) != 0:
         raise ValueError("n must be 3, 6 or 12")
     return x
 

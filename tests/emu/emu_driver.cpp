@@ -7,23 +7,74 @@
 
 using namespace dvc;
 
-// One launch per dependency level; the workgroups of a level run concurrently.
+// Slice launches of a chain (SliceArgs) with their buffers (two of each, by launch parity).
+struct EmuSlices {
+    int nslice = 0, per = 0, evals = 0, cnt8 = 0;
+    std::vector<char> sliced;
+    std::vector<unsigned char> state;
+    std::vector<double> part;
+};
+
+template <int BPV>
+static void run_slice(const SearchArgs &a, const SliceArgs &sa, int threads) {
+    const unsigned blocks = (unsigned)(sa.cnt8 * sa.nslice);
+    const size_t slds = (size_t)(threads / 64) * 64 * 17 * 4;
+    emu::grid_dim.x = blocks;
+    const unsigned step = threads > 256 ? 2 : 4;
+    for (unsigned b0 = 0; b0 < blocks; b0 += step) {
+        const unsigned nb = blocks - b0 < step ? blocks - b0 : step;
+        if (threads == 256) emu::run_grid(b0, nb, 256, slds, [&] { slice_kernel<BPV, 256>(a, sa); });
+        else emu::run_grid(b0, nb, 512, slds, [&] { slice_kernel<BPV, 512>(a, sa); });
+    }
+}
+
+// One launch per dependency level; the workgroups of a level run concurrently (eight at
+// a time).  Levels marked in sl.sliced run as slice launches followed by the resuming
+// one-workgroup search, as the library enqueues them.
 template <int BPV, int B>
-static void run_search(const SearchArgs &a, size_t lds, const std::vector<int> &lp, const std::vector<int> &lo) {
+static void run_search(const SearchArgs &a, size_t lds, const std::vector<int> &lp, const std::vector<int> &lo,
+                       EmuSlices *sl = nullptr) {
     for (size_t L = 0; L + 1 < lo.size(); L++) {
         int cnt = lo[L + 1] - lo[L];
         const int *pts = lp.data() + lo[L];
+        SliceArgs rs{};
+        if (sl && sl->sliced[L]) {
+            SliceArgs sa{};
+            sa.level_pts = pts;
+            sa.cnt = cnt;
+            sa.cnt8 = (cnt + 7) / 8 * 8;
+            sa.nslice = sl->nslice;
+            sa.per = sl->per;
+            for (int q = 0; q < 2; q++) {
+                sa.state[q] = sl->state.data() + (size_t)q * sl->cnt8 * kStateStride;
+                sa.part[q] = sl->part.data() + (size_t)q * sl->cnt8 * sl->nslice * kPartStride;
+            }
+            const int threads = sl->per >= 512 ? 512 : 256;
+            for (int t = 0; t < sl->evals; t++) {
+                sa.t = t;
+                run_slice<BPV>(a, sa, threads);
+            }
+            rs = sa;
+            rs.t = sl->evals;
+        }
         emu::grid_dim.x = cnt;
         for (int b0 = 0; b0 < cnt; b0 += 8) {
             int nb = cnt - b0 < 8 ? cnt - b0 : 8;
-            emu::run_grid(b0, nb, B, lds, [&] { search_kernel<BPV, B, false>(a, pts); });
+            if (rs.cnt > 0)
+                emu::run_grid(b0, nb, 512, (size_t)8 * 64 * 17 * 4, [&] { resume_kernel<BPV, 512>(a, pts, rs); });
+            else
+                emu::run_grid(b0, nb, B, lds, [&] { search_kernel<BPV, B, false>(a, pts); });
         }
     }
 }
 
-extern "C" int emu_search(const dvc_run_params *rc, const void *ref, const void *tar, int bpv, int W, int H, int T,
-                          const double *pts, int npts, const int *order, int nproc, const int *neigh, int k,
-                          int block_threads, int gather_mode, int precompute, dvc_point_result *out) {
+// slice_wgs: -1 no slice launches, else the largest workgroup count of a sliced launch
+// (0: every launch with at least two slices); slice_evals slice launches per chain
+// launch; slice_per samples per slice.  Slice launches need precompute.
+extern "C" int emu_search_sliced(const dvc_run_params *rc, const void *ref, const void *tar, int bpv, int W, int H,
+                                 int T, const double *pts, int npts, const int *order, int nproc, const int *neigh,
+                                 int k, int block_threads, int gather_mode, int precompute, int slice_wgs,
+                                 int slice_evals, int slice_per, dvc_point_result *out, int *sliced_launches) {
     Geometry G = make_geometry(*rc);
     int N = npts;
     if (nproc > 1 && nproc < npts) N = nproc;
@@ -70,12 +121,32 @@ extern "C" int emu_search(const dvc_run_params *rc, const void *ref, const void 
             else return -1;
         }
     }
-    if (bpv == 1 && block_threads == 64) run_search<1, 64>(a, lds, lp, lo);
-    else if (bpv == 1 && block_threads == 256) run_search<1, 256>(a, lds, lp, lo);
-    else if (bpv == 1 && block_threads == 128) run_search<1, 128>(a, lds, lp, lo);
-    else if (bpv == 2 && block_threads == 128) run_search<2, 128>(a, lds, lp, lo);
-    else if (bpv == 2 && block_threads == 64) run_search<2, 64>(a, lds, lp, lo);
-    else if (bpv == 2 && block_threads == 256) run_search<2, 256>(a, lds, lp, lo);
+    EmuSlices sl;
+    EmuSlices *slp = nullptr;
+    if (sliced_launches) *sliced_launches = 0;
+    if (slice_wgs >= 0) {
+        if (!precompute || rc->basin_radius != 0.0 || (slice_per != 256 && slice_per % 512 != 0) || slice_evals < 1)
+            return -1;
+        sl.per = slice_per;
+        sl.evals = slice_evals;
+        sl.nslice = (a.nsamp + slice_per - 1) / slice_per;
+        const int cnt8 = slice_plan(lo, sl.nslice, slice_wgs > 0 ? slice_wgs : 1 << 30, 8, sl.sliced);
+        if (cnt8 > 0) {
+            sl.cnt8 = cnt8;
+            sl.state.assign((size_t)2 * cnt8 * kStateStride, 0xA5);
+            sl.part.assign((size_t)2 * cnt8 * sl.nslice * kPartStride, -7.0);
+            slp = &sl;
+            int ns = 0;
+            for (char c : sl.sliced) ns += c ? 1 : 0;
+            if (sliced_launches) *sliced_launches = ns;
+        }
+    }
+    if (bpv == 1 && block_threads == 64) run_search<1, 64>(a, lds, lp, lo, slp);
+    else if (bpv == 1 && block_threads == 256) run_search<1, 256>(a, lds, lp, lo, slp);
+    else if (bpv == 1 && block_threads == 128) run_search<1, 128>(a, lds, lp, lo, slp);
+    else if (bpv == 2 && block_threads == 128) run_search<2, 128>(a, lds, lp, lo, slp);
+    else if (bpv == 2 && block_threads == 64) run_search<2, 64>(a, lds, lp, lo, slp);
+    else if (bpv == 2 && block_threads == 256) run_search<2, 256>(a, lds, lp, lo, slp);
     else return -1;
     for (int i = 0; i < npts; i++) {
         out[i] = res[i];
@@ -87,16 +158,24 @@ extern "C" int emu_search(const dvc_run_params *rc, const void *ref, const void 
     return 0;
 }
 
+extern "C" int emu_search(const dvc_run_params *rc, const void *ref, const void *tar, int bpv, int W, int H, int T,
+                          const double *pts, int npts, const int *order, int nproc, const int *neigh, int k,
+                          int block_threads, int gather_mode, int precompute, dvc_point_result *out) {
+    return emu_search_sliced(rc, ref, tar, bpv, W, H, T, pts, npts, order, nproc, neigh, k, block_threads, gather_mode,
+                             precompute, -1, 0, 0, out, nullptr);
+}
+
 // gn_step<N> alone on one wave: with the SSD moments V[i][j] = A[i][j], V[i][12] = -b[i],
 // V[i][13] = 0 it solves A x = b, by the column-pivoted QR of the search kernel (chol 0)
-// or by its Cholesky solve with the QR fallback (chol 1).
+// or by its Cholesky solve with the QR fallback (chol 1), or by the lane-uniform LDL^T
+// solve of the slice launches with the QR fallback (chol 2).
 extern "C" int emu_gn_solve(int n, const double *A, const double *b, double *x, int chol) {
     if (n != 3 && n != 6 && n != 12) return -1;
     SearchArgs a;
     std::memset(&a, 0, sizeof(a));
     a.obj = DVC_OBJ_SSD;
     a.nsamp = 1;
-    std::vector<double> V(256, 0.0);
+    std::vector<double> V(256, 0.0), ne(256, 0.0);
     for (int i = 0; i < n; i++) {
         for (int j = 0; j < n; j++) V[i * 16 + j] = A[i * n + j];
         V[i * 16 + 12] = -b[i];
@@ -107,14 +186,18 @@ extern "C" int emu_gn_solve(int n, const double *A, const double *b, double *x, 
         if (threadIdx.x == 0)
             for (int q = 0; q < 12; q++) { S.par[q] = 0.0; S.upd[q] = 0.0; }
         emu::wave_sync();
-        if (chol) {
-            if (n == 3) gn_step<3, true>(a, S, V.data());
-            else if (n == 6) gn_step<6, true>(a, S, V.data());
-            else gn_step<12, true>(a, S, V.data());
+        if (chol == 2) {
+            if (n == 3) gn_step<3, true, true>(a, S, V.data(), ne.data());
+            else if (n == 6) gn_step<6, true, true>(a, S, V.data(), ne.data());
+            else gn_step<12, true, true>(a, S, V.data(), ne.data());
+        } else if (chol) {
+            if (n == 3) gn_step<3, true>(a, S, V.data(), ne.data());
+            else if (n == 6) gn_step<6, true>(a, S, V.data(), ne.data());
+            else gn_step<12, true>(a, S, V.data(), ne.data());
         } else {
-            if (n == 3) gn_step<3, false>(a, S, V.data());
-            else if (n == 6) gn_step<6, false>(a, S, V.data());
-            else gn_step<12, false>(a, S, V.data());
+            if (n == 3) gn_step<3, false>(a, S, V.data(), ne.data());
+            else if (n == 6) gn_step<6, false>(a, S, V.data(), ne.data());
+            else gn_step<12, false>(a, S, V.data(), ne.data());
         }
         emu::wave_sync();
         if (threadIdx.x == 0)

@@ -92,18 +92,20 @@ def test_gn_solve_matches_oracle_bitwise(oracle, n):
         assert np.array_equal(got[ok], want[ok]), (A, got, want)
 
 
+@pytest.mark.parametrize("uniform", [False, True])
 @pytest.mark.parametrize("n", [3, 6, 12])
-def test_gn_solve_cholesky(oracle, n):
+def test_gn_solve_cholesky(oracle, n, uniform):
     """The search kernel's own solve: Cholesky of the symmetric normal matrix, with the
     column-pivoted QR taking over when a pivot is not above 1e-9 of the largest diagonal.
     Well-conditioned and badly scaled systems agree with an LU solve to the conditioning
-    bound; singular, near-singular and all-zero systems give the QR's bits."""
+    bound; singular, near-singular and all-zero systems give the QR's bits.  uniform: the
+    LDL^T of the slice launches, every lane holding the whole system (same pivot rule)."""
     rng = np.random.default_rng(70 + n)
     for scale in (np.ones(n), np.logspace(0, 3, n)):
         J = rng.normal(size=(40, n)) * scale
         A = J.T @ J
         b = rng.normal(size=n)
-        got = emu.gn_solve(A, b, chol=True)
+        got = emu.gn_solve(A, b, chol=True, uniform=uniform)
         want = np.linalg.solve(A, b)
         assert np.linalg.norm(got - want) <= 1e-13 * np.linalg.cond(A) * np.linalg.norm(want)
     singular = []
@@ -121,7 +123,7 @@ def test_gn_solve_cholesky(oracle, n):
     for A in singular:
         b = rng.normal(size=n)
         want = oracle.colpiv_solve(A, b)
-        got = emu.gn_solve(A, b, chol=True)
+        got = emu.gn_solve(A, b, chol=True, uniform=uniform)
         assert np.array_equal(np.isnan(want), np.isnan(got))
         ok = ~np.isnan(want)
         assert np.array_equal(got[ok], want[ok]), (A, got, want)
@@ -204,3 +206,50 @@ def test_12dof_range_failures(oracle):
     assert (o["status"] == D.RANGE_FAIL).any() and (o["status"] == D.POINT_GOOD).any()
     g, o = _cmp(oracle, case, dict(case.params, disp_max=1, rigid_trans=(0.0, 0.0, 0.0)), threads=128)
     assert np.all(o["status"] == D.RANGE_FAIL) and np.all(o["iters"] >= 3)
+
+
+def _sliced(oracle, case, params, slices, threads=64, nproc=0):
+    """Slice launches (one launch per evaluation, samples split over workgroups, the last
+    workgroup of a point taking the step) against the oracle and against the
+    one-workgroup search of the same build."""
+    order, neigh = D.sort_order_neighbors(case.points, start=case.start)
+    run = oracle.make_run(shape=case.ref.shape, bytes_per=case.ref.dtype.itemsize, **params)
+    o, _, _ = oracle.dvc_run(run, case.ref, case.tar, case.points, order, neigh, nproc=nproc, nthreads=4)
+    pr = D.make_params(**params)
+    g, ns = emu.search(pr, case.ref, case.tar, case.points, order, neigh, nproc=nproc, threads=threads,
+                       precompute=True, slices=slices)
+    w = emu.search(pr, case.ref, case.tar, case.points, order, neigh, nproc=nproc, threads=threads, precompute=True)
+    assert ns > 0
+    for other in (o, w):
+        assert np.array_equal(g["status"], other["status"])
+        assert np.array_equal(g["iters"], other["iters"])
+    good = o["status"] == 0
+    assert np.abs(g["par"][:, :3] - o["par"][:, :3]).max() <= 1e-3
+    if good.any():
+        assert (np.abs(g["obj"][good] - o["obj"][good]) / o["obj"][good]).max() <= 1e-5
+        # only the summation order of the per-point sums differs from the one-workgroup search
+        assert np.abs(g["par"][good] - w["par"][good]).max() <= 1e-5
+    return g, o
+
+
+@pytest.mark.parametrize("evals", [1, 2, 4])
+def test_sliced_affine_sphere_u16_12dof(oracle, evals):
+    """600 samples in three slices of 256; with fewer slice launches than evaluations the
+    one-workgroup search resumes the points from their carried state."""
+    case = synth.affine_u16(size=72, n=2, spacing=10, npts=600, diameter=21)
+    g, o = _sliced(oracle, case, dict(case.params, disp_max=5), (0, evals, 256), threads=128)
+    assert np.all(o["status"] == 0)
+
+
+@pytest.mark.parametrize("ndof,obj", [(6, "ssd"), (6, "zssd"), (3, "znssd"), (6, "nssd"), (6, "sad")])
+def test_sliced_dofs_objectives(oracle, ndof, obj):
+    case = synth.affine_u16(size=72, n=2, spacing=10, npts=700, diameter=21)
+    _sliced(oracle, case, dict(case.params, disp_max=5, ndof=ndof, obj=obj), (0, 3, 256), nproc=5)
+
+
+def test_sliced_translation_cube_u8_range_fail(oracle):
+    """Cube clouds (grid samples); the outer points' boxes leave the volume (Range_Fail at
+    set-up), in slice launches with two slices of 512 samples."""
+    case = synth.translation_u8(size=56, n=3, spacing=12, npts=1000)
+    g, o = _sliced(oracle, case, dict(case.params, subvol_size=21, disp_max=4), (0, 4, 512))
+    assert (o["status"] == -1).any() and (o["status"] == 0).any()

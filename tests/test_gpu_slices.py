@@ -1,0 +1,82 @@
+"""Slice launches of the serial chain on the GPU (dvc_set_slice_launches): a chain
+launch's points run one launch per Gauss-Newton evaluation with each point's samples
+split over several workgroups, then the one-workgroup search resumes the points still
+searching.  Checked against the oracle's serial dvc loop (Core/dvc.cpp:229-320 around
+Search::process_point, Core/Search.cpp:95-246) and against the one-workgroup search of
+the same library.
+
+Bar (per assertion): statuses and iteration counts equal to the oracle's, |du| <= 1e-3
+voxel, objectives <= 1e-5 relative; against the one-workgroup search (the same FP32
+sample sums in another order) |dpar| <= 1e-6.
+"""
+import numpy as np
+import pytest
+
+import digitalvolumecorrelation_amd as D
+from digitalvolumecorrelation_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(ctx, case, params, order, neigh, slices):
+    ctx.set_slice_launches(*slices)
+    try:
+        ctx.prepare(D.make_params(**params), case.points, order, neigh)
+        ctx.execute()
+        res = ctx.fetch()
+        info = ctx.slice_info()
+    finally:
+        ctx.set_slice_launches(0, 0, 0)
+    return res, info
+
+
+def _check(g, o, w):
+    assert np.array_equal(g["status"], o["status"])
+    assert np.array_equal(g["iters"], o["iters"])
+    good = o["status"] == 0
+    assert np.abs(g["par"][:, :3] - o["par"][:, :3]).max() <= 1e-3
+    if good.any():
+        assert (np.abs(g["obj"][good] - o["obj"][good]) / np.abs(o["obj"][good])).max() <= 1e-5
+        assert np.abs(g["par"][good] - w["par"][good]).max() <= 1e-6
+    assert np.array_equal(g["status"], w["status"])
+    return good
+
+
+@pytest.fixture(scope="module")
+def affine():
+    # 12-DOF sphere clouds of the affine workload's shape on a smaller volume
+    return synth.affine_u16(size=128, n=5, spacing=12, npts=3000, diameter=31)
+
+
+@pytest.mark.parametrize("slices", [(0, 4, 512), (0, 1, 512), (0, 2, 256), (0, 4, 1024)])
+def test_sliced_affine_u16_12dof(ctx, oracle, affine, slices):
+    case = affine
+    p = dict(case.params, disp_max=8)
+    order, neigh = D.sort_order_neighbors(case.points, case.start)
+    ctx.upload(0, case.ref)
+    ctx.upload(1, case.tar)
+    g, info = _run(ctx, case, p, order, neigh, slices)
+    w, winfo = _run(ctx, case, p, order, neigh, (-1, 0, 0))
+    run = oracle.make_run(shape=case.ref.shape, bytes_per=2, **p)
+    o, _, _ = oracle.dvc_run(run, case.ref, case.tar, case.points, order, neigh, nthreads=8)
+    print(f"slices {slices}: {info[0]} sliced launches, {info[1]} workgroups per point, {info[2]} slice launches each")
+    assert info[0] > 0 and winfo[0] == 0
+    good = _check(g, o, w)
+    assert good.all()
+
+
+@pytest.mark.parametrize("ndof,obj", [(6, "ssd"), (3, "znssd"), (6, "nssd")])
+def test_sliced_dofs_objectives_range_fail(ctx, oracle, ndof, obj):
+    # the outer ring of points has boxes leaving the volume (Range_Fail at set-up)
+    case = synth.translation_u8(size=72, n=5, spacing=11, npts=2000)
+    p = dict(case.params, ndof=ndof, obj=obj, subvol_size=21, disp_max=4)
+    order, neigh = D.sort_order_neighbors(case.points, case.start)
+    ctx.upload(0, case.ref)
+    ctx.upload(1, case.tar)
+    g, info = _run(ctx, case, p, order, neigh, (0, 3, 512))
+    w, _ = _run(ctx, case, p, order, neigh, (-1, 0, 0))
+    run = oracle.make_run(shape=case.ref.shape, bytes_per=1, **p)
+    o, _, _ = oracle.dvc_run(run, case.ref, case.tar, case.points, order, neigh, nthreads=8)
+    assert info[0] > 0
+    _check(g, o, w)
+    assert (o["status"] == -1).any() and (o["status"] == 0).any()

@@ -64,14 +64,18 @@ def test_uniform_geometry_matches_one_geometry_search(ctx, case):
     ctx.upload(0, case.ref)
     ctx.upload(1, case.tar)
     rp = D.make_params(**p)
-    plain = ctx.search(rp, case.points, order, neigh)
-    ctx.set_point_geometry([p["geom"]] * n, [p["subvol_size"]] * n, [p["subvol_npts"]] * n)
+    # the per-point path runs one workgroup per point: compare it with the same launches
+    # of the uniform path (slice launches sum the samples in another order)
+    ctx.set_slice_launches(-1)
     try:
+        plain = ctx.search(rp, case.points, order, neigh)
+        ctx.set_point_geometry([p["geom"]] * n, [p["subvol_size"]] * n, [p["subvol_npts"]] * n)
         ctx.prepare(rp, case.points, order, neigh)
         ctx.execute()
         g = ctx.fetch()
     finally:
         ctx.set_point_geometry()
+        ctx.set_slice_launches(0)
     for f in ("status", "iters", "obj", "par"):
         assert np.array_equal(g[f], plain[f]), f
 
