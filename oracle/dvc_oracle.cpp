@@ -935,7 +935,11 @@ struct Searcher {
             for (int p = 0; p < nd; p++) gm[p] /= np;
             for (int j = 0; j < np; j++)
                 for (int p = 0; p < nd; p++)
+#ifdef ORC_PLANT_ZNSSD_DROP_TERM
+                    J[(size_t)j * nd + p] = (g[(size_t)j * nd + p] - gm[p]) / N;   /* planted mistake (tests only) */
+#else
                     J[(size_t)j * nd + p] = (g[(size_t)j * nd + p] - gm[p]) / N - (t[j] - tm) * c[p] / (N * N * N);
+#endif
         }
         (void)r;
         for (int j = 0; j < np; j++) e[j] = base[j];
@@ -954,9 +958,22 @@ struct Searcher {
             double obj;
             iters += 1;
             if (lm_prep(jump, nd, e, J, obj) != 0) return -1;
+#ifdef ORC_PLANT_LATE_STOP
+            if (i > 1) {            /* planted mistake (tests only): the test skips iteration 1 */
+#else
             if (i > 0) {
+#endif
                 double del_obj = fabs(obj - obj_old);
                 double del_mag = std::sqrt(upd[0] * upd[0] + upd[1] * upd[1] + upd[2] * upd[2]);
+#ifdef ORC_PLANT_DELMAG_ALL
+                /* planted mistake (tests only): every parameter of the step in the step size */
+                del_mag = 0.0;
+                for (int q = 0; q < nd; q++) del_mag += upd[q] * upd[q];
+                del_mag = std::sqrt(del_mag);
+#endif
+#ifdef ORC_PLANT_MAGTOL
+                mag_tol = 0.5 * mag_tol;   /* planted mistake (tests only): half the step tolerance */
+#endif
                 if ((del_obj < obj_tol) || (del_mag < mag_tol)) break;
             }
             for (int a = 0; a < nd; a++) {

@@ -51,7 +51,24 @@ def load() -> C.CDLL:
         return _lib
     if not LIB.exists():
         build()
-    lib = C.CDLL(str(LIB))
+    _lib = _bind(C.CDLL(str(LIB)))
+    return _lib
+
+
+def load_planted(define: str) -> C.CDLL:
+    """A build of the oracle with one planted mistake (an ORC_PLANT_* macro of
+    dvc_oracle.cpp), for the tests that show a check fails on it.  Never the oracle."""
+    out = HERE / "_build" / f"liboracle_{define.lower()}.so"
+    src = HERE / "dvc_oracle.cpp"
+    if not out.exists() or out.stat().st_mtime < src.stat().st_mtime:
+        out.parent.mkdir(exist_ok=True)
+        subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-fopenmp", "-ffp-contract=off", f"-D{define}",
+                        "-shared", "-o", str(out) + ".tmp", str(src)], check=True)
+        Path(str(out) + ".tmp").replace(out)
+    return _bind(C.CDLL(str(out)))
+
+
+def _bind(lib: C.CDLL) -> C.CDLL:
     P, I, D, VP = C.POINTER, C.c_int, C.c_double, C.c_void_p
     sig = {
         "orc_version": (I, []),
@@ -76,7 +93,6 @@ def load() -> C.CDLL:
         f = getattr(lib, name)
         f.restype = res
         f.argtypes = args
-    _lib = lib
     return lib
 
 
@@ -219,7 +235,7 @@ def dvc_run_geo(run: Run, ref: np.ndarray, tar: np.ndarray, points, order, neigh
 
 
 def dvc_run_start(run: Run, ref: np.ndarray, tar: np.ndarray, points, order, neigh, start, mode=1, nproc=0,
-                  nthreads=1):
+                  nthreads=1, lib=None):
     """Given-start passes (mode 1) or given-start + judge passes (mode 2); returns
     (results, proposal-pass results, judge-pass results, kept, seconds)."""
     p = np.ascontiguousarray(points, dtype=np.float64)
@@ -234,8 +250,9 @@ def dvc_run_start(run: Run, ref: np.ndarray, tar: np.ndarray, points, order, nei
     kept = np.zeros(len(p), dtype=np.int32)
     sec = C.c_double()
     vp = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
-    load().orc_dvc_run_start(C.byref(run), vp(ref), vp(tar), _dp(p), len(p), _ip(o), _ip(nb), nb.shape[1], nproc,
-                             nthreads, int(mode), _dp(st), vp(prop), vp(judge), vp(res), _ip(kept), C.byref(sec))
+    (lib or load()).orc_dvc_run_start(C.byref(run), vp(ref), vp(tar), _dp(p), len(p), _ip(o), _ip(nb), nb.shape[1],
+                                      nproc, nthreads, int(mode), _dp(st), vp(prop), vp(judge), vp(res), _ip(kept),
+                                      C.byref(sec))
     return res, prop, judge, kept, sec.value
 
 
