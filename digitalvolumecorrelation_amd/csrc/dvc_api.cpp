@@ -71,7 +71,10 @@ struct dvc_ctx {
     double *d_start = nullptr;                 // [npts*12] given start warps
     dvc_point_result *d_res_prop = nullptr;    // proposal pass (DVC_START_JUDGED)
     dvc_point_result *d_res_judge = nullptr;   // judge pass (DVC_START_JUDGED)
-    int *d_kept = nullptr;                     // 1 where the proposal's result was kept
+    int *d_kept = nullptr;                     // 1 proposal kept, 0 judge kept, 2 repaired
+    int *d_repair = nullptr;                   // 1 where a judged point is searched again
+    int *d_rep_levels = nullptr;               // repair launches (the serial chain's launches)
+    std::vector<int> rep_off;
     // slice launches of the chain (SliceArgs): largest workgroup count of a sliced launch
     // (0 automatic, -1 off), slice launches per chain launch before the one-workgroup
     // search resumes the points still searching (0 automatic), samples per workgroup
@@ -135,7 +138,10 @@ void free_plan(dvc_ctx *c) {
     c->slice_cnt8 = c->slice_n = 0;
     c->sliced.clear();
     hipFree(c->d_start); hipFree(c->d_res_prop); hipFree(c->d_res_judge); hipFree(c->d_kept);
+    hipFree(c->d_repair); hipFree(c->d_rep_levels);
     c->d_start = nullptr; c->d_res_prop = c->d_res_judge = nullptr; c->d_kept = nullptr;
+    c->d_repair = nullptr; c->d_rep_levels = nullptr;
+    c->rep_off.clear();
     c->starts_set = false;
     c->d_rvals = nullptr;
     c->d_rstat = nullptr;
@@ -554,33 +560,41 @@ int dvc_search_prepare(dvc_ctx *ctx, const dvc_run_params *rc, const double *poi
     if (int e = dev_alloc(ctx, &ctx->d_order, (size_t)npts)) return e;
     if (int e = dev_alloc(ctx, &ctx->d_pos, (size_t)npts)) return e;
     if (int e = dev_alloc(ctx, &ctx->d_neigh, (size_t)npts * k)) return e;
-    std::vector<int> level_pts, level_off;
+    std::vector<int> level_pts, level_off, rep_pts, rep_off;
     const int mode = ctx->start_mode;
-    if (mode == DVC_START_CHAIN) {
-        dependency_levels(order, pos, neigh, npts, k, N, level_pts, level_off);
-        // levels wider than the chip holds at once: launches of at most one round,
-        // filled with later points whose neighbours are done
+    // the serial chain's launches: dependency levels, or, where a level is wider than the
+    // chip holds at once, launches of at most one round filled with later points whose
+    // neighbours are done
+    auto chain_launches = [&](std::vector<int> &lp, std::vector<int> &lo) {
+        dependency_levels(order, pos, neigh, npts, k, N, lp, lo);
         const int cap = ctx->round_cap > 0 ? ctx->round_cap : ctx->cus * std::max(1, 512 / bt);
         int widest = 0;
-        for (size_t L = 0; L + 1 < level_off.size(); L++) widest = std::max(widest, level_off[L + 1] - level_off[L]);
-        if (ctx->round_cap >= 0 && widest > cap) capacity_schedule(order, pos, neigh, k, N, cap, level_pts, level_off);
+        for (size_t L = 0; L + 1 < lo.size(); L++) widest = std::max(widest, lo[L + 1] - lo[L]);
+        if (ctx->round_cap >= 0 && widest > cap) capacity_schedule(order, pos, neigh, k, N, cap, lp, lo);
+    };
+    if (mode == DVC_START_CHAIN) {
+        chain_launches(level_pts, level_off);
     } else {
         // no point reads a result of the same pass: one launch over the whole order
         level_pts.assign(order, order + N);
         level_off = {0, N};
+        // the judged mode's repair launches run the serial chain's launches
+        if (mode == DVC_START_JUDGED) chain_launches(rep_pts, rep_off);
     }
     // within each launch, workgroups in the XCD-aware spatial order (the points of one
     // launch do not read each other's results, so any order gives the same results)
-    {
+    auto spatial = [&](std::vector<int> &lp, const std::vector<int> &lo) {
         std::vector<int> seg, idx;
-        for (size_t L = 0; L + 1 < level_off.size(); L++) {
-            const int lo = level_off[L], cnt = level_off[L + 1] - lo;
+        for (size_t L = 0; L + 1 < lo.size(); L++) {
+            const int l0 = lo[L], cnt = lo[L + 1] - l0;
             if (cnt < 2) continue;
-            spatial_block_order(points, level_pts.data() + lo, cnt, kXcds, idx);
-            seg.assign(level_pts.begin() + lo, level_pts.begin() + lo + cnt);
-            for (int b = 0; b < cnt; b++) level_pts[lo + b] = seg[idx[b]];
+            spatial_block_order(points, lp.data() + l0, cnt, kXcds, idx);
+            seg.assign(lp.begin() + l0, lp.begin() + l0 + cnt);
+            for (int b = 0; b < cnt; b++) lp[l0 + b] = seg[idx[b]];
         }
-    }
+    };
+    spatial(level_pts, level_off);
+    spatial(rep_pts, rep_off);
     if (int e = dev_alloc(ctx, &ctx->d_levels, level_pts.size())) return e;
     if (int e = dev_alloc(ctx, &ctx->d_res, (size_t)npts)) return e;
     ctx->starts_set = false;
@@ -592,7 +606,12 @@ int dvc_search_prepare(dvc_ctx *ctx, const dvc_run_params *rc, const double *poi
         if (int e = dev_alloc(ctx, &ctx->d_res_prop, (size_t)npts)) return e;
         if (int e = dev_alloc(ctx, &ctx->d_res_judge, (size_t)npts)) return e;
         if (int e = dev_alloc(ctx, &ctx->d_kept, (size_t)npts)) return e;
+        if (int e = dev_alloc(ctx, &ctx->d_repair, (size_t)npts)) return e;
+        if (int e = dev_alloc(ctx, &ctx->d_rep_levels, rep_pts.size())) return e;
         HIPCHK(hipMemsetAsync(ctx->d_kept, 0, sizeof(int) * (size_t)npts, ctx->stream));
+        HIPCHK(hipMemsetAsync(ctx->d_repair, 0, sizeof(int) * (size_t)npts, ctx->stream));
+        HIPCHK(hipMemcpyAsync(ctx->d_rep_levels, rep_pts.data(), sizeof(int) * rep_pts.size(), hipMemcpyHostToDevice,
+                              ctx->stream));
     }
     std::vector<int> ref_order;
     if (table) {
@@ -628,6 +647,7 @@ int dvc_search_prepare(dvc_ctx *ctx, const dvc_run_params *rc, const double *poi
     ctx->chosen_threads = bt;
     ctx->ref_threads = 256;
     ctx->level_off = level_off;
+    ctx->rep_off = rep_off;
     ctx->plan_mode = mode;
     ctx->searched.assign(npts, 0);
     for (int i = 0; i < npts; i++) ctx->searched[i] = pos[i] < N ? 1 : 0;
@@ -742,8 +762,21 @@ static int enqueue_search(dvc_ctx *ctx, hipStream_t st) {
         a.res = ctx->d_res_judge;
         if (launch_search(a, ctx->d_levels, bt, ctx->nproc, lds, st) != 0)
             return fail(ctx, DVC_ERR_HIP, "search kernel launch failed");
-        if (launch_judge(ctx->d_order, ctx->nproc, ctx->d_res_prop, ctx->d_res_judge, ctx->d_res, ctx->d_kept, st) != 0)
+        if (launch_judge(ctx->d_order, ctx->nproc, ctx->d_res_prop, ctx->d_res_judge, ctx->d_res, ctx->d_kept,
+                         ctx->d_repair, st) != 0)
             return fail(ctx, DVC_ERR_HIP, "judge launch failed");
+        // repair: the points where the passes do not both converge and agree, searched
+        // again in the serial chain's launches from the reference's start guess over the
+        // final results (every other point leaves its launch at once)
+        a.repair = ctx->d_repair;
+        a.res = ctx->d_res;
+        a.nbres = ctx->d_res;
+        const int depth = (int)ctx->rep_off.size() - 1;
+        for (int L = 0; L < depth; L++) {
+            const int cnt = ctx->rep_off[L + 1] - ctx->rep_off[L];
+            if (launch_search(a, ctx->d_rep_levels + ctx->rep_off[L], bt, cnt, lds, st) != 0)
+                return fail(ctx, DVC_ERR_HIP, "repair launch failed");
+        }
     }
     return DVC_OK;
 }

@@ -54,6 +54,7 @@ struct SearchArgs {
     RefStats *rstat;                 // [nproc]
     const PtGeo *pgeo;               // [npts] per-point geometry, or null (one geometry for all)
     const int *ref_order;            // [nproc] processing position of each reference_kernel workgroup, or null
+    const int *repair;               // [npts] search only points whose entry is nonzero (repair launches), or null
 };
 
 // Slice launches of the serial chain: the points of one chain launch run one launch per
@@ -106,7 +107,7 @@ int launch_strain(const StrainArgs &a, void *stream);
 int launch_reference(const SearchArgs &a, int block_threads, size_t lds_bytes, void *stream);
 int launch_zero_int(int *p, long long n, void *stream);
 int launch_judge(const int *order, int n, const dvc_point_result *prop, const dvc_point_result *judge,
-                 dvc_point_result *out, int *kept, void *stream);
+                 dvc_point_result *out, int *kept, int *repair, void *stream);
 int launch_swap16(uint16_t *p, long long n, void *stream);
 
 // context internals for the proposal network (dvc_net.hip), implemented in dvc_api.cpp

@@ -1928,6 +1928,7 @@ __global__ void __launch_bounds__(B) search_kernel(SearchArgs a, const int *leve
     DVC_SHARED(double[256], V);
     const int tid = threadIdx.x;
     const int n = level_pts[blockIdx.x];
+    if (a.repair && !a.repair[n]) return;         // a repair launch: this point's result stands
     const int p = a.pos[n];
     if constexpr (UNEVEN) {
         DVC_SHARED(SearchArgs, la);
@@ -2326,28 +2327,42 @@ int launch_objective(int kind, const double *ref, const double *tar, int n, doub
 
 // ------------------------------------------------------------- small kernels
 
-// Keep-if-not-worse selection of a proposal pass against its judge pass (the
-// reference's start guess applied to the proposal pass's results).  The proposal's
-// result is kept when it converged and its objective is no worse than the judge's by
-// more than the reference's own objective tolerance (1e-6, Search.cpp:222-229).
+// Selection between a proposal pass and its judge pass (the reference's start guess
+// applied to the proposal pass's results).  Where both converged and agree
+// (translations within 1e-3 voxel, objectives within 1e-5 relative) the judge's result
+// is the point's (kept 0).  Every other point is marked for repair (kept 2): repair
+// launches search it again in the serial chain's order from the reference's start guess
+// over the final results.
 __global__ void judge_kernel(const int *order, int n, const dvc_point_result *prop,
-                             const dvc_point_result *judge, dvc_point_result *out, int *kept) {
+                             const dvc_point_result *judge, dvc_point_result *out, int *kept, int *repair) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int m = order[i];
     const dvc_point_result &a = prop[m];
     const dvc_point_result &b = judge[m];
-    const bool keep = a.status == DVC_POINT_GOOD && (b.status != DVC_POINT_GOOD || a.obj <= b.obj + 1e-6);
-    out[m] = keep ? a : b;
-    kept[m] = keep ? 1 : 0;
+    bool agree = a.status == DVC_POINT_GOOD && b.status == DVC_POINT_GOOD;
+    if (agree) {
+        double du = 0.0;
+        for (int q = 0; q < 3; q++) du = fmax(du, fabs(a.par[q] - b.par[q]));
+        agree = du <= 1e-3 && fabs(a.obj - b.obj) <= 1e-5 * fabs(b.obj);
+    }
+    if (agree) {
+        out[m] = b;
+        kept[m] = 0;
+        repair[m] = 0;
+    } else {
+        out[m] = b;
+        kept[m] = 2;
+        repair[m] = 1;
+    }
 }
 
 #ifndef DVC_EMU
 int launch_judge(const int *order, int n, const dvc_point_result *prop, const dvc_point_result *judge,
-                 dvc_point_result *out, int *kept, void *stream) {
+                 dvc_point_result *out, int *kept, int *repair, void *stream) {
     if (n <= 0) return 0;
     hipLaunchKernelGGL(judge_kernel, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, order, n, prop,
-                       judge, out, kept);
+                       judge, out, kept, repair);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 #endif

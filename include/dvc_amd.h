@@ -200,10 +200,14 @@ int dvc_set_point_geometry(dvc_ctx *ctx, const int32_t *geom, const int32_t *siz
  *   DVC_START_GIVEN  : every point searched from its given start warp start[12n..12n+ndof).
  *   DVC_START_JUDGED : the given-start pass, then a judge pass in which every point starts
  *                      from the reference's own guess (mean translation of good neighbours
- *                      earlier in order) computed on the first pass's results; a point keeps
- *                      the given-start result only if it converged and its objective is no
- *                      worse than the judge's by more than 1e-6 (the objective tolerance of
- *                      Search.cpp:222-229), else the judge's.
+ *                      earlier in order) computed on the first pass's results.  Where both
+ *                      passes converged and agree (translations within 1e-3 voxel,
+ *                      objectives within 1e-5 relative) the judge's result is the point's.
+ *                      Every other point is searched again (repair) in the serial chain's
+ *                      launches, from the reference's start guess over the final results.
+ *                      Every result is thus the plain search from the reference's start
+ *                      guess; the given starts only make the neighbours' results available
+ *                      at once.
  * dvc_set_start_mode applies to the next dvc_search_prepare (default DVC_START_CHAIN;
  * dvc_search_points always runs the chain). */
 #define DVC_START_CHAIN  0
@@ -218,8 +222,9 @@ int dvc_set_start_mode(dvc_ctx *ctx, int mode);
  * with DVC_RANGE_FAIL in the given-start pass. */
 int dvc_search_set_starts(dvc_ctx *ctx, const double *start, int on_device);
 /* After a DVC_START_JUDGED execute: the given-start pass's results, the judge pass's
- * results (each [npts], either may be NULL) and kept[npts] (1: given-start result kept,
- * 0: judge's, -1: not searched; may be NULL).  dvc_search_fetch returns the selection. */
+ * results (each [npts], either may be NULL) and kept[npts] (0: the judge's result,
+ * 2: repaired, -1: not searched; may be NULL).  dvc_search_fetch returns the
+ * selection. */
 int dvc_search_fetch_passes(dvc_ctx *ctx, dvc_point_result *proposal, dvc_point_result *judge, int32_t *kept);
 
 /* Launch tuning: threads per point (64, 128, 256, 512 or 1024; 0 = automatic), applied

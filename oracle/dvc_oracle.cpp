@@ -1311,11 +1311,41 @@ int orc_dvc_run_start(const orc_run *rc, const void *refv, const void *tarv, con
             Pt sp(p[3 * n], p[3 * n + 1], p[3 * n + 2]);
             store(judge, n, S.process_point(sp, nres));
         }
+        /* selection: where both passes converged and agree (translations within 1e-3
+           voxel, objectives within 1e-5 relative) the judge's result (the reference's
+           start guess over the given-start results) is the point's; every other point is
+           searched again (repair) */
         for (int n = 0; n < npts; n++) {
             if (pos[n] >= (int)N) { kept[n] = -1; continue; }
-            bool keep = prop[n].status == 0 && (judge[n].status != 0 || prop[n].obj <= judge[n].obj + 1e-6);
-            results[n] = keep ? prop[n] : judge[n];
-            kept[n] = keep ? 1 : 0;
+            const orc_result &a = prop[n], &b = judge[n];
+            bool agree = a.status == 0 && b.status == 0;
+            if (agree) {
+                double du = 0.0;
+                for (int q = 0; q < 3; q++) du = std::max(du, fabs(a.par[q] - b.par[q]));
+                agree = du <= 1e-3 && fabs(a.obj - b.obj) <= 1e-5 * fabs(b.obj);
+            }
+            if (agree) {
+                results[n] = b;
+                kept[n] = 0;
+            } else {
+                kept[n] = 2;
+            }
+        }
+        /* repair: the disagreeing points in processing order, each from the reference's
+           start guess (Search.cpp:277-377) over the final results of its earlier
+           neighbours, as the serial loop (dvc.cpp:233-303) runs them */
+        for (unsigned int i = 0; i < N; i++) {
+            int n = order[i];
+            if (kept[n] != 2) continue;
+            nres.clear();
+            for (int j = 0; j < k; j++) {
+                int m = neigh[(size_t)n * k + j];
+                nres.push_back(pos[m] < (int)i ? (const Result *)&results[m] : (const Result *)&none);
+            }
+            results[n].status = -3; results[n].iters = 0; results[n].obj = 0.0;
+            for (int q = 0; q < 12; q++) results[n].par[q] = 0.0;
+            Pt sp(p[3 * n], p[3 * n + 1], p[3 * n + 2]);
+            store(results, n, S.process_point(sp, nres));
         }
     }
     auto t1 = std::chrono::steady_clock::now();

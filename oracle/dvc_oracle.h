@@ -90,9 +90,11 @@ int    orc_dvc_run_geo(const orc_run *rc, const void *ref, const void *tar,
 
 /* Given-start passes (no reference counterpart): mode 1 searches every point of the
  * first nproc of order[] from start[12n..]; mode 2 adds the judge pass (the reference's
- * start guess over the first pass's results at earlier positions) and keeps the first
- * pass's result where it is good and no worse than the judge's objective + 1e-6.
- * prop/judge/kept are required for mode 2 (kept: 1 kept, 0 judge, -1 unsearched). */
+ * start guess over the first pass's results at earlier positions).  Where both passes
+ * converged and agree (translations within 1e-3 voxel, objectives within 1e-5
+ * relative) the judge's result is the point's; the other points are searched again in
+ * processing order from the reference's start guess over the final results (repair).
+ * prop/judge/kept are required for mode 2 (kept: 0 judge, 2 repaired, -1 unsearched). */
 int    orc_dvc_run_start(const orc_run *rc, const void *ref, const void *tar,
                          const double *pts, int npts, const int *order, const int *neigh, int k,
                          int nproc, int nthreads, int mode, const double *start, orc_result *prop,
