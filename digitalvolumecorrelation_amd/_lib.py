@@ -80,6 +80,8 @@ def load() -> C.CDLL:
         "dvc_set_launch_mode": (I, [VP, C.c_int]),
         "dvc_search_info": (I, [VP, C.c_void_p, C.c_void_p, C.c_void_p]),
         "dvc_set_slice_launches": (I, [VP, I, I, I]),
+        "dvc_search_set_strain": (I, [VP, I, D, I, I]),
+        "dvc_search_fetch_strain": (I, [VP, VP, VP, VP, VP, VP, VP]),
         "dvc_search_slice_info": (I, [VP, C.c_void_p, C.c_void_p, C.c_void_p]),
         "dvc_set_start_mode": (I, [VP, C.c_int]),
         "dvc_set_point_geometry": (I, [VP, P(I), P(I), P(I), I]),
@@ -297,6 +299,20 @@ class Context:
         -1 never); `evaluations` slice launches before the one-workgroup search resumes."""
         self._check(self.lib.dvc_set_slice_launches(self.h, int(max_workgroups), int(evaluations),
                                                     int(samples_per_workgroup)))
+
+    def set_search_strain(self, ndp=27, objt=1.0, refill=False, planar=0):
+        """Fit the strain window of every execute's results on the device (ndp 0: off)."""
+        self._check(self.lib.dvc_search_set_strain(self.h, int(ndp), float(objt), int(bool(refill)), int(planar)))
+
+    def fetch_search_strain(self):
+        """Outputs of the last execute's strain fit, as Context.strain returns them."""
+        n = self._npts
+        o = {"disp": np.zeros((n, 3)), "npts": np.zeros(n, np.int32), "swr": np.zeros(n),
+             "E": np.zeros((n, 9)), "L": np.zeros((n, 9)), "DG": np.zeros((n, 9))}
+        vp = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+        self._check(self.lib.dvc_search_fetch_strain(self.h, vp(o["disp"]), vp(o["npts"]), vp(o["swr"]), vp(o["E"]),
+                                                     vp(o["L"]), vp(o["DG"])))
+        return o
 
     def slice_info(self):
         """(sliced chain launches, workgroups per point, slice launches per chain launch)."""

@@ -83,6 +83,12 @@ struct dvc_ctx {
     int slice_cnt8 = 0, slice_n = 0;           // buffer shape of the current plan
     unsigned char *d_state = nullptr;          // [2][cnt8][kStateStride]
     double *d_part = nullptr;                  // [2][cnt8][nslice][kPartStride]
+    // the strain fit of the search's results after every execute (ndp 0: none)
+    int strain_ndp = 0, strain_refill = 0, strain_planar = 0;
+    double strain_objt = 1.0;
+    double pts_mn[3]{}, pts_mx[3]{};           // bounding box of the prepared points (plane detection)
+    double *d_sdisp = nullptr, *d_sobj = nullptr, *d_sout = nullptr;   // inputs; outputs [3n disp, n swr, 27n E L DG]
+    int *d_sstat = nullptr, *d_snpts = nullptr;
     // the level launches of a prepared search, captured once and replayed
     int use_graph = 1;
     hipGraph_t graph = nullptr;
@@ -139,6 +145,9 @@ void free_plan(dvc_ctx *c) {
     c->sliced.clear();
     hipFree(c->d_start); hipFree(c->d_res_prop); hipFree(c->d_res_judge); hipFree(c->d_kept);
     hipFree(c->d_repair); hipFree(c->d_rep_levels);
+    hipFree(c->d_sdisp); hipFree(c->d_sobj); hipFree(c->d_sout); hipFree(c->d_sstat); hipFree(c->d_snpts);
+    c->d_sdisp = c->d_sobj = c->d_sout = nullptr;
+    c->d_sstat = c->d_snpts = nullptr;
     c->d_start = nullptr; c->d_res_prop = c->d_res_judge = nullptr; c->d_kept = nullptr;
     c->d_repair = nullptr; c->d_rep_levels = nullptr;
     c->rep_off.clear();
@@ -644,6 +653,12 @@ int dvc_search_prepare(dvc_ctx *ctx, const dvc_run_params *rc, const double *poi
     ctx->geo = G;
     ctx->npts = npts; ctx->nproc = N; ctx->k = k;
     for (int q = 0; q < 3; q++) { ctx->off_min[q] = omn[q]; ctx->off_max[q] = omx[q]; }
+    for (int q = 0; q < 3; q++) { ctx->pts_mn[q] = 1e300; ctx->pts_mx[q] = -1e300; }
+    for (int i = 0; i < npts; i++)
+        for (int q = 0; q < 3; q++) {
+            ctx->pts_mn[q] = std::min(ctx->pts_mn[q], points[3 * i + q]);
+            ctx->pts_mx[q] = std::max(ctx->pts_mx[q], points[3 * i + q]);
+        }
     ctx->chosen_threads = bt;
     ctx->ref_threads = 256;
     ctx->level_off = level_off;
@@ -688,11 +703,40 @@ static int plan_slices(dvc_ctx *ctx) {
     return DVC_OK;
 }
 
+static int enqueue_search_only(dvc_ctx *ctx, hipStream_t st);
+
+// The strain fit of the search's results (strain.cpp:402-591), enqueued after the search.
+static int enqueue_strain(dvc_ctx *ctx, hipStream_t st) {
+    const int n = ctx->npts;
+    if (launch_fields(ctx->d_res, ctx->d_pos, n, ctx->nproc, ctx->d_sdisp, ctx->d_sstat, ctx->d_sobj, st) != 0)
+        return fail(ctx, DVC_ERR_HIP, "strain input launch failed");
+    StrainArgs a;
+    a.pts = ctx->d_pts; a.disp = ctx->d_sdisp; a.objmin = ctx->d_sobj; a.status = ctx->d_sstat; a.neigh = ctx->d_neigh;
+    a.k = ctx->k; a.n = n; a.ndp = ctx->strain_ndp; a.refill = ctx->strain_refill;
+    // plane detection (strain.cpp:340-365)
+    a.yz = (ctx->strain_planar & 1) || std::fabs(ctx->pts_mx[0] - ctx->pts_mn[0]) < 0.001;
+    a.zx = (ctx->strain_planar & 2) || std::fabs(ctx->pts_mx[1] - ctx->pts_mn[1]) < 0.001;
+    a.xy = (ctx->strain_planar & 4) || std::fabs(ctx->pts_mx[2] - ctx->pts_mn[2]) < 0.001;
+    a.objt = ctx->strain_objt;
+    double *o = ctx->d_sout;
+    a.out_disp = o; a.out_swr = o + 3 * (size_t)n; a.out_E = o + 4 * (size_t)n; a.out_L = o + 13 * (size_t)n;
+    a.out_DG = o + 22 * (size_t)n;
+    a.out_npts = ctx->d_snpts;
+    if (launch_strain(a, st) != 0) return fail(ctx, DVC_ERR_HIP, "strain launch failed");
+    return DVC_OK;
+}
+
+static int enqueue_search(dvc_ctx *ctx, hipStream_t st) {
+    if (int e = enqueue_search_only(ctx, st)) return e;
+    if (ctx->strain_ndp > 0) return enqueue_strain(ctx, st);
+    return DVC_OK;
+}
+
 // Enqueue the whole search on `st`: clear the results, then one launch per dependency
 // level (a level's points only read results of earlier levels).  A level run as slice
 // launches takes slice_evals_of(ctx) launches of the slice kernel, then the
 // one-workgroup search resumes the points that are still searching.
-static int enqueue_search(dvc_ctx *ctx, hipStream_t st) {
+static int enqueue_search_only(dvc_ctx *ctx, hipStream_t st) {
     const Geometry &G = ctx->geo;
     const dvc_run_params &rc = ctx->rc;
     SearchArgs a;
@@ -887,8 +931,17 @@ int dvc_search_execute(dvc_ctx *ctx) {
         return fail(ctx, DVC_ERR_STATE, "dvc_search_set_starts first");
     if (int e = check_volumes(ctx)) return e;
     HIPCHK(hipSetDevice(ctx->device));
-    if (!ctx->gexec)
+    if (!ctx->gexec) {
         if (int e = plan_slices(ctx)) return e;
+        if (ctx->strain_ndp > 0 && !ctx->d_sout) {
+            const size_t n = (size_t)ctx->npts;
+            if (int e = dev_alloc(ctx, &ctx->d_sdisp, 3 * n)) return e;
+            if (int e = dev_alloc(ctx, &ctx->d_sobj, n)) return e;
+            if (int e = dev_alloc(ctx, &ctx->d_sstat, n)) return e;
+            if (int e = dev_alloc(ctx, &ctx->d_snpts, n)) return e;
+            if (int e = dev_alloc(ctx, &ctx->d_sout, 31 * n)) return e;
+        }
+    }
     if (ctx->use_graph && !ctx->gexec) {
         if (hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
             return fail(ctx, DVC_ERR_HIP, "stream capture could not start");
@@ -1023,6 +1076,36 @@ int dvc_search_slice_info(dvc_ctx *ctx, int32_t *sliced_launches, int32_t *slice
     if (sliced_launches) *sliced_launches = n;
     if (slices_per_point) *slices_per_point = n ? slices_of(ctx) : 0;
     if (evaluations) *evaluations = n ? slice_evals_of(ctx) : 0;
+    return DVC_OK;
+}
+
+int dvc_search_set_strain(dvc_ctx *ctx, int32_t ndp, double objt, int32_t refill, int32_t planar) {
+    if (!ctx) return DVC_ERR_ARG;
+    if (ndp < 0 || ndp > 75) return fail(ctx, DVC_ERR_ARG, "ndp must be 0 (off) or in 1..75");
+    if (planar < 0 || planar > 7) return fail(ctx, DVC_ERR_ARG, "planar must be a 3-bit mask");
+    drop_graph(ctx);
+    ctx->strain_ndp = ndp;
+    ctx->strain_objt = objt;
+    ctx->strain_refill = refill ? 1 : 0;
+    ctx->strain_planar = planar;
+    return DVC_OK;
+}
+
+int dvc_search_fetch_strain(dvc_ctx *ctx, double *out_disp, int32_t *out_npts, double *out_swr, double *out_E,
+                            double *out_L, double *out_DG) {
+    if (!ctx) return DVC_ERR_ARG;
+    if (!ctx->plan_ready || ctx->strain_ndp <= 0 || !ctx->d_sout)
+        return fail(ctx, DVC_ERR_STATE, "no strain fit: dvc_search_set_strain, prepare and execute first");
+    HIPCHK(hipSetDevice(ctx->device));
+    const size_t n = (size_t)ctx->npts;
+    const double *o = ctx->d_sout;
+    struct { void *dst; const void *src; size_t bytes; } cp[6] = {
+        {out_disp, o, 3 * n * sizeof(double)}, {out_swr, o + 3 * n, n * sizeof(double)},
+        {out_E, o + 4 * n, 9 * n * sizeof(double)}, {out_L, o + 13 * n, 9 * n * sizeof(double)},
+        {out_DG, o + 22 * n, 9 * n * sizeof(double)}, {out_npts, ctx->d_snpts, n * sizeof(int32_t)}};
+    for (auto &c : cp)
+        if (c.dst) HIPCHK(hipMemcpyAsync(c.dst, c.src, c.bytes, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
     return DVC_OK;
 }
 

@@ -104,6 +104,8 @@ int launch_interp(const uint8_t *vol, int bpv, int W, int H, int T, const int b0
 int launch_objective(int kind, const double *ref, const double *tar, int n, double *obj_out,
                      double *residual, void *stream);
 int launch_strain(const StrainArgs &a, void *stream);
+int launch_fields(const dvc_point_result *res, const int *pos, int npts, int nproc, double *disp, int *status,
+                  double *objmin, void *stream);
 int launch_reference(const SearchArgs &a, int block_threads, size_t lds_bytes, void *stream);
 int launch_zero_int(int *p, long long n, void *stream);
 int launch_judge(const int *order, int n, const dvc_point_result *prop, const dvc_point_result *judge,

@@ -2822,6 +2822,28 @@ __global__ void __launch_bounds__(64 * kStrainWaves) strain_kernel(StrainArgs a)
 }
 
 #ifndef DVC_EMU
+// The strain fit's inputs from a search's result records: displacement, status and
+// objective per point; points outside the searched prefix of the order are not_search
+// (DataCloud.cpp:168-170).
+__global__ void fields_kernel(const dvc_point_result *res, const int *pos, int npts, int nproc, double *disp,
+                              int *status, double *objmin) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= npts) return;
+    const dvc_point_result &r = res[n];
+    const bool searched = pos[n] < nproc;
+    for (int q = 0; q < 3; q++) disp[3 * n + q] = searched ? r.par[q] : 0.0;
+    status[n] = searched ? r.status : DVC_NOT_SEARCH;
+    objmin[n] = searched ? r.obj : 0.0;
+}
+
+int launch_fields(const dvc_point_result *res, const int *pos, int npts, int nproc, double *disp, int *status,
+                  double *objmin, void *stream) {
+    if (npts <= 0) return 0;
+    hipLaunchKernelGGL(fields_kernel, dim3((npts + 255) / 256), dim3(256), 0, (hipStream_t)stream, res, pos, npts,
+                       nproc, disp, status, objmin);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 int launch_strain(const StrainArgs &a, void *stream) {
     if (a.n <= 0) return 0;
     int blocks = (a.n + kStrainWaves - 1) / kStrainWaves;

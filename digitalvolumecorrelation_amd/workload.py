@@ -97,3 +97,34 @@ def search_bytes(points: np.ndarray, offsets: np.ndarray, results: np.ndarray, n
         tar = support_voxels(warp_cloud(stable, points[i], results["par"][i][:ndof], ndof))
         tot += bytes_per * (ref + tar)
     return (tot / len(good) if len(good) else 0.0), len(good)
+
+
+def cells(xyz: np.ndarray) -> int:
+    """Number of distinct unit cells (truncated coordinates, Point.cpp:57-66) holding samples."""
+    return int(len(np.unique(np.floor(xyz).astype(np.int64), axis=0)))
+
+
+def search_flops(points: np.ndarray, offsets: np.ndarray, results: np.ndarray, ndof: int,
+                 sample: int | None = None) -> tuple[float, float, int]:
+    """Mean algorithmic floating-point work per searched point in the reference's own
+    formulation (Interpolate.cpp:377-402, 472-567; Search.cpp:194-246, 490-525): the
+    64x64 coefficient product of every distinct cell the reference and target clouds
+    touch (2*4096 per cell) and the 64-term evaluation of every sample in every objective
+    evaluation (2*64 per sample).  Evaluations per point: one reference sampling, (1 +
+    ndof) per Gauss-Newton iteration (the forward-difference Jacobian of LM_prep_at) and
+    the final objective (Search.cpp:187).  Target cells are those of the converged cloud
+    (a lower bound of the union over the search).  Returns (mean FLOP per point, mean
+    evaluations per point, points sampled)."""
+    good = np.flatnonzero(results["status"] == 0)
+    if sample is not None and len(good) > sample:
+        good = good[np.linspace(0, len(good) - 1, sample).astype(np.int64)]
+    n = len(offsets)
+    tot = ev = 0.0
+    for i in good:
+        stable = points[i] + offsets
+        c = cells(stable) + cells(warp_cloud(stable, points[i], results["par"][i][:ndof], ndof))
+        e = 1 + int(results["iters"][i]) * (1 + ndof) + 1
+        tot += 2.0 * 4096 * c + 2.0 * 64 * n * e
+        ev += e
+    m = max(len(good), 1)
+    return tot / m, ev / m, len(good)

@@ -327,6 +327,16 @@ int  dvc_synth_warp_tiles(dvc_ctx *ctx, int tile, const double *maps, int64_t nm
  * written straight into its start table (replaces dvc_search_set_starts). */
 int  dvc_search_propose(dvc_ctx *ctx, dvc_net *net);
 
+/* The strain fit of a prepared search's results, on the device after every
+ * dvc_search_execute (the reference's strain program on the dvc program's output,
+ * strain.cpp:402-591 over dvc.cpp's .disp and .sort.csv): the search's neighbour table,
+ * window ndp (1..75; 0: no fit), objective threshold objt, refill and planar as
+ * dvc_strain_fit.  Takes effect at the next execute; dvc_search_fetch_strain copies the
+ * last fit's outputs back (layouts of dvc_strain_fit; any pointer may be NULL). */
+int dvc_search_set_strain(dvc_ctx *ctx, int32_t ndp, double objt, int32_t refill, int32_t planar);
+int dvc_search_fetch_strain(dvc_ctx *ctx, double *out_disp, int32_t *out_npts, double *out_swr, double *out_E,
+                            double *out_L, double *out_DG);
+
 /* ----------------------------------------------------------------- strain */
 
 /* The strain-window quadratic fit of strain.cpp:402-591 for n points: pts[3n],

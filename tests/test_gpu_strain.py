@@ -44,3 +44,25 @@ def test_filtering_refill_and_planar(ctx, oracle):
     plane = pts[np.isclose(pts[:, 1], pts[0, 1])]
     dp = plane @ np.array([[0.01, 0, 0.003], [0, 0, 0], [0.002, 0, -0.004]]).T
     _cmp(ctx, oracle, plane, dp, np.zeros(len(plane), np.int32), np.zeros(len(plane)), 20)
+
+
+def test_strain_after_search_on_device(ctx):
+    """dvc_search_set_strain: the fit runs on the device after every execute, from the
+    search's own results and neighbour table; it equals dvc_strain_fit on the fetched
+    results (the same kernel on the same inputs), not-searched points included."""
+    case = synth.translation_u8(size=80, n=4, spacing=8)
+    order, neigh = D.sort_order_neighbors(case.points, case.start)
+    ctx.upload(0, case.ref)
+    ctx.upload(1, case.tar)
+    ctx.set_search_strain(ndp=27)
+    try:
+        ctx.prepare(D.make_params(**case.params), case.points, order, neigh, nproc=40)
+        ctx.execute()
+        res = ctx.fetch()
+        dev = ctx.fetch_search_strain()
+    finally:
+        ctx.set_search_strain(ndp=0)
+    assert (res["status"] == D.NOT_SEARCH).sum() == len(case.points) - 40
+    host = ctx.strain(case.points, res["par"][:, :3].copy(), res["status"], res["obj"], neigh, ndp=27)
+    for f in ("disp", "npts", "swr", "E", "L", "DG"):
+        assert np.array_equal(dev[f], host[f]), f
