@@ -1,0 +1,205 @@
+"""The three large workloads on the GPU, each made on the device at its full size, with the
+oracle on a prefix of the processing order (volumes copied back with dvc_volume_download)
+and size-independent properties over every point.
+
+* Batched uneven workload: 512^3 uint8 pair under three plane sinusoids (amplitude 3,
+  wavelength 128), 50,000 random points, every point a cube or sphere of size 21-61 with
+  1000-10000 samples, tricubic, 6-DOF, SSD and ZNSSD.  The oracle runs one RunControl per
+  point (orc_dvc_run_geo) on the first 64 points of the order.
+* Learned-proposal evaluation: 1024^3 uint8 pair (translation up to 10 voxels plus
+  sinusoids), 100,000 random points, cube 31 / 2197, ZNSSD, 12-DOF: the serial chain
+  against the oracle on a 16-point prefix, then the judged proposal mode, whose selection
+  rule is checked point by point.
+* Full scale: 1536^3 uint16 pair (8 Gaussian bumps of amplitude 4 and width 200 plus 1 %
+  volumetric strain), 10^6 points on a 100^3 grid at spacing 14, cube 41 / 8000, ZNSSD,
+  12-DOF: the oracle on a 16-point prefix; every point good, error against the imposed
+  field, and the strain fit against the field's analytic gradient.
+
+Bar against the oracle (Core/dvc.cpp:233-303 around Search::process_point): statuses and
+iteration counts equal, displacements within 1e-3 voxel, objectives within 1e-5 relative
+(points that end at the 20-evaluation limit: 1e-4, as in test_gpu_start_modes).
+"""
+import numpy as np
+import pytest
+
+import digitalvolumecorrelation_amd as D
+from digitalvolumecorrelation_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+THREADS = 16          # the oracle's threads: the host share of one GPU
+
+
+def _close(g, o, label):
+    assert np.array_equal(g["status"], o["status"]), label
+    assert np.array_equal(g["iters"], o["iters"]), label
+    good = o["status"] == 0
+    conv = good & (o["iters"] < 20)
+    limit = good & (o["iters"] >= 20)
+    du = np.abs(g["par"][good, :3] - o["par"][good, :3]).max() if good.any() else 0.0
+    rel = np.abs(g["obj"] - o["obj"]) / np.where(good, np.abs(o["obj"]), 1.0)
+    rc = rel[conv].max() if conv.any() else 0.0
+    rl = rel[limit].max() if limit.any() else 0.0
+    print(f"  {label}: {len(good)} points against the oracle, good {good.sum()}, max |du| {du:.2e}, "
+          f"max obj rel {rc:.2e} ({limit.sum()} at the limit: {rl:.2e})", flush=True)
+    assert du <= 1e-3 and rc <= 1e-5 and rl <= 1e-4, label
+
+
+@pytest.fixture(scope="module")
+def uneven(ctx):
+    case, geom, size, nsamp = synth.uneven_u8(ctx)
+    ref = ctx.download(0, case.shape, np.uint8)
+    tar = ctx.download(1, case.shape, np.uint8)
+    order, neigh = D.knn_order_neighbors(case.points, start=case.start)
+    return case, geom, size, nsamp, ref, tar, order, neigh
+
+
+@pytest.mark.timeout(300)
+@pytest.mark.parametrize("obj", ["znssd", "ssd"])
+def test_uneven_512_prefix_and_properties(ctx, oracle, uneven, obj):
+    case, geom, size, nsamp, ref, tar, order, neigh = uneven
+    n = len(case.points)
+    assert n == 50_000 and size.min() == 21 and size.max() == 61
+    assert nsamp.min() >= 1000 and nsamp.max() <= 10000
+    params = dict(case.params, obj=obj)
+    ctx.upload(0, ref)
+    ctx.upload(1, tar)
+    ctx.set_point_geometry(geom, size, nsamp)
+    try:
+        ctx.prepare(D.make_params(**params), case.points, order, neigh)
+        ctx.execute()
+        g = ctx.fetch()
+    finally:
+        ctx.set_point_geometry()
+    print(f"  {obj}: {ctx.levels()} chain launches, {ctx.kernel_ms():.1f} ms", flush=True)
+    M = 64
+    run = oracle.make_run(shape=case.shape, bytes_per=1, jacobian=0, **params)
+    o, sec, m = oracle.dvc_run_geo(run, ref, tar, case.points, order, neigh, geom, size, nsamp, nproc=M,
+                                   nthreads=THREADS)
+    assert m == M
+    idx = order[:M]
+    _close(g[idx], o[idx], f"uneven {obj}")
+    # every point: searched, good (the field stays inside disp_max 6), and near the
+    # imposed field, which a 6-DOF warp of a 21-61 voxel subvolume only approximates
+    good = g["status"] == 0
+    err = np.abs(g["par"][good, :3] - case.truth[good]).max(axis=1)
+    print(f"  {obj}: good {good.sum()} / {n}, error against the field: median {np.median(err):.3f}, "
+          f"max {err.max():.3f} voxel", flush=True)
+    assert good.sum() >= n - 50
+    assert np.median(err) <= 0.45 and err.max() <= 2.0
+
+
+@pytest.fixture(scope="module")
+def learned(ctx):
+    case = synth.learned_eval_u8(ctx)
+    ref = ctx.download(0, case.shape, np.uint8)
+    tar = ctx.download(1, case.shape, np.uint8)
+    order, neigh = D.knn_order_neighbors(case.points, start=case.start)
+    return case, ref, tar, order, neigh
+
+
+@pytest.mark.timeout(300)
+def test_learned_eval_1024_chain_and_judged(ctx, oracle, learned):
+    from digitalvolumecorrelation_amd import proposal as P
+    case, ref, tar, order, neigh = learned
+    n = len(case.points)
+    assert n == 100_000
+    ctx.upload(0, ref)
+    ctx.upload(1, tar)
+    rp = D.make_params(**case.params)
+    plain = ctx.search(rp, case.points, order, neigh)
+    M = 16
+    run = oracle.make_run(shape=case.shape, bytes_per=1, jacobian=0, **case.params)
+    o, sec, m = oracle.dvc_run(run, ref, tar, case.points, order, neigh, nproc=M, nthreads=THREADS)
+    idx = order[:M]
+    _close(plain[idx], o[idx], "learned-eval chain")
+    ok = plain["status"] == 0
+    err = np.abs(plain["par"][ok, :3] - case.truth[ok]).max(axis=1)
+    print(f"  chain: good {ok.sum()} / {n}, error against the field: median {np.median(err):.4f}", flush=True)
+    assert ok.sum() >= n - 10
+
+    net = P.ProposalNet(ctx, max_batch=4096, params=P.ProposalNet.load_params(P.WEIGHTS))
+    ctx.set_start_mode(D.START_JUDGED)
+    try:
+        ctx.prepare(rp, case.points, order, neigh)
+        net.propose()
+        ctx.execute()
+        sel = ctx.fetch()
+        prop, judge, kept = ctx.fetch_passes()
+    finally:
+        ctx.set_start_mode(D.START_CHAIN)
+        net.close()
+    # the selection rule, point by point: the judge's result where both passes converged
+    # and agree (1e-3 voxel, 1e-5 relative), every other point searched again
+    assert set(np.unique(kept)) <= {0, 2}
+    both = (prop["status"] == 0) & (judge["status"] == 0)
+    du = np.abs(prop["par"][:, :3] - judge["par"][:, :3]).max(axis=1)
+    agree = both & (du <= 1e-3) & (np.abs(prop["obj"] - judge["obj"]) <= 1e-5 * np.abs(judge["obj"]))
+    assert np.array_equal(kept == 0, agree)
+    for f in ("status", "iters", "obj", "par"):
+        assert np.array_equal(sel[f][agree], judge[f][agree]), f
+    sgood = sel["status"] == 0
+    dpl = np.abs(sel["par"][:, :3] - plain["par"][:, :3]).max(axis=1)
+    serr = np.abs(sel["par"][sgood, :3] - case.truth[sgood]).max(axis=1)
+    both_good = sgood & ok
+    worse = sel["obj"][both_good] - plain["obj"][both_good] > 1e-5 * np.abs(plain["obj"][both_good])
+    print(f"  judged: {int(agree.sum())} judge results, {int((kept == 2).sum())} repaired; good {sgood.sum()}; "
+          f"within 1e-3 of the chain {int((both_good & (dpl <= 1e-3)).sum())}, within 1e-2 "
+          f"{int((both_good & (dpl <= 1e-2)).sum())}; objective above the chain's by > 1e-5 relative: "
+          f"{int(worse.sum())}; error against the field: median {np.median(serr):.4f}", flush=True)
+    assert sgood.sum() >= ok.sum() - 10
+    assert np.median(serr) <= np.median(err) + 1e-3
+
+
+@pytest.fixture(scope="module")
+def full(ctx):
+    case = synth.full_scale_u16(ctx)
+    order, neigh = D.knn_order_neighbors(case.points, start=case.start)
+    return case, order, neigh
+
+
+def _field_gradient(X, bumps, strain):
+    """d u_i / d X_j of bump_field: strain * I plus every bump's Gaussian derivative."""
+    G = np.broadcast_to(strain * np.eye(3), (len(X), 3, 3)).copy()
+    for b in np.asarray(bumps).reshape(-1, 7):
+        d = X - b[:3]
+        g = np.exp(-(d ** 2).sum(axis=1) / (2.0 * b[3] ** 2))
+        G -= (g / b[3] ** 2)[:, None, None] * b[4:7][None, :, None] * d[:, None, :]
+    return G
+
+
+@pytest.mark.timeout(300)
+def test_full_scale_1536_prefix_and_properties(ctx, oracle, full):
+    case, order, neigh = full
+    n = len(case.points)
+    assert n == 1_000_000 and case.shape == (1536, 1536, 1536)
+    rp = D.make_params(**case.params)
+    ctx.prepare(rp, case.points, order, neigh)
+    ctx.execute()
+    print(f"  {ctx.levels()} chain launches, {ctx.kernel_ms():.0f} ms", flush=True)
+    g = ctx.fetch()
+    good = g["status"] == 0
+    err = np.abs(g["par"][good, :3] - case.truth[good]).max(axis=1)
+    print(f"  good {good.sum()} / {n}; error against the field: median {np.median(err):.4f}, max {err.max():.4f}",
+          flush=True)
+    assert good.all()
+    assert np.median(err) <= 0.02 and err.max() <= 0.06
+    # strain of the 27-point windows against the field's gradient, away from the grid's faces
+    s = ctx.strain(case.points, g["par"][:, :3].copy(), g["status"], g["obj"], neigh, ndp=27)
+    lo, hi = case.points.min(axis=0), case.points.max(axis=0)
+    inner = np.all((case.points > lo + 1.0) & (case.points < hi - 1.0), axis=1)
+    DG = s["DG"].reshape(-1, 3, 3)[inner]
+    G = _field_gradient(case.points[inner], case.bumps, case.strain)
+    dg = np.abs(DG - G).max(axis=(1, 2))
+    print(f"  strain: |DG - grad u| median {np.median(dg):.2e}, max {dg.max():.2e} over {inner.sum()} points",
+          flush=True)
+    assert np.median(dg) <= 3e-4 and dg.max() <= 1e-3
+    # the oracle on the first 16 points of the processing order
+    ref = ctx.download(0, case.shape, np.uint16)
+    tar = ctx.download(1, case.shape, np.uint16)
+    M = 16
+    run = oracle.make_run(shape=case.shape, bytes_per=2, jacobian=0, **case.params)
+    o, sec, m = oracle.dvc_run(run, ref, tar, case.points, order, neigh, nproc=M, nthreads=THREADS)
+    del ref, tar
+    idx = order[:M]
+    _close(g[idx], o[idx], "full scale")
