@@ -71,6 +71,7 @@ def load() -> C.CDLL:
         "dvc_interpolate": (I, [VP, C.c_int, P(RunParams), P(D), P(D), I, P(D), C.c_int, P(D), P(D)]),
         "dvc_interpolate_box": (I, [VP, C.c_int, P(D), P(D), P(D), P(D), I, P(D), C.c_int, P(D), P(D)]),
         "dvc_objective": (I, [VP, C.c_int, P(D), P(D), I, P(D), P(D)]),
+        "dvc_obj_val_at": (I, [VP, P(RunParams), P(D), P(D), P(D), I, P(D), P(D), I, P(I)]),
         "dvc_search_points": (I, [VP, P(RunParams), P(D), I, P(I), I, P(I), I, VP]),
         "dvc_search_prepare": (I, [VP, P(RunParams), P(D), I, P(I), I, P(I), I]),
         "dvc_search_execute": (I, [VP]),
@@ -271,6 +272,24 @@ class Context:
         self._check(self.lib.dvc_objective(self.h, OBJ[kind] if isinstance(kind, str) else kind, _dp(r), _dp(t),
                                            len(r), C.byref(o), _dp(res) if residual else None))
         return (o.value, res) if residual else o.value
+
+    def obj_val_at(self, params, point, x, start=None, residual=False):
+        """dvc_obj_val_at: Search::obj_val_at(x[, residual]) for one point whose
+        correlated-volume box sits on point + start (default: params' rigid_trans).
+        Returns the objective, or (objective, residual)."""
+        p = np.ascontiguousarray(point, dtype=np.float64)
+        xv = np.ascontiguousarray(x, dtype=np.float64)
+        st = None if start is None else np.ascontiguousarray(start, dtype=np.float64)
+        n = C.c_int32(0)
+        o = C.c_double()
+        self._check(self.lib.dvc_obj_val_at(self.h, C.byref(params), _dp(p), _dp(st) if st is not None else None,
+                                            _dp(xv), len(xv), C.byref(o), None, 0, C.byref(n)))
+        if not residual:
+            return o.value
+        res = np.zeros(n.value)
+        self._check(self.lib.dvc_obj_val_at(self.h, C.byref(params), _dp(p), _dp(st) if st is not None else None,
+                                            _dp(xv), len(xv), C.byref(o), _dp(res), len(res), C.byref(n)))
+        return o.value, res
 
     # ---- search
     def set_block_threads(self, n: int):

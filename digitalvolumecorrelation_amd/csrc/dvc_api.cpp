@@ -348,19 +348,14 @@ int dvc_volume_attach(dvc_ctx *ctx, int slot, const void *dptr, int bpv, int64_t
 
 // Interpolate::center_on + kernels + tri_cub_Lek / tri_lin / nearest on the boxes of an
 // Interpolate object (est box, act box = est box less one voxel, kern_4d extent box[3]).
-static int interp_boxes(dvc_ctx *ctx, int slot, HBox est, HBox act, const int box[3], const double center[3],
-                        const double *xyz, int32_t n, const double bbox[6], int method, double *out, double *grad) {
-    if (slot != 0 && slot != 1) return fail(ctx, DVC_ERR_ARG, "bad slot");
-    const Volume &v = ctx->vol[slot];
-    if (!v.d) return fail(ctx, DVC_ERR_STATE, "volume not uploaded");
-    if (n < 0 || (n > 0 && (!xyz || !out || !bbox || !center))) return fail(ctx, DVC_ERR_ARG, "bad arrays");
-    if (method < 0 || method > 2) return fail(ctx, DVC_ERR_ARG, "bad interpolation method");
-    if (grad && method != DVC_INTERP_TRICUBIC) return fail(ctx, DVC_ERR_ARG, "gradients are tricubic only");
-    for (int q = 0; q < 3; q++)
-        if (box[q] < 4) return fail(ctx, DVC_ERR_ARG, "interpolation box smaller than 4 voxels");
+// Interpolate::center_on (Interpolate.cpp:569-574) and the range rule of tri_cub_Lek /
+// tri_lin / nearest (Interpolate.cpp:404-567): the box's first voxel b0 after centring
+// on `center`, DVC_ERR_RANGE when the points' box leaves the active box, a point's cell
+// leaves the coefficient box or the box leaves the volume.
+static int interp_place(dvc_ctx *ctx, const Volume &v, HBox est, HBox act, const int box[3], const double center[3],
+                        const double *xyz, int32_t n, const double bbox[6], int method, int b0[3]) {
     est.center_on(center);
     act.center_on(center);
-    int b0[3];
     bool ok = true;
     int64_t dims[3] = {v.W, v.H, v.T};
     for (int q = 0; q < 3; q++) {
@@ -378,6 +373,21 @@ static int interp_boxes(dvc_ctx *ctx, int slot, HBox est, HBox act, const int bo
                 if (cell < 0 || cell > box[q] - lim) { ok = false; break; }
             }
     if (!ok) return fail(ctx, DVC_ERR_RANGE, "points outside the active interpolation box");
+    return DVC_OK;
+}
+
+static int interp_boxes(dvc_ctx *ctx, int slot, HBox est, HBox act, const int box[3], const double center[3],
+                        const double *xyz, int32_t n, const double bbox[6], int method, double *out, double *grad) {
+    if (slot != 0 && slot != 1) return fail(ctx, DVC_ERR_ARG, "bad slot");
+    const Volume &v = ctx->vol[slot];
+    if (!v.d) return fail(ctx, DVC_ERR_STATE, "volume not uploaded");
+    if (n < 0 || (n > 0 && (!xyz || !out || !bbox || !center))) return fail(ctx, DVC_ERR_ARG, "bad arrays");
+    if (method < 0 || method > 2) return fail(ctx, DVC_ERR_ARG, "bad interpolation method");
+    if (grad && method != DVC_INTERP_TRICUBIC) return fail(ctx, DVC_ERR_ARG, "gradients are tricubic only");
+    for (int q = 0; q < 3; q++)
+        if (box[q] < 4) return fail(ctx, DVC_ERR_ARG, "interpolation box smaller than 4 voxels");
+    int b0[3];
+    if (int e = interp_place(ctx, v, est, act, box, center, xyz, n, bbox, method, b0)) return e;
     if (n == 0) return DVC_OK;
     HIPCHK(hipSetDevice(ctx->device));
     double *d_xyz = nullptr, *d_out = nullptr, *d_grad = nullptr;
@@ -447,6 +457,117 @@ int dvc_objective(dvc_ctx *ctx, int kind, const double *ref, const double *tar, 
     if (r != 0) return fail(ctx, DVC_ERR_HIP, "objective launch failed");
     if (e != hipSuccess) return hip_fail(ctx, e, "objective");
     return DVC_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ obj_val_at
+
+namespace {
+
+// Device buffers of one point evaluation of n samples: the stable and moved clouds,
+// their sample values (ref_subvol, tar_subvol of Search), the residual and the objective.
+struct PointWork {
+    double *xyz_s, *xyz_m, *vref, *vtar, *res, *obj;
+};
+
+size_t point_work_bytes(int n) { return sizeof(double) * (9 * (size_t)n + 1); }
+
+PointWork carve_point_work(void *base, int n) {
+    double *d = (double *)base;
+    const size_t m = (size_t)n;
+    PointWork w;
+    w.xyz_s = d; w.xyz_m = d + 3 * m; w.vref = d + 6 * m; w.vtar = d + 7 * m; w.res = d + 8 * m; w.obj = d + 9 * m;
+    return w;
+}
+
+// Host side of one point evaluation: room for the stable and moved clouds ([3n] each)
+// and the sphere offsets ([3n], spheres only).
+struct PointClouds {
+    double *sph, *stable, *moving;
+};
+
+int obj_val_args(dvc_ctx *ctx, const dvc_run_params *rc, const double point[3], const double *x, int32_t nx,
+                 const double *obj, const double *residual, int32_t residual_len, Geometry &G) {
+    if (int e = check_params(ctx, rc)) return e;
+    if (int e = check_volumes(ctx)) return e;
+    if (!point || !x || !obj) return fail(ctx, DVC_ERR_ARG, "null point, parameters or objective");
+    if (nx < 1 || nx > 12) return fail(ctx, DVC_ERR_ARG, "nx must be in 1..12");
+    for (int q = 0; q < 3; q++)
+        if (!std::isfinite(point[q])) return fail(ctx, DVC_ERR_ARG, "non-finite point");
+    G = make_geometry(*rc);
+    if (residual && residual_len < G.subv_num) return fail(ctx, DVC_ERR_ARG, "residual shorter than the cloud");
+    return DVC_OK;
+}
+
+// Search::obj_val_at (Search.cpp:448-488) in the state Search::search_pt_setup
+// (Search.cpp:248-275) leaves for one point: the clouds are built on the host in C, the
+// sample values, residual and objective live in w on the device.  The reference cloud
+// is sampled with the interpolation box centred on the point, the moved cloud with the
+// box centred on the point moved by the start translation.  Returns after the stream
+// has drained.
+int obj_val_run(dvc_ctx *ctx, const dvc_run_params *rc, const Geometry &G, const double point[3],
+                const double start[3], const double *x, int32_t nx, const PointClouds &C, const PointWork &w,
+                double *obj, double *residual) {
+    const int n = G.subv_num;
+    double sbox[6], sref[3], mbox[6];
+    if (rc->geom == DVC_GEOM_SPHERE) sphere_offsets(*rc, G, C.sph);
+    stable_cloud(*rc, G, C.sph, point, C.stable, sbox, sref);
+    HBox est(G.est_mn, G.est_mx), act(G.act_mn, G.act_mx);
+    for (int q = 0; q < 3; q++) { act.cn[q] = G.act_cn[q]; est.cn[q] = G.est_cn[q]; }
+    int b0r[3], b0t[3];
+    if (interp_place(ctx, ctx->vol[0], est, act, G.box, point, C.stable, n, sbox, rc->interp, b0r))
+        return fail(ctx, DVC_ERR_RANGE, "the reference cloud leaves the interpolation box (Intrp_Fail)");
+    for (int q = 0; q < nx; q++)
+        if (!std::isfinite(x[q])) return fail(ctx, DVC_ERR_RANGE, "non-finite parameters (Range_Fail)");
+    affine_cloud(C.stable, n, sbox, sref, x, nx, C.moving, mbox);
+    double cen[3];
+    for (int q = 0; q < 3; q++) cen[q] = point[q] + start[q];
+    if (interp_place(ctx, ctx->vol[1], est, act, G.box, cen, C.moving, n, mbox, rc->interp, b0t))
+        return fail(ctx, DVC_ERR_RANGE, "the moved cloud leaves the interpolation box (Range_Fail)");
+    HIPCHK(hipSetDevice(ctx->device));
+    const Volume &vr = ctx->vol[0], &vt = ctx->vol[1];
+    hipStream_t st = ctx->stream;
+    hipError_t he = hipMemcpyAsync(w.xyz_s, C.stable, sizeof(double) * 3 * n, hipMemcpyHostToDevice, st);
+    if (he == hipSuccess) he = hipMemcpyAsync(w.xyz_m, C.moving, sizeof(double) * 3 * n, hipMemcpyHostToDevice, st);
+    int e = he == hipSuccess ? 0 : -1;
+    if (e == 0)
+        e = launch_interp(vr.d, vr.bpv, (int)vr.W, (int)vr.H, (int)vr.T, b0r, G.box, w.xyz_s, n, rc->interp, w.vref,
+                          nullptr, st);
+    if (e == 0)
+        e = launch_interp(vt.d, vt.bpv, (int)vt.W, (int)vt.H, (int)vt.T, b0t, G.box, w.xyz_m, n, rc->interp, w.vtar,
+                          nullptr, st);
+    if (e == 0) e = launch_objective(rc->obj, w.vref, w.vtar, n, w.obj, residual ? w.res : nullptr, st);
+    if (e == 0) he = hipMemcpyAsync(obj, w.obj, sizeof(double), hipMemcpyDeviceToHost, st);
+    if (e == 0 && he == hipSuccess && residual)
+        he = hipMemcpyAsync(residual, w.res, sizeof(double) * n, hipMemcpyDeviceToHost, st);
+    hipError_t hs = hipStreamSynchronize(st);
+    if (he != hipSuccess) return hip_fail(ctx, he, "obj_val_at copy");
+    if (e != 0) return fail(ctx, DVC_ERR_HIP, "obj_val_at launch failed");
+    if (hs != hipSuccess) return hip_fail(ctx, hs, "obj_val_at");
+    return DVC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dvc_obj_val_at(dvc_ctx *ctx, const dvc_run_params *rc, const double point[3], const double start[3],
+                   const double *x, int32_t nx, double *obj, double *residual, int32_t residual_len, int32_t *nsamp) {
+    if (!ctx) return DVC_ERR_ARG;
+    Geometry G;
+    if (int e = obj_val_args(ctx, rc, point, x, nx, obj, residual, residual_len, G)) return e;
+    const int n = G.subv_num;
+    if (nsamp) *nsamp = n;
+    std::vector<double> host((size_t)9 * n);
+    PointClouds C{host.data(), host.data() + 3 * (size_t)n, host.data() + 6 * (size_t)n};
+    HIPCHK(hipSetDevice(ctx->device));
+    void *buf = nullptr;
+    HIPCHK(hipMalloc(&buf, point_work_bytes(n)));
+    int e = obj_val_run(ctx, rc, G, point, start ? start : rc->rigid_trans, x, nx, C, carve_point_work(buf, n), obj,
+                        residual);
+    hipFree(buf);
+    return e;
 }
 
 int dvc_search_prepare(dvc_ctx *ctx, const dvc_run_params *rc, const double *points, int32_t npts,

@@ -72,9 +72,7 @@ inline Geometry make_geometry(const dvc_run_params &rc) {
 // Sphere sample offsets aspect*rel of one seed (FloatingCloud.cpp:51-78): the same
 // libstdc++ mt19937 and uniform_real_distribution<double> stream, rejection in the
 // unit-aspect ball, then the aspect scaling.
-inline void sphere_template(const dvc_run_params &rc, const Geometry &G, std::vector<double> &sph,
-                            double omn[3], double omx[3]) {
-    sph.assign((size_t)3 * G.subv_num, 0.0);
+inline void sphere_offsets(const dvc_run_params &rc, const Geometry &G, double *sph) {
     std::mt19937 mt(rc.sphere_seed);
     std::uniform_real_distribution<double> dist(-G.subv_rad, G.subv_rad);
     int count = 0;
@@ -88,12 +86,124 @@ inline void sphere_template(const dvc_run_params &rc, const Geometry &G, std::ve
             count += 1;
         }
     }
+}
+
+inline void sphere_template(const dvc_run_params &rc, const Geometry &G, std::vector<double> &sph,
+                            double omn[3], double omx[3]) {
+    sph.assign((size_t)3 * G.subv_num, 0.0);
+    sphere_offsets(rc, G, sph.data());
     for (int q = 0; q < 3; q++) { omn[q] = 1e300; omx[q] = -1e300; }
     for (int i = 0; i < G.subv_num; i++)
         for (int q = 0; q < 3; q++) {
             omn[q] = std::min(omn[q], sph[3 * i + q]);
             omx[q] = std::max(omx[q], sph[3 * i + q]);
         }
+}
+
+// The stable cloud of one point (FloatingCloud.cpp:40-128, as Search.cpp:99-118 builds
+// it): xyz[3 * subv_num], its bounding box box[6] (min xyz, max xyz) grown point by point
+// from the empty box of Cloud's constructor, and its reference point.  Spheres take the
+// offsets sph of sphere_offsets; cubes the g^3 grid over the aspect-scaled cube.
+inline void stable_cloud(const dvc_run_params &rc, const Geometry &G, const double *sph, const double pt[3],
+                         double *xyz, double box[6], double ref[3]) {
+#pragma clang fp contract(off)
+    const int n = G.subv_num;
+    if (rc.geom == DVC_GEOM_CUBE) {
+        double mn[3], mx[3], inc[3];
+        for (int q = 0; q < 3; q++) {
+            mn[q] = pt[q] + -G.subv_rad * rc.aspect[q];
+            mx[q] = pt[q] + G.subv_rad * rc.aspect[q];
+            ref[q] = (mx[q] + mn[q]) / 2.0;
+            inc[q] = (mx[q] - mn[q]) / (G.g - 1);
+        }
+        int j = 0;
+        for (int i = 0; i < G.g; i++)
+            for (int k = 0; k < G.g; k++)
+                for (int l = 0; l < G.g; l++, j++) {
+                    xyz[3 * j + 0] = mn[0] + l * inc[0];
+                    xyz[3 * j + 1] = mn[1] + k * inc[1];
+                    xyz[3 * j + 2] = mn[2] + i * inc[2];
+                }
+    } else {
+        for (int q = 0; q < 3; q++) ref[q] = pt[q];
+        for (int j = 0; j < n; j++)
+            for (int q = 0; q < 3; q++) xyz[3 * j + q] = pt[q] + sph[3 * j + q];
+    }
+    for (int q = 0; q < 3; q++) { box[q] = 9999.99; box[3 + q] = -9999.99; }
+    for (int j = 0; j < n; j++) {
+        double lo[3], hi[3];
+        bool reset = false;
+        for (int q = 0; q < 3; q++) {
+            lo[q] = box[q]; hi[q] = box[3 + q];
+            if (xyz[3 * j + q] < lo[q]) { lo[q] = xyz[3 * j + q]; reset = true; }
+        }
+        for (int q = 0; q < 3; q++)
+            if (xyz[3 * j + q] > hi[q]) { hi[q] = xyz[3 * j + q]; reset = true; }
+        if (reset)
+            for (int q = 0; q < 3; q++) { box[q] = lo[q]; box[3 + q] = hi[q]; }
+    }
+}
+
+// FloatingCloud::affine_to(x, nx) (FloatingCloud.cpp:236-286) with
+// SearchParams::set_param_vect (SearchParams.cpp:53-107) on a fresh parameter vector:
+// translate the stable cloud, then (nx > 3) rotate about the moved reference point,
+// (nx > 6) strain about it, and take the moved cloud's box.
+inline void affine_cloud(const double *stable, int n, const double sbox[6], const double sref[3], const double *x,
+                         int nx, double *moving, double mbox[6]) {
+#pragma clang fp contract(off)
+    double v[12] = {0.0};
+    for (int q = 0; q < nx && q < 12; q++) v[q] = x[q];
+    double A[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, S[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (nx > 3) {
+        const double phi = v[3], the = v[4], psi = v[5];
+        A[0] = cos(the) * cos(phi);
+        A[1] = cos(the) * sin(phi);
+        A[2] = -sin(the);
+        A[3] = sin(psi) * sin(the) * cos(phi) - cos(psi) * sin(phi);
+        A[4] = sin(psi) * sin(the) * sin(phi) + cos(psi) * cos(phi);
+        A[5] = cos(the) * sin(psi);
+        A[6] = cos(psi) * sin(the) * cos(phi) + sin(psi) * sin(phi);
+        A[7] = cos(psi) * sin(the) * sin(phi) - sin(psi) * cos(phi);
+        A[8] = cos(the) * cos(psi);
+    }
+    if (nx > 6) {
+        S[0] = v[6];  S[1] = v[9];  S[2] = v[11];
+        S[3] = v[9];  S[4] = v[7];  S[5] = v[10];
+        S[6] = v[11]; S[7] = v[10]; S[8] = v[8];
+    }
+    double mref[3];
+    for (int q = 0; q < 3; q++) {
+        mref[q] = sref[q] + v[q];
+        mbox[q] = sbox[q] + v[q];
+        mbox[3 + q] = sbox[3 + q] + v[q];
+    }
+    for (int j = 0; j < n; j++) {
+        double p[3];
+        for (int q = 0; q < 3; q++) p[q] = stable[3 * j + q] + v[q];
+        if (nx > 3) {
+            double r[3], o[3];
+            for (int q = 0; q < 3; q++) r[q] = p[q] - mref[q];
+            for (int q = 0; q < 3; q++) o[q] = A[3 * q] * r[0] + A[3 * q + 1] * r[1] + A[3 * q + 2] * r[2];
+            for (int q = 0; q < 3; q++) p[q] = p[q] + (o[q] - r[q]);
+            if (nx > 6) {
+                for (int q = 0; q < 3; q++) r[q] = p[q] - mref[q];
+                for (int q = 0; q < 3; q++) o[q] = S[3 * q] * r[0] + S[3 * q + 1] * r[1] + S[3 * q + 2] * r[2];
+                for (int q = 0; q < 3; q++) p[q] = p[q] + o[q];
+            }
+        }
+        for (int q = 0; q < 3; q++) moving[3 * j + q] = p[q];
+    }
+    if (nx > 3) {
+        double lo[3] = {9999.9, 9999.9, 9999.9}, hi[3] = {-9999.9, -9999.9, -9999.9};
+        bool reset = false;
+        for (int j = 0; j < n; j++)
+            for (int q = 0; q < 3; q++) {
+                if (moving[3 * j + q] < lo[q]) { lo[q] = moving[3 * j + q]; reset = true; }
+                if (moving[3 * j + q] > hi[q]) { hi[q] = moving[3 * j + q]; reset = true; }
+            }
+        if (reset)
+            for (int q = 0; q < 3; q++) { mbox[q] = lo[q]; mbox[3 + q] = hi[q]; }
+    }
 }
 
 // Reorder sample offsets by their unit cells, z slowest and x fastest.  A sample's

@@ -232,22 +232,57 @@ Search::Search(RunControl *run) : rc(run) {
 
 Search::~Search() {}
 
-void Search::process_point(int t, int n, DataCloud *data, int /*test*/) {
-    // Search::starting_param (Search.cpp:277-377) from the neighbours' records
+void Search::starting_param(Point /*srch_pt*/, std::vector<ResultRecord> &neigh_res) {
     double sx = 0.0, sy = 0.0, sz = 0.0;
     int good = 0;
-    for (int m : data->neigh[n]) {
-        const ResultRecord &r = data->results[t][m];
+    for (const ResultRecord &r : neigh_res)
         if (r.status == dvcio::point_good) {
             sx += r.par_min[0]; sy += r.par_min[1]; sz += r.par_min[2];
             good += 1;
         }
-    }
-    dvc_run_params p = params_;
+    for (size_t q = 0; q < par_min.size(); q++) par_min[q] = 0.0;
     if (good != 0) {
         sx /= good; sy /= good; sz /= good;
-        p.rigid_trans[0] = sx; p.rigid_trans[1] = sy; p.rigid_trans[2] = sz;
+        par_min[0] = sx; par_min[1] = sy; par_min[2] = sz;
+    } else {
+        par_min[0] = params_.rigid_trans[0]; par_min[1] = params_.rigid_trans[1]; par_min[2] = params_.rigid_trans[2];
     }
+}
+
+void Search::search_pt_setup(Point srch_pt, std::vector<ResultRecord> &neigh_res) {
+    starting_param(srch_pt, neigh_res);
+    pt_[0] = srch_pt.x(); pt_[1] = srch_pt.y(); pt_[2] = srch_pt.z();
+    for (int q = 0; q < 3; q++) start_[q] = par_min[q];
+    setup_ = true;
+}
+
+double Search::evaluate(const std::vector<double> &x, std::vector<double> *residual) {
+    if (!setup_) throw Device_Fail{DVC_ERR_STATE, "obj_val_at before search_pt_setup"};
+    int32_t n = 0;
+    double v = 0.0;
+    int rcode = dvc_obj_val_at(ctx_, &params_, pt_, start_, x.data(), (int32_t)x.size(), &v, nullptr, 0, &n);
+    if (rcode == DVC_OK && residual) {
+        residual->assign(n, 0.0);
+        rcode = dvc_obj_val_at(ctx_, &params_, pt_, start_, x.data(), (int32_t)x.size(), &v, residual->data(), n, &n);
+    }
+    if (rcode == DVC_ERR_RANGE) throw Range_Fail();
+    check(rcode, ctx_, "dvc_obj_val_at");
+    return v;
+}
+
+double Search::obj_val_at(const std::vector<double> x) { return evaluate(x, nullptr); }
+
+double Search::obj_val_at(const std::vector<double> x, std::vector<double> &residual) {
+    return evaluate(x, &residual);
+}
+
+void Search::process_point(int t, int n, DataCloud *data, int /*test*/) {
+    // Search::search_pt_setup / starting_param (Search.cpp:248-377) from the neighbours' records
+    std::vector<ResultRecord> neigh_res;
+    for (int m : data->neigh[n]) neigh_res.push_back(data->results[t][m]);
+    search_pt_setup(data->points[n], neigh_res);
+    dvc_run_params p = params_;
+    for (int q = 0; q < 3; q++) p.rigid_trans[q] = start_[q];
     // one point whose only neighbour is itself: its start guess is p.rigid_trans
     const double pt[3] = {data->points[n].x(), data->points[n].y(), data->points[n].z()};
     const int32_t ord = 0, nb = 0;

@@ -156,6 +156,18 @@ public:
     // fills srch_data->results[t] as the reference loop does
     void process_all(int t, DataCloud *srch_data, unsigned int N, std::vector<dvc_point_result> &out);
 
+    // Search::search_pt_setup (Search.cpp:248-275): make srch_pt the point obj_val_at
+    // evaluates, with par_min set by starting_param from the neighbours' records; the
+    // correlated volume's interpolation box then sits on srch_pt moved by that start
+    void search_pt_setup(Point srch_pt, std::vector<ResultRecord> &neigh_res);
+    // Search::starting_param (Search.cpp:277-377): par_min = the mean translation of the
+    // good records, else rc->rigid_trans; rotation and strain 0
+    void starting_param(Point srch_pt, std::vector<ResultRecord> &neigh_res);
+    // Search::obj_val_at (Search.h:88-89, Search.cpp:448-488) at the point of the last
+    // search_pt_setup or process_point; throws Range_Fail
+    double obj_val_at(const std::vector<double> x);
+    double obj_val_at(const std::vector<double> x, std::vector<double> &residual);
+
     int bytes_per;
     double subv_rad;
     int subv_num;
@@ -166,6 +178,10 @@ private:
     RunControl *rc;
     dvc_run_params params_;
     dvc_ctx *ctx_;
+    bool setup_ = false;             // a point is set up for obj_val_at
+    double pt_[3] = {0.0, 0.0, 0.0};
+    double start_[3] = {0.0, 0.0, 0.0};
+    double evaluate(const std::vector<double> &x, std::vector<double> *residual);
 };
 
 dvc_run_params run_params(const RunControl *run);

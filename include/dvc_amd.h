@@ -160,6 +160,22 @@ int dvc_interpolate_box(dvc_ctx *ctx, int slot, const double region_min[3], cons
 int dvc_objective(dvc_ctx *ctx, int kind, const double *ref, const double *tar, int32_t n,
                   double *obj, double *residual);
 
+/* Search::obj_val_at(x) and obj_val_at(x, residual) (Search.h:88-89,
+ * Search.cpp:448-488) for one point, in the state Search::search_pt_setup
+ * (Search.cpp:248-275) leaves: the point's FloatingCloud (rc's geometry, aspect and
+ * sphere seed, Search.cpp:99-118) is sampled in the reference volume with the
+ * interpolation box centred on point[3] (ref_subvol); the box is then centred on
+ * point + start[0..2], the start guess's translation (par_min after starting_param;
+ * NULL: rc->rigid_trans, the guess of a point without good neighbours); the cloud
+ * under FloatingCloud::affine_to(x, nx) (1 <= nx <= 12) is sampled in the correlated
+ * volume (tar_subvol), and *obj = the objective of rc->obj over the two.  residual
+ * (may be NULL, else residual_len >= the cloud's size) receives obj_fcn_res's
+ * residual vector; *nsamp (may be NULL) the cloud's size.  DVC_ERR_RANGE where the
+ * reference throws Range_Fail (or, sampling the reference cloud, Intrp_Fail). */
+int dvc_obj_val_at(dvc_ctx *ctx, const dvc_run_params *rc, const double point[3], const double start[3],
+                   const double *x, int32_t nx, double *obj, double *residual, int32_t residual_len,
+                   int32_t *nsamp);
+
 /* ----------------------------------------------------------------- search */
 
 /* The dvc point loop (dvc.cpp:229-303) around Search::process_point

@@ -1151,6 +1151,29 @@ int orc_interp(const orc_run *rc, const void *vol, const double center[3], const
     return it.tri_cub(pts.data(), n, bb, out, grad);
 }
 
+/* Search::obj_val_at (Search.cpp:448-488) after search_pt_setup (Search.cpp:248-275)
+ * for one point: the reference cloud sampled with the box on pt, the box moved to pt +
+ * start (the start guess's translation), the cloud under affine_to(x, nx) sampled in the
+ * correlated volume, the objective (and residual, may be NULL).  Returns the cloud's size,
+ * or -1 where the reference throws (Intrp_Fail sampling the reference, Range_Fail). */
+int orc_obj_val_at(const orc_run *rc, const void *refv, const void *tarv, const double pt[3], const double start[3],
+                   const double *x, int nx, double *obj, double *res) {
+    Searcher s(*rc, make_vol(rc, refv), make_vol(rc, tarv), g_sbi(), 1);
+    s.fc = cloud_for(rc, pt);
+    Pt sp(pt[0], pt[1], pt[2]);
+    s.interp->center_on(sp);
+    s.interp->load(s.ref);
+    if (s.sample(s.fc->stable, s.ref_sv.data(), nullptr) != 0) return -1;
+    Pt off = sp;
+    off.move_by(start[0], start[1], start[2]);
+    s.interp->center_on(off);
+    s.interp->load(s.tar);
+    double o = 0.0;
+    if (s.obj_val_at(x, nx, o, res, nullptr) != 0) return -1;
+    *obj = o;
+    return s.subv_num;
+}
+
 double orc_objective(int kind, const double *ref, const double *tar, int n, double *residual) {
     return objective(kind, ref, tar, n, residual);
 }

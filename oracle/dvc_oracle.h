@@ -69,6 +69,10 @@ int    orc_interp(const orc_run *rc, const void *vol, const double center[3],
                   double *out, double *grad);
 
 double orc_objective(int kind, const double *ref, const double *tar, int n, double *residual);
+/* Search::obj_val_at after search_pt_setup for one point (box of the correlated volume on
+ * pt + start): returns the cloud's size, or -1 on Range_Fail / Intrp_Fail. */
+int orc_obj_val_at(const orc_run *rc, const void *refv, const void *tarv, const double pt[3], const double start[3],
+                   const double *x, int nx, double *obj, double *res);
 
 /* DataCloud::sort_order_neighbors (DataCloud.cpp:69-144). neigh is n x k, k=min(75,n). */
 int    orc_sort_order_neighbors(const double *pts, int n, const double start[3],

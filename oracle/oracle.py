@@ -80,6 +80,7 @@ def _bind(lib: C.CDLL) -> C.CDLL:
         "orc_affine": (I, [P(Run), P(D), P(D), I, P(D), P(D)]),
         "orc_interp": (I, [P(Run), VP, P(D), P(D), I, P(D), I, P(D), P(D)]),
         "orc_objective": (D, [I, P(D), P(D), I, P(D)]),
+        "orc_obj_val_at": (I, [P(Run), VP, VP, P(D), P(D), P(D), I, P(D), P(D)]),
         "orc_sort_order_neighbors": (I, [P(D), I, P(D), P(I), P(I), I]),
         "orc_dvc_run": (I, [P(Run), VP, VP, P(D), I, P(I), P(I), I, I, I, VP, P(D)]),
         "orc_dvc_run_geo": (I, [P(Run), VP, VP, P(D), I, P(I), P(I), I, I, I, P(I), P(I), P(I), VP, P(D)]),
@@ -187,6 +188,24 @@ def objective(kind, ref, tar, residual=False):
     v = load().orc_objective(OBJ[kind] if isinstance(kind, str) else kind, _dp(r), _dp(t), len(r),
                              _dp(res) if residual else None)
     return (v, res) if residual else v
+
+
+def obj_val_at(run, ref, tar, point, start, x, residual=False):
+    """Search::obj_val_at(x[, residual]) after search_pt_setup for one point (the
+    correlated volume's box on point + start); None on Range_Fail."""
+    ref = np.ascontiguousarray(ref)
+    tar = np.ascontiguousarray(tar)
+    p = np.ascontiguousarray(point, dtype=np.float64)
+    s = np.ascontiguousarray(start, dtype=np.float64)
+    xv = np.ascontiguousarray(x, dtype=np.float64)
+    n = load().orc_subv_num(run.geom, run.subvol_npts)
+    res = np.zeros(n) if residual else None
+    o = C.c_double()
+    m = load().orc_obj_val_at(C.byref(run), ref.ctypes.data_as(C.c_void_p), tar.ctypes.data_as(C.c_void_p), _dp(p),
+                              _dp(s), _dp(xv), len(xv), C.byref(o), _dp(res) if residual else None)
+    if m < 0:
+        return None
+    return (o.value, res) if residual else o.value
 
 
 def sort_order_neighbors(points, start=None, nthreads=8):

@@ -238,3 +238,53 @@ def test_translation_grid_from_zero_start(oracle, small_case):
     good = res["status"] == 0
     assert good.sum() == 2
     assert np.abs(res["par"][good, :3] - small_case.truth[good]).max() < 0.05
+
+
+# ------------------------------------------------------------------ obj_val_at
+
+@pytest.fixture(scope="module")
+def shifted_pair():
+    ref = synth.smoothed_noise((64, 64, 64), 1.5, np.uint8, seed=9)
+    t = np.array([2.0, -1.0, 3.0])
+    return ref, synth.translate(ref, t), t
+
+
+@pytest.mark.parametrize("geom", ["cube", "sphere"])
+@pytest.mark.parametrize("obj", ["ssd", "znssd", "sad"])
+def test_obj_val_at_integer_translation(oracle, shifted_pair, geom, obj):
+    """Catmull-Rom commutes with integer shifts.  The correlated volume is the reference
+    moved by an integer t, so obj_val_at(t) with the box on point + t samples the
+    reference's own values: objective and residual vanish.  A warp 0.4 voxel off does
+    not, the 12-DOF vector with zero rotation and strain is the 3-DOF one, and the
+    opposite translation leaves the box (Range_Fail)."""
+    ref, tar, t = shifted_pair
+    run = oracle.make_run(shape=ref.shape, bytes_per=1, ndof=12, obj=obj, geom=geom, subvol_size=15,
+                          subvol_npts=500, disp_max=4)
+    pt = np.array([31.0, 30.5, 32.25])
+    o, res = oracle.obj_val_at(run, ref, tar, pt, t, t, residual=True)
+    scale = 1e-9 if obj != "znssd" else 1e-12
+    assert abs(o) <= scale and np.abs(res).max() <= 1e-6
+    o12 = oracle.obj_val_at(run, ref, tar, pt, t, np.r_[t, np.zeros(9)])
+    assert o12 == o
+    off = oracle.obj_val_at(run, ref, tar, pt, t, t + [0.4, 0.0, 0.0])
+    assert off > 1e3 * max(abs(o), scale)
+    assert oracle.obj_val_at(run, ref, tar, pt, t, -t) is None
+
+
+def test_obj_val_at_composes_cloud_warp_interp_objective(oracle, shifted_pair):
+    """obj_val_at = objective(interp(reference, stable cloud, box on the point),
+    interp(correlated, affine_to(x) cloud, box on point + start)), bit for bit, for a
+    start other than the translation searched."""
+    ref, tar, t = shifted_pair
+    run = oracle.make_run(shape=ref.shape, bytes_per=1, ndof=12, obj="znssd", geom="sphere", subvol_size=17,
+                          subvol_npts=700, disp_max=5)
+    pt = np.array([30.3, 33.1, 29.6])
+    start = np.array([1.0, 0.0, 2.0])
+    x = np.array([2.1, -0.8, 2.9, 0.02, -0.01, 0.015, 0.003, -0.002, 0.001, 0.0005, 0.0, -0.001])
+    o, res = oracle.obj_val_at(run, ref, tar, pt, start, x, residual=True)
+    xs, bs, _ = oracle.cloud(run, pt)
+    xm, bm = oracle.affine(run, pt, x, 12)
+    vr = oracle.interp(run, ref, pt, xs, bs)
+    vt = oracle.interp(run, tar, pt + start, xm, bm)
+    o2, res2 = oracle.objective("znssd", vr, vt, residual=True)
+    assert o == o2 and np.array_equal(res, res2)
