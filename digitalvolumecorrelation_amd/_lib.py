@@ -72,6 +72,7 @@ def load() -> C.CDLL:
         "dvc_interpolate_box": (I, [VP, C.c_int, P(D), P(D), P(D), P(D), I, P(D), C.c_int, P(D), P(D)]),
         "dvc_objective": (I, [VP, C.c_int, P(D), P(D), I, P(D), P(D)]),
         "dvc_obj_val_at": (I, [VP, P(RunParams), P(D), P(D), P(D), I, P(D), P(D), I, P(I)]),
+        "dvc_obj_val_at_noalloc": (I, [VP, P(RunParams), P(D), P(D), P(D), I, P(D), P(D), I, P(I)]),
         "dvc_search_points": (I, [VP, P(RunParams), P(D), I, P(I), I, P(I), I, VP]),
         "dvc_search_prepare": (I, [VP, P(RunParams), P(D), I, P(I), I, P(I), I]),
         "dvc_search_execute": (I, [VP]),

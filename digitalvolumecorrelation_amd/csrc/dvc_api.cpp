@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <random>
 #include <string>
 #include <thread>
@@ -30,7 +31,36 @@ struct Volume {
     int64_t W = 0, H = 0, T = 0;
 };
 
+// The largest cloud a run allows: a cube of 37^3 samples for subvol_npts 50,000
+// (cube counts round up to a cube, Search.cpp:30-31).
+constexpr int kNoAllocSamples = 50653;
+// Device scratch of one dvc_obj_val_at_noalloc call: stable and moved clouds, sample
+// values, residual and objective (9 n + 1 doubles), rounded up to 256 bytes.
+constexpr size_t kNoAllocScratchBytes = (sizeof(double) * (9 * (size_t)kNoAllocSamples + 1) + 255) / 256 * 256;
+constexpr int kMaxDevices = 64;
+
+std::mutex g_scratch_mu;                              // one no-alloc call at a time in this process
+void *g_scratch[kMaxDevices] = {};                    // the scratch array's address on each device
+double g_host_stage[9 * (size_t)kNoAllocSamples];     // sphere offsets, stable and moved clouds
+
 }  // namespace
+
+// dvc_obj_val_at_noalloc's device scratch: one copy on every GPU the library runs on,
+// held until the process ends.  Host code never names it: it uses the address that
+// hipGetSymbolAddress returns for each device (g_scratch).
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+// Set-up of the no-alloc entry point on `device`, which must be current: the scratch
+// array's address there, looked up once per device.
+static int scratch_setup(int device) {
+    if (device < 0 || device >= kMaxDevices) return -1;
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (g_scratch[device]) return 0;
+    void *ptr = nullptr;
+    if (hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return -1;
+    g_scratch[device] = ptr;
+    return 0;
+}
 
 struct dvc_ctx {
     int device = 0;
@@ -251,6 +281,7 @@ int dvc_create(dvc_ctx **out, int device) {
     ctx->cus = prop.multiProcessorCount;
     ctx->lds_max = prop.sharedMemPerBlock;
     if (search_set_lds_limit(ctx->lds_max) != 0) { delete ctx; return DVC_ERR_HIP; }
+    if (scratch_setup(device) != 0) { delete ctx; return DVC_ERR_HIP; }
     if (hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
         delete ctx;
@@ -568,6 +599,24 @@ int dvc_obj_val_at(dvc_ctx *ctx, const dvc_run_params *rc, const double point[3]
                         residual);
     hipFree(buf);
     return e;
+}
+
+int dvc_obj_val_at_noalloc(dvc_ctx *ctx, const dvc_run_params *rc, const double point[3], const double start[3],
+                           const double *x, int32_t nx, double *obj, double *residual, int32_t residual_len,
+                           int32_t *nsamp) {
+    if (!ctx) return DVC_ERR_ARG;
+    Geometry G;
+    if (int e = obj_val_args(ctx, rc, point, x, nx, obj, residual, residual_len, G)) return e;
+    const int n = G.subv_num;
+    if (n > kNoAllocSamples) return fail(ctx, DVC_ERR_ARG, "cloud larger than the no-alloc scratch (50,653 samples)");
+    if (nsamp) *nsamp = n;
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    void *dev = ctx->device >= 0 && ctx->device < kMaxDevices ? g_scratch[ctx->device] : nullptr;
+    if (!dev) return fail(ctx, DVC_ERR_STATE, "no scratch address on this device");
+    const size_t m = (size_t)n;
+    PointClouds C{g_host_stage, g_host_stage + 3 * m, g_host_stage + 6 * m};
+    return obj_val_run(ctx, rc, G, point, start ? start : rc->rigid_trans, x, nx, C, carve_point_work(dev, n), obj,
+                       residual);
 }
 
 int dvc_search_prepare(dvc_ctx *ctx, const dvc_run_params *rc, const double *points, int32_t npts,

@@ -176,6 +176,19 @@ int dvc_obj_val_at(dvc_ctx *ctx, const dvc_run_params *rc, const double point[3]
                    const double *x, int32_t nx, double *obj, double *residual, int32_t residual_len,
                    int32_t *nsamp);
 
+/* dvc_obj_val_at without any allocation: no counterpart in the reference.  For callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops): the same arguments, results and errors, with the clouds, sample values,
+ * residual and objective in a fixed device array of 3.5 MiB on each GPU (set up by
+ * dvc_create, held until the process ends) and the host-side clouds in a fixed host
+ * array.  Clouds of at most 50,653 samples (every geometry a run allows); larger ones
+ * are DVC_ERR_ARG.  All calls on one GPU share its array, which keeps the last call's
+ * contents; the library serialises the calls of one process, and calls of several
+ * processes each use their own copy. */
+int dvc_obj_val_at_noalloc(dvc_ctx *ctx, const dvc_run_params *rc, const double point[3], const double start[3],
+                           const double *x, int32_t nx, double *obj, double *residual, int32_t residual_len,
+                           int32_t *nsamp);
+
 /* ----------------------------------------------------------------- search */
 
 /* The dvc point loop (dvc.cpp:229-303) around Search::process_point
