@@ -159,6 +159,7 @@ int main(int argc, char *argv[]) {
     in.append_time_date(run.sta_fname, "Run start:\t", t_start);
     in.append_time_date(run.sta_fname, "Run finish:\t", t_end);
     std::ofstream sta(run.sta_fname, std::ios::out | std::ios::app);
+    sta << dvc::neighbour_method(npts) << std::endl << std::endl;
     sta << std::setprecision(0) << std::fixed;
     sta << N << " points processed in " << elapsed << " seconds" << std::endl;
     sta << std::setprecision(3) << std::fixed;

@@ -127,6 +127,13 @@ bool grid_neighbours(long long n) {
     return n > kExactNeighbourLimit;
 }
 
+std::string neighbour_method(long long n) {
+    if (!grid_neighbours(n)) return "neighbour tables: full sort per point (DataCloud.cpp:95-140 tie order)";
+    return "neighbour tables: grid k-NN, equal distances in point-index order (the reference's std::sort tie order "
+           "is not reproduced above " + std::to_string(kExactNeighbourLimit) +
+           " points; DVC_NEIGHBOURS=exact forces the full sort)";
+}
+
 void DataCloud::sort_order_neighbors(Point start) {
     const int n = (int)points.size();
     std::vector<double> p((size_t)3 * n);

@@ -98,6 +98,8 @@ public:
 // DVC_NEIGHBOURS=exact or =grid chooses explicitly.
 constexpr long long kExactNeighbourLimit = 20000;
 bool grid_neighbours(long long n);
+// One line naming the neighbour method used for n points, for the .stat file.
+std::string neighbour_method(long long n);
 
 // A process-wide device context for the objects below (device from DVC_DEVICE, else 0).
 dvc_ctx *shared_context();

@@ -41,6 +41,7 @@ def test_dvc_and_strain_programs(tmp_path, bins, oracle):
     assert np.all(uvw[~good] == 0.0) and np.all(obj[~good] == 0.0)
     sta = (d / "result.stat").read_text()
     assert f"number successful = {int(good.sum())}" in sta and "pt/sec" in sta
+    assert "neighbour tables: full sort per point" in sta
 
     r = U.run([bins / "strain", "-sw", "27", "-A", "result.disp"], d, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
