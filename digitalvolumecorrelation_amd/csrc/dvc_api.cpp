@@ -927,7 +927,7 @@ static int enqueue_search_only(dvc_ctx *ctx, hipStream_t st) {
     if (launch_zero_int((int *)ctx->d_res, res_words, st) != 0)
         return fail(ctx, DVC_ERR_HIP, "zero launch");
     a.ref_order = ctx->d_ref_order;
-    if (a.rvals && launch_reference(a, ctx->ref_threads, (size_t)a.nsamp_pad * sizeof(float), st) != 0)
+    if (a.rvals && launch_reference(a, ctx->ref_threads, st) != 0)
         return fail(ctx, DVC_ERR_HIP, "reference kernel launch failed");
     a.ref_order = nullptr;
     if (ctx->plan_mode == DVC_START_CHAIN) {

@@ -106,7 +106,7 @@ int launch_objective(int kind, const double *ref, const double *tar, int n, doub
 int launch_strain(const StrainArgs &a, void *stream);
 int launch_fields(const dvc_point_result *res, const int *pos, int npts, int nproc, double *disp, int *status,
                   double *objmin, void *stream);
-int launch_reference(const SearchArgs &a, int block_threads, size_t lds_bytes, void *stream);
+int launch_reference(const SearchArgs &a, int block_threads, void *stream);
 int launch_zero_int(int *p, long long n, void *stream);
 int launch_judge(const int *order, int n, const dvc_point_result *prop, const dvc_point_result *judge,
                  dvc_point_result *out, int *kept, int *repair, void *stream);

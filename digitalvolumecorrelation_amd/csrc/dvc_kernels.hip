@@ -2208,8 +2208,7 @@ int search_static_lds(int bpv, int block_threads, size_t *bytes) {
     return -1;
 }
 
-int launch_reference(const SearchArgs &a, int block_threads, size_t lds, void *stream) {
-    (void)lds;
+int launch_reference(const SearchArgs &a, int block_threads, void *stream) {
     if (a.nproc <= 0) return 0;
     hipStream_t st = (hipStream_t)stream;
     const int bpv = a.bpv;

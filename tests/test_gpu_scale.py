@@ -10,6 +10,8 @@ and size-independent properties over every point.
   sinusoids), 100,000 random points, cube 31 / 2197, ZNSSD, 12-DOF: the serial chain
   against the oracle on a 16-point prefix, then the judged proposal mode, whose selection
   rule is checked point by point.
+* Volume upload at full size: a 1536^3 uint16 volume from host memory, little-endian and
+  big-endian behind a header, against its download.
 * Full scale: 1536^3 uint16 pair (8 Gaussian bumps of amplitude 4 and width 200 plus 1 %
   volumetric strain), 10^6 points on a 100^3 grid at spacing 14, cube 41 / 8000, ZNSSD,
   12-DOF: the oracle on a 16-point prefix; every point good, error against the imposed
@@ -19,6 +21,8 @@ Bar against the oracle (Core/dvc.cpp:233-303 around Search::process_point): stat
 iteration counts equal, displacements within 1e-3 voxel, objectives within 1e-5 relative
 (points that end at the 20-evaluation limit: 1e-4, as in test_gpu_start_modes).
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -203,3 +207,31 @@ def test_full_scale_1536_prefix_and_properties(ctx, oracle, full):
     del ref, tar
     idx = order[:M]
     _close(g[idx], o[idx], "full scale")
+
+
+@pytest.mark.timeout(300)
+def test_volume_upload_full_size_both_endians(ctx):
+    """dvc_volume_upload of a 1536^3 uint16 volume (6.75 GiB) from host memory: once
+    little-endian without a header, once big-endian after a 37-byte header (swapped on
+    the device; the reference swaps while reading, Interpolate.cpp:219-250).  Both device
+    copies equal the host volume, voxel for voxel."""
+    n = 1536
+    vol = np.empty((n, n, n), np.uint16)
+    i = np.arange(n, dtype=np.uint16)
+    plane = i[None, :] * np.uint16(7) + i[:, None] * np.uint16(3) + (i[None, :] * i[:, None]) % np.uint16(251)
+    for z in range(n):
+        vol[z] = plane + np.uint16((z * 11) & 0xFFFF)
+    ctx.upload(0, vol)
+    back = ctx.download(0, vol.shape, np.uint16)
+    assert np.array_equal(back, vol)
+    header = b"DVC-HEADER-" + bytes(range(26))
+    raw = np.empty(len(header) + vol.nbytes, np.uint8)
+    raw[:len(header)] = np.frombuffer(header, np.uint8)
+    raw[len(header):].view(">u2").reshape(vol.shape)[...] = vol
+    first = np.frombuffer(raw[len(header):len(header) + 4096].tobytes(), ">u2")
+    assert np.array_equal(first, vol.reshape(-1)[:2048]) and raw[len(header) + 3] == vol[0, 0, 1] & 0xFF
+    rc = ctx.lib.dvc_volume_upload(ctx.h, 1, raw.ctypes.data_as(C.c_void_p), len(header), 2, 1, n, n, n)
+    assert rc == 0
+    del raw
+    ctx.lib.dvc_volume_download(ctx.h, 1, back.ctypes.data_as(C.c_void_p))
+    assert np.array_equal(back, vol)
