@@ -696,28 +696,28 @@ __device__ __forceinline__ bool eval_begin(const SearchArgs &a, PtShared &S, int
         if (ndof > 3 && lane < 3) sincos(S.evalpar[3 + lane], &sv, &cv);
         const double trig[6] = {rdlane(cv, 0), rdlane(sv, 0), rdlane(cv, 1), rdlane(sv, 1), rdlane(cv, 2),
                                 rdlane(sv, 2)};
-        if (tid == 0) set_warp(S, ndof, trig);
-    }
-    if (tid == 0) {
-        // non-finite parameters (singular normal matrix) are a range failure; a flag of
-        // its own, so that thread 0 writing S.fail below cannot race with this read
-        S.pfail = 0;
-        for (int q = 0; q < ndof; q++)
-            if (!isfinite(S.evalpar[q])) S.pfail = 1;
-    }
-    __syncthreads();
-    if (S.pfail) return false;
-    // Range rule.  With 3 DOF the moving bounding box is the stable one shifted; with
-    // more it is accumulated in the sample pass, which then runs speculatively: every
-    // sample is clamped into the interpolation box (inside the volume), and a failed
-    // range check discards the sums.
-    if (tid == 0) {
+        if (tid == 0) {
 #pragma clang fp contract(off)
-        if (ndof <= 3) {
-            for (int q = 0; q < 3; q++) { S.bb[q] = S.sbb_mn[q] + S.u[q]; S.bb[3 + q] = S.sbb_mx[q] + S.u[q]; }
-            S.fail = range_ok(a, S, S.bb) ? 0 : 1;
-        } else {
-            S.fail = S.box_ok ? 0 : 1;
+            set_warp(S, ndof, trig);
+            // non-finite parameters (singular normal matrix) are a range failure
+            int pfail = 0;
+            for (int q = 0; q < ndof; q++)
+                if (!isfinite(S.evalpar[q])) pfail = 1;
+            // Range rule.  With 3 DOF the moving bounding box is the stable one shifted;
+            // with more it is accumulated in the sample pass, which then runs
+            // speculatively: every sample is clamped into the interpolation box (inside
+            // the volume), and a failed range check discards the sums.
+            int fail = pfail;
+            if (!pfail) {
+                if (ndof <= 3) {
+                    for (int q = 0; q < 3; q++) { S.bb[q] = S.sbb_mn[q] + S.u[q]; S.bb[3 + q] = S.sbb_mx[q] + S.u[q]; }
+                    fail = range_ok(a, S, S.bb) ? 0 : 1;
+                } else {
+                    fail = S.box_ok ? 0 : 1;
+                }
+            }
+            S.pfail = pfail;
+            S.fail = fail;
         }
     }
     __syncthreads();
@@ -947,7 +947,7 @@ __device__ __forceinline__ void eval_samples(const SearchArgs &a, PtShared &S, i
 // False (uniformly) on Range_Fail.
 __device__ __forceinline__ bool eval_close(const SearchArgs &a, PtShared &S, int ndof) {
     const int tid = threadIdx.x;
-    if (ndof > 3 && tid == 0) {
+    if (tid == 0 && ndof > 3) {
         for (int q = 0; q < 3; q++) {
             // resize_bbox sentinels (FloatingCloud.cpp:208-214)
             S.bb[q] = fmin(S.bbr[q], 9999.9);
@@ -955,9 +955,7 @@ __device__ __forceinline__ bool eval_close(const SearchArgs &a, PtShared &S, int
         }
         S.fail = range_ok(a, S, S.bb) ? 0 : 1;
     }
-    __syncthreads();
-    if (S.fail) return false;
-    if (tid == 0) {
+    if (tid == 0 && !S.fail) {
         const int kind = a.obj;
         const double nn = (double)a.nsamp;
         const double st = S.sums[0], sx1 = S.sums[1], sx2 = S.sums[2];
@@ -984,7 +982,7 @@ __device__ __forceinline__ bool eval_close(const SearchArgs &a, PtShared &S, int
         }
     }
     __syncthreads();
-    return true;
+    return S.fail == 0;
 }
 
 // The whole evaluation on one workgroup.  With need_j the analytic-Jacobian moments are
