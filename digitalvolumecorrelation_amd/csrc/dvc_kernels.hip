@@ -29,7 +29,7 @@
 //     matrix cores with v_mfma_f32_16x16x4_f32 (A = B = the same staged register),
 //     flushing the FP32 accumulator into FP64 every 64 samples.
 //   * Objective values are FP64 sums over the FP32 sample values; the 12x12 step solve
-//     (Cholesky or LDL^T of the normal equations, with column-pivoted Householder QR as
+//     (Cholesky or Gauss-Jordan of the normal equations, with column-pivoted Householder QR as
 //     Eigen's colPivHouseholderQr for (near-)singular systems) and all point geometry
 //     (cloud, warp, bounding boxes, range checks) are FP64 with the reference's
 //     operation order and no fused multiply-add, so statuses follow the reference's
@@ -79,6 +79,49 @@ __device__ __forceinline__ double wave_max(double x) {
     for (int o = 32; o > 0; o >>= 1) x = fmax(x, __shfl_xor(x, o, 64));
     return x;
 }
+
+// Totals over the 64 lanes of a fully active wavefront by data-parallel primitives (DPP)
+// instead of LDS permutes: pairs, quads, half-rows and rows of 16 by quad_perm and row
+// mirrors, then row 15 into rows 1 and 3 (row_bcast:15) and row 31 into rows 2 and 3
+// (row_bcast:31), the total landing in lane 63, which is read back as a uniform value.
+// Each step moves the two 32-bit halves of the double.  Rows a broadcast step leaves out
+// take `old` (0 for sums, the value itself for min / max).
+#ifdef DVC_EMU
+__device__ __forceinline__ double wave_total_sum(double x) { return wave_sum(x); }
+__device__ __forceinline__ double wave_total_min(double x) { return wave_min(x); }
+__device__ __forceinline__ double wave_total_max(double x) { return wave_max(x); }
+#else
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_f64(double src, double old) {
+    const long long s = __double_as_longlong(src), o = __double_as_longlong(old);
+    const int lo = __builtin_amdgcn_update_dpp((int)o, (int)s, CTRL, ROWS, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(s >> 32), CTRL, ROWS, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+template <class Op>
+__device__ __forceinline__ double wave_total(double x, Op op, bool sum) {
+    constexpr int kXor1 = 0xB1, kXor2 = 0x4E;                  // quad_perm [1,0,3,2], [2,3,0,1]
+    constexpr int kHalfMirror = 0x141, kMirror = 0x140, kBcast15 = 0x142, kBcast31 = 0x143;
+    x = op(x, dpp_f64<kXor1, 0xf>(x, x));
+    x = op(x, dpp_f64<kXor2, 0xf>(x, x));
+    x = op(x, dpp_f64<kHalfMirror, 0xf>(x, x));
+    x = op(x, dpp_f64<kMirror, 0xf>(x, x));
+    x = op(x, dpp_f64<kBcast15, 0xa>(x, sum ? 0.0 : x));
+    x = op(x, dpp_f64<kBcast31, 0xc>(x, sum ? 0.0 : x));
+    long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readlane((int)b, 63), hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+__device__ __forceinline__ double wave_total_sum(double x) {
+    return wave_total(x, [](double u, double v) { return u + v; }, true);
+}
+__device__ __forceinline__ double wave_total_min(double x) {
+    return wave_total(x, [](double u, double v) { return fmin(u, v); }, false);
+}
+__device__ __forceinline__ double wave_total_max(double x) {
+    return wave_total(x, [](double u, double v) { return fmax(u, v); }, false);
+}
+#endif
 
 // Deterministic block sum of K doubles per thread; results land in out[0..K).
 template <int B, int K>
@@ -903,9 +946,9 @@ __device__ __forceinline__ void eval_samples(const SearchArgs &a, PtShared &S, i
     }
     // block totals of the sums and of the moving bounding box
     {
-        double v3[3] = {wave_sum(s_tc), wave_sum(s_x1), wave_sum(s_x2)};
+        double v3[3] = {wave_total_sum(s_tc), wave_total_sum(s_x1), wave_total_sum(s_x2)};
         if (merged)
-            for (int q = 0; q < 3; q++) { mn[q] = wave_min(mn[q]); mx[q] = wave_max(mx[q]); }
+            for (int q = 0; q < 3; q++) { mn[q] = wave_total_min(mn[q]); mx[q] = wave_total_max(mx[q]); }
         if (lane == 0) {
             for (int q = 0; q < 3; q++) red[wid * 9 + q] = v3[q];
             for (int q = 0; q < 3; q++) { red[wid * 9 + 3 + q] = mn[q]; red[wid * 9 + 6 + q] = mx[q]; }
@@ -997,49 +1040,6 @@ __device__ __forceinline__ bool evaluate(const SearchArgs &a, PtShared &S, int n
 
 // ------------------------------------------------------------- step solve
 
-
-// One entry of the objective's Gauss-Newton normal equations (Search.cpp:234's system,
-// as LM_prep_at assembles it) from the block's moments V: (J^T J)_ij for j < N, the
-// right-hand side (-J^T e)_i for j == N.  The same arithmetic as normal_cols.
-__device__ __forceinline__ double normal_entry(const SearchArgs &a, const PtShared &S, const double *V, int i, int j,
-                                               int N) {
-#pragma clang fp contract(off)
-    const double n = (double)a.nsamp;
-    const int kind = a.obj;
-    const bool rhs = j == N;
-    const int jj = rhs ? 0 : j;
-    const double Sgj = V[jj * 16 + 14];
-    if (kind == DVC_OBJ_SAD || kind == DVC_OBJ_SSD) {
-        if (!rhs) return V[i * 16 + jj];
-        return -(V[i * 16 + 12] - V[i * 16 + 13]);
-    } else if (kind == DVC_OBJ_ZSSD) {
-        const double tm = S.tmean, Src = V[13 * 16 + 14];
-        if (!rhs) return V[i * 16 + jj] - V[i * 16 + 14] * Sgj / n;
-        const double Sg = V[i * 16 + 14];
-        return -((V[i * 16 + 12] - tm * Sg) - V[i * 16 + 13] + (Sg / n) * Src);
-    } else if (kind == DVC_OBJ_NSSD) {
-        const double T2 = S.T2, T = sqrt(T2), Rn = sqrt(S.Rn2), rm = S.rmean;
-        const double ci = V[i * 16 + 12] + rm * V[i * 16 + 14];
-        if (!rhs) {
-            const double cj = V[jj * 16 + 12] + rm * Sgj;
-            return V[i * 16 + jj] / T2 - ci * cj / (T2 * T2);
-        }
-        const double Sgr = V[i * 16 + 13] + rm * V[i * 16 + 14];
-        return -(-Sgr / (T * Rn) + ci * S.Strr / (T2 * T * Rn));
-    }
-    const double tm = S.tmean, N2 = S.N2, Nn = sqrt(N2), R = sqrt(S.R2);
-    const double ci = V[i * 16 + 12] - tm * V[i * 16 + 14];
-    if (!rhs) {
-        const double cj = V[jj * 16 + 12] - tm * Sgj;
-        return (V[i * 16 + jj] - V[i * 16 + 14] * Sgj / n) / N2 - ci * cj / (N2 * N2);
-    }
-    // sum_j (g_ij - mean g_i) (r_j - mean r): the FP32-centred reference samples sum to
-    // Src, not exactly zero
-    const double Sg = V[i * 16 + 14];
-    const double gr = V[i * 16 + 13] - Sg * S.Src / n;
-    return -(-gr / (Nn * R) + ci * S.Str / (N2 * Nn * R));
-}
-
 // Cholesky solve of the symmetric normal equations held one column per lane (lane j < N:
 // column j of A, lane N: right-hand side b), on wavefront 0.  Step k takes the pivot
 // A_kk and the column L_ik (i > k) of the updated trailing matrix from lane k, both
@@ -1089,8 +1089,8 @@ __device__ __forceinline__ bool chol_solve(double (&col)[N], int lane, double (&
     return true;
 }
 
-// Reciprocals of the scale factors of the normal equations (normal_entry), once per step
-// for the uniform solve: its entries are then products, not quotients.
+// Reciprocals of the scale factors of the normal equations (normal_cols), once per step
+// for the slice launches' solve: its entries are then products, not quotients.
 struct NormalScale {
     double inv_n, inv_a, inv_b, inv_c, inv_d;
 };
@@ -1108,44 +1108,6 @@ __device__ __forceinline__ NormalScale normal_scale(const SearchArgs &a, const P
     return f;
 }
 
-// normal_entry with the quotients replaced by products with the reciprocals in f.
-__device__ __forceinline__ double normal_entry_scaled(const SearchArgs &a, const PtShared &S, const double *V, int i,
-                                                      int j, int N, const NormalScale &f) {
-#pragma clang fp contract(off)
-    const int kind = a.obj;
-    const bool rhs = j == N;
-    const int jj = rhs ? 0 : j;
-    const double Sgj = V[jj * 16 + 14], Sg = V[i * 16 + 14];
-    if (kind == DVC_OBJ_SAD || kind == DVC_OBJ_SSD) {
-        if (!rhs) return V[i * 16 + jj];
-        return -(V[i * 16 + 12] - V[i * 16 + 13]);
-    } else if (kind == DVC_OBJ_ZSSD) {
-        const double tm = S.tmean, Src = V[13 * 16 + 14];
-        if (!rhs) return V[i * 16 + jj] - Sg * Sgj * f.inv_n;
-        return -((V[i * 16 + 12] - tm * Sg) - V[i * 16 + 13] + (Sg * f.inv_n) * Src);
-    } else if (kind == DVC_OBJ_NSSD) {
-        const double rm = S.rmean;
-        const double ci = V[i * 16 + 12] + rm * Sg;
-        if (!rhs) return V[i * 16 + jj] * f.inv_a - ci * (V[jj * 16 + 12] + rm * Sgj) * f.inv_b;
-        return -(-(V[i * 16 + 13] + rm * Sg) * f.inv_c + ci * S.Strr * f.inv_d);
-    }
-    const double tm = S.tmean;
-    const double ci = V[i * 16 + 12] - tm * Sg;
-    if (!rhs) return (V[i * 16 + jj] - Sg * Sgj * f.inv_n) * f.inv_a - ci * (V[jj * 16 + 12] - tm * Sgj) * f.inv_b;
-    const double gr = V[i * 16 + 13] - Sg * S.Src * f.inv_n;
-    return -(-gr * f.inv_c + ci * S.Str * f.inv_d);
-}
-
-// Solve of the symmetric normal equations A x = b by an LDL^T factorisation, the same on
-// every lane of the wave (A's lower triangle row by row, then b, in `e`: every lane holds
-// the whole system in registers, so no step exchanges values between lanes).  Step k
-// takes the pivot D_k = A_kk of the updated trailing matrix and its reciprocal, the
-// multipliers L_ik = A_ik / D_k and the update A_ij -= L_ik A_jk; the forward
-// substitution runs alongside, then x = L^-T (D^-1 y).  Each step has one division on its
-// dependency chain and no square root.  Returns false, having computed nothing usable,
-// when a pivot is not above 1e-9 of the largest diagonal entry (or is not finite): the
-// caller then solves by column-pivoted QR, whose rank rule covers the (near-)singular
-// systems.  (D_k is the square of the Cholesky pivot, so the test is the same.)
 // 1 / d for a positive finite d: the hardware reciprocal estimate and two Newton steps
 // r' = r + r (1 - d r), which carry its error below double rounding (within an ulp of
 // the quotient).  Shorter on the dependency chain than the division.
@@ -1156,42 +1118,83 @@ __device__ __forceinline__ double fast_rcp(double d) {
     return r;
 }
 
+// Row i of the objective's Gauss-Newton normal equations (Search.cpp:234's system, as
+// LM_prep_at assembles it) from the block's moments V, for gj_rows: r[j] = (J^T J)_ij
+// from the lower triangle's entry (max(i, j), min(i, j)), as the other solves read it,
+// and r[N] = (-J^T e)_i.  The arithmetic of normal_cols, with the quotients replaced by
+// products with the reciprocals in f.
 template <int N>
-__device__ __forceinline__ bool ldl_uniform(double (&e)[N * (N + 1) / 2 + N]) {
+__device__ __forceinline__ void normal_row(const SearchArgs &a, const PtShared &S, const double *V, int i,
+                                           const NormalScale &f, double (&r)[N + 1]) {
 #pragma clang fp contract(off)
-#define A_(i, j) e[(i) * ((i) + 1) / 2 + (j)]
-#define B_(i) e[N * (N + 1) / 2 + (i)]
+    const int kind = a.obj;
+    const double Sg = V[i * 16 + 14];
+    auto vl = [&](int j) { return i > j ? V[i * 16 + j] : V[j * 16 + i]; };
+    if (kind == DVC_OBJ_SAD || kind == DVC_OBJ_SSD) {
+#pragma unroll
+        for (int j = 0; j < N; j++) r[j] = vl(j);
+        r[N] = -(V[i * 16 + 12] - V[i * 16 + 13]);
+    } else if (kind == DVC_OBJ_ZSSD) {
+        const double tm = S.tmean, Src = V[13 * 16 + 14];
+#pragma unroll
+        for (int j = 0; j < N; j++) r[j] = vl(j) - Sg * V[j * 16 + 14] * f.inv_n;
+        r[N] = -((V[i * 16 + 12] - tm * Sg) - V[i * 16 + 13] + (Sg * f.inv_n) * Src);
+    } else if (kind == DVC_OBJ_NSSD) {
+        const double rm = S.rmean;
+        const double ci = V[i * 16 + 12] + rm * Sg;
+#pragma unroll
+        for (int j = 0; j < N; j++) r[j] = vl(j) * f.inv_a - ci * (V[j * 16 + 12] + rm * V[j * 16 + 14]) * f.inv_b;
+        r[N] = -(-(V[i * 16 + 13] + rm * Sg) * f.inv_c + ci * S.Strr * f.inv_d);
+    } else {
+        const double tm = S.tmean;
+        const double ci = V[i * 16 + 12] - tm * Sg;
+#pragma unroll
+        for (int j = 0; j < N; j++) {
+            const double Sgj = V[j * 16 + 14];
+            r[j] = (vl(j) - Sg * Sgj * f.inv_n) * f.inv_a - ci * (V[j * 16 + 12] - tm * Sgj) * f.inv_b;
+        }
+        const double gr = V[i * 16 + 13] - Sg * S.Src * f.inv_n;
+        r[N] = -(-gr * f.inv_c + ci * S.Str * f.inv_d);
+    }
+}
+
+// Solve of the symmetric normal equations A x = b by Gauss-Jordan elimination with one
+// row per lane: lane i < N holds row i of A and b_i in r[0..N].  Step k reads the pivot
+// row k from lane k into uniform registers, and every other row subtracts its multiple
+// A_ik / A_kk of it, rows above the pivot as well as below; A ends diagonal, and lane i
+// returns x_i = b_i / A_ii.  The pivots A_kk are those of LDL^T (the diagonals of the
+// successive Schur complements), tested as there: returns false, having computed nothing
+// usable, when a pivot is not above 1e-9 of the largest diagonal entry (or is not
+// finite), and the caller solves by column-pivoted QR.  Lanes at or above N compute
+// nothing used.  Compared with an LDL^T computed whole on every lane, a step needs
+// ~N + 1 dependent multiply-adds instead of the whole trailing update, and a lane holds
+// N + 1 FP64 values instead of the whole system.
+template <int N>
+__device__ __forceinline__ bool gj_rows(double (&r)[N + 1], int lane, double &x) {
+#pragma clang fp contract(off)
     double dmax = 0.0;
 #pragma unroll
-    for (int k = 0; k < N; k++) dmax = fmax(dmax, A_(k, k));
+    for (int k = 0; k < N; k++) dmax = fmax(dmax, rdlane(r[k], k));
     const double tol = 1e-9 * dmax;
+    double rinv_own = 0.0;
 #pragma unroll
     for (int k = 0; k < N; k++) {
-        const double d = A_(k, k);
+        double pr[N + 1];
+#pragma unroll
+        for (int j = k; j <= N; j++) pr[j] = rdlane(r[j], k);
+        const double d = pr[k];
         if (!(d > tol) || !(d < 1.0e300)) return false;
-        const double r = fast_rcp(d);
-        // rows from the bottom up: row i reads column k only at rows j <= i, which still
-        // hold A_jk when row i replaces its own entry by L_ik
+        const double rinv = fast_rcp(d);
+        if (lane == k) {
+            rinv_own = rinv;
+        } else {
+            const double f = r[k] * rinv;
+            r[k] = 0.0;
 #pragma unroll
-        for (int i = N - 1; i > k; i--) {
-            const double l = A_(i, k) * r;
-#pragma unroll
-            for (int j = k + 1; j <= i; j++) A_(i, j) -= l * A_(j, k);
-            B_(i) -= l * B_(k);
-            A_(i, k) = l;
+            for (int j = k + 1; j <= N; j++) r[j] -= f * pr[j];
         }
-        A_(k, k) = r;                 // D_k is not needed again: keep its reciprocal
     }
-    // x = L^-T (D^-1 y), x_i over y_i in place
-#pragma unroll
-    for (int i = N - 1; i >= 0; i--) {
-        double sacc = B_(i) * A_(i, i);
-#pragma unroll
-        for (int q = i + 1; q < N; q++) sacc -= A_(q, i) * B_(q);
-        B_(i) = sacc;
-    }
-#undef A_
-#undef B_
+    x = r[N] * rinv_own;
     return true;
 }
 
@@ -1263,10 +1266,10 @@ __device__ __forceinline__ void normal_cols(const SearchArgs &a, const PtShared 
 
 // One Gauss-Newton step on wavefront 0 (all 64 lanes call it): the normal equations
 // J^T J dx = -J^T e of the objective's residual, assembled from the moments in V, then
-// solved by Cholesky with one column per lane (chol_solve), or with UNIFORM by LDL^T on
-// every lane at once (ldl_uniform; the lanes first build one entry each into the LDS
-// scratch `ne`, N (N + 1) / 2 + N doubles, then every lane reads the whole system): fewer
-// cross-lane steps, more registers.  When the factorisation rejects a pivot the step is
+// solved by Cholesky with one column per lane (chol_solve), or with UNIFORM by
+// Gauss-Jordan with one row per lane (gj_rows; the slice launches' solve, with the
+// shorter dependency chain).  `ne` is LDS scratch of N (N + 1) / 2 + N doubles, unused
+// by these two.  When the factorisation rejects a pivot the step is
 // solved by column-pivoted Householder QR
 // (Eigen 3.4 ColPivHouseholderQR::computeInPlace and _solve_impl, as used at
 // Search.cpp:234): lane j keeps column j of J^T J in registers (lane N the right-hand
@@ -1278,38 +1281,18 @@ __device__ __forceinline__ void gn_step(const SearchArgs &a, PtShared &S, const 
 #pragma clang fp contract(off)
     const int lane = threadIdx.x & 63;
     if (CHOL && UNIFORM) {
-        constexpr int NE = N * (N + 1) / 2 + N;
+        // row `lane` of the system (the lower triangle's entries, as the other solves
+        // read them), then Gauss-Jordan with one row per lane
         const NormalScale f = normal_scale(a, S);
-        for (int q = lane; q < NE; q += 64) {
-            int i, j;
-            if (q < N * (N + 1) / 2) {
-                i = 0;
-                while ((i + 1) * (i + 2) / 2 <= q) i++;
-                j = q - i * (i + 1) / 2;
-            } else {
-                i = q - N * (N + 1) / 2;
-                j = N;
+        const int i = lane < N ? lane : 0;
+        double r[N + 1];
+        normal_row<N>(a, S, V, i, f, r);
+        double x = 0.0;
+        if (gj_rows<N>(r, lane, x)) {
+            if (lane < N) {
+                S.upd[lane] = x;
+                S.par[lane] += x;
             }
-            ne[q] = normal_entry_scaled(a, S, V, i, j, N, f);
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        double e[NE];
-#pragma unroll
-        for (int q = 0; q < NE; q++) e[q] = ne[q];
-        const bool ok = ldl_uniform<N>(e);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        if (ok) {
-            if (lane == 0)
-#pragma unroll
-                for (int i = 0; i < N; i++) {
-                    const double x = e[N * (N + 1) / 2 + i];
-                    S.upd[i] = x;
-                    S.par[i] += x;
-                }
             return;
         }
     }

@@ -99,7 +99,7 @@ def test_gn_solve_cholesky(oracle, n, uniform):
     column-pivoted QR taking over when a pivot is not above 1e-9 of the largest diagonal.
     Well-conditioned and badly scaled systems agree with an LU solve to the conditioning
     bound; singular, near-singular and all-zero systems give the QR's bits.  uniform: the
-    LDL^T of the slice launches, every lane holding the whole system (same pivot rule)."""
+    slice launches' Gauss-Jordan with one row per lane (same pivot rule)."""
     rng = np.random.default_rng(70 + n)
     for scale in (np.ones(n), np.logspace(0, 3, n)):
         J = rng.normal(size=(40, n)) * scale
