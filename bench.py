@@ -196,6 +196,7 @@ def uneven(a, D, synth):
             ctx.set_point_geometry(geom, sub, nsamp)
             ctx.set_start_mode(mode)
             ctx.set_block_threads(threads)
+            ctx.set_slice_launches(0, 0, 0)       # the default shape; the autotuner may change it
             t1 = time.time()
             ctx.prepare(params, case.points, order, neigh)
             prep = time.time() - t1

@@ -110,7 +110,10 @@ struct dvc_ctx {
     // search resumes the points still searching (0 automatic), samples per workgroup
     int slice_wgs = 0, slice_evals = 0, slice_per = 512;
     std::vector<char> sliced;                  // chain launches run as slice launches
+    std::vector<int> slice_nsl;                // workgroups per point of each chain launch (when sliced)
     int slice_cnt8 = 0, slice_n = 0;           // buffer shape of the current plan
+    std::vector<int> level_pts_h;              // points of the chain launches (host copy of d_levels)
+    std::vector<int> pt_nsamp;                 // samples of each point (per-point geometry only)
     unsigned char *d_state = nullptr;          // [2][cnt8][kStateStride]
     double *d_part = nullptr;                  // [2][cnt8][nslice][kPartStride]
     // the strain fit of the search's results after every execute (ndp 0: none)
@@ -173,6 +176,7 @@ void free_plan(dvc_ctx *c) {
     c->d_state = nullptr; c->d_part = nullptr;
     c->slice_cnt8 = c->slice_n = 0;
     c->sliced.clear();
+    c->slice_nsl.clear();
     hipFree(c->d_start); hipFree(c->d_res_prop); hipFree(c->d_res_judge); hipFree(c->d_kept);
     hipFree(c->d_repair); hipFree(c->d_rep_levels);
     hipFree(c->d_sdisp); hipFree(c->d_sobj); hipFree(c->d_sout); hipFree(c->d_sstat); hipFree(c->d_snpts);
@@ -832,7 +836,13 @@ int dvc_search_prepare(dvc_ctx *ctx, const dvc_run_params *rc, const double *poi
     ctx->chosen_threads = bt;
     ctx->ref_threads = 256;
     ctx->level_off = level_off;
+    ctx->level_pts_h = level_pts;
     ctx->rep_off = rep_off;
+    ctx->pt_nsamp.clear();
+    if (uneven) {
+        ctx->pt_nsamp.resize(npts);
+        for (int i = 0; i < npts; i++) ctx->pt_nsamp[i] = pgeo[i].nsamp;
+    }
     ctx->plan_mode = mode;
     ctx->searched.assign(npts, 0);
     for (int i = 0; i < npts; i++) ctx->searched[i] = pos[i] < N ? 1 : 0;
@@ -843,26 +853,42 @@ int dvc_search_prepare(dvc_ctx *ctx, const dvc_run_params *rc, const double *poi
 // Slice launches per chain launch before the one-workgroup search resumes.
 static int slice_evals_of(const dvc_ctx *ctx) { return ctx->slice_evals > 0 ? ctx->slice_evals : 4; }
 
-// Samples per point split into workgroups of ctx->slice_per samples.
+// Samples per point split into workgroups of ctx->slice_per samples (with per-point
+// geometry: of the point with the most samples).
 static int slices_of(const dvc_ctx *ctx) {
-    return (ctx->geo.subv_num + ctx->slice_per - 1) / ctx->slice_per;
+    int ns = ctx->geo.subv_num;
+    for (int v : ctx->pt_nsamp) ns = std::max(ns, v);
+    return (ns + ctx->slice_per - 1) / ctx->slice_per;
 }
 
 // Threads per slice workgroup: one sample per thread up to 512, more above.
 static int slice_threads_of(const dvc_ctx *ctx) { return ctx->slice_per >= kSliceThreads[1] ? kSliceThreads[1] : kSliceThreads[0]; }
 
-// Which chain launches of the prepared search run as slice launches, and their buffers.
-// Slice launches serve the serial chain with one geometry for all points, no translation
-// grid and the HBM table of reference samples.  Called outside stream capture.
+// Which chain launches of the prepared search run as slice launches, their workgroups per
+// point, and their buffers.  Slice launches serve the serial chain with no translation
+// grid and the HBM table of reference samples.  With per-point geometry a chain launch
+// takes the slice count of its point with the most samples.  Called outside stream
+// capture.
 static int plan_slices(dvc_ctx *ctx) {
-    ctx->sliced.assign(ctx->level_off.empty() ? 0 : ctx->level_off.size() - 1, 0);
-    const bool eligible = ctx->plan_mode == DVC_START_CHAIN && ctx->d_rvals && !ctx->d_pgeo &&
-                          ctx->rc.basin_radius == 0.0 && ctx->slice_wgs >= 0;
+    const int nl = ctx->level_off.empty() ? 0 : (int)ctx->level_off.size() - 1;
+    ctx->sliced.assign(nl, 0);
+    ctx->slice_nsl.assign(nl, 0);
+    const bool eligible = ctx->plan_mode == DVC_START_CHAIN && ctx->d_rvals && ctx->rc.basin_radius == 0.0 &&
+                          ctx->slice_wgs >= 0;
     if (!eligible) return DVC_OK;
-    const int nslice = slices_of(ctx);
     const int max_wgs = ctx->slice_wgs > 0 ? ctx->slice_wgs : 4 * ctx->cus;
-    const int cnt8 = slice_plan(ctx->level_off, nslice, max_wgs, kXcds, ctx->sliced);
+    for (int L = 0; L < nl; L++) {
+        int ns = ctx->geo.subv_num;
+        if (!ctx->pt_nsamp.empty()) {
+            ns = 0;
+            for (int q = ctx->level_off[L]; q < ctx->level_off[L + 1]; q++)
+                ns = std::max(ns, ctx->pt_nsamp[ctx->level_pts_h[q]]);
+        }
+        ctx->slice_nsl[L] = (ns + ctx->slice_per - 1) / ctx->slice_per;
+    }
+    const int cnt8 = slice_plan(ctx->level_off, ctx->slice_nsl, max_wgs, kXcds, ctx->sliced);
     if (cnt8 == 0) return DVC_OK;
+    const int nslice = slices_of(ctx);
     if (cnt8 > ctx->slice_cnt8 || nslice > ctx->slice_n || !ctx->d_state) {
         const int c8 = std::max(cnt8, ctx->slice_cnt8), ns = std::max(nslice, ctx->slice_n);
         if (int e = dev_alloc(ctx, &ctx->d_state, (size_t)2 * c8 * kStateStride)) return e;
@@ -933,7 +959,6 @@ static int enqueue_search_only(dvc_ctx *ctx, hipStream_t st) {
     if (ctx->plan_mode == DVC_START_CHAIN) {
         const int depth = (int)ctx->level_off.size() - 1;
         SliceArgs sa{};
-        sa.nslice = slices_of(ctx);
         sa.per = ctx->slice_per;
         for (int q = 0; q < 2; q++) {
             sa.state[q] = ctx->d_state ? ctx->d_state + (size_t)q * ctx->slice_cnt8 * kStateStride : nullptr;
@@ -946,6 +971,7 @@ static int enqueue_search_only(dvc_ctx *ctx, hipStream_t st) {
             if (L < (int)ctx->sliced.size() && ctx->sliced[L]) {
                 sa.level_pts = lp;
                 sa.cnt = cnt;
+                sa.nslice = ctx->slice_nsl[L];
                 sa.cnt8 = (cnt + kXcds - 1) / kXcds * kXcds;
                 for (int t = 0; t < slice_evals_of(ctx); t++) {
                     sa.t = t;
@@ -1187,18 +1213,19 @@ int dvc_search_autotune(dvc_ctx *ctx, int reps, int32_t *best_threads, float *be
         }
         ctx->ref_threads = best_ref;
     }
-    // slice launches of the chain (unless switched off): off, and a few widths and
-    // launch counts, the fastest kept
-    if (ctx->plan_mode == DVC_START_CHAIN && ctx->slice_wgs >= 0 && ctx->d_rvals && !ctx->d_pgeo &&
-        ctx->rc.basin_radius == 0.0) {
-        const int keep_w = ctx->slice_wgs, keep_e = ctx->slice_evals;
+    // slice launches of the chain (unless switched off): off, and a few widths, launch
+    // counts and samples per workgroup (more launches and larger slices for the long
+    // searches and large clouds of per-point geometry), the fastest kept
+    if (ctx->plan_mode == DVC_START_CHAIN && ctx->slice_wgs >= 0 && ctx->d_rvals && ctx->rc.basin_radius == 0.0) {
         const int C = ctx->cus;
-        const int cands[][2] = {{-1, 0}, {2 * C, 4}, {4 * C, 3}, {4 * C, 4}, {4 * C, 6}, {8 * C, 4}};
-        int bw = keep_w, be = keep_e;
+        const int cands[][3] = {{-1, 0, 512},     {2 * C, 4, 512},  {4 * C, 3, 512},  {4 * C, 4, 512},  {4 * C, 6, 512},
+                                {8 * C, 4, 512},  {4 * C, 7, 1024}, {8 * C, 7, 1024}, {8 * C, 10, 1024}};
+        int bw = ctx->slice_wgs, be = ctx->slice_evals, bp = ctx->slice_per;
         float bsl = -1.0f;
         for (const auto &c : cands) {
             ctx->slice_wgs = c[0];
             ctx->slice_evals = c[1];
+            ctx->slice_per = c[2];
             drop_graph(ctx);
             if (int e = dvc_search_execute(ctx)) return e;
             float tot = 0.0f;
@@ -1209,10 +1236,11 @@ int dvc_search_autotune(dvc_ctx *ctx, int reps, int32_t *best_threads, float *be
                 tot += ms;
             }
             const float avg = tot / (float)reps;
-            if (bsl < 0.0f || avg < bsl) { bsl = avg; bw = c[0]; be = c[1]; }
+            if (bsl < 0.0f || avg < bsl) { bsl = avg; bw = c[0]; be = c[1]; bp = c[2]; }
         }
         ctx->slice_wgs = bw;
         ctx->slice_evals = be;
+        ctx->slice_per = bp;
         bms = bsl;
     }
     drop_graph(ctx);

@@ -260,21 +260,21 @@ inline void fill_search_args(SearchArgs &a, const dvc_run_params &rc, const Geom
 }
 
 // Slice launches of the serial chain (SliceArgs): which chain launches split their points'
-// samples over several workgroups.  A launch of cnt points is sliced when its cnt8 *
-// nslice workgroups (cnt8: cnt rounded up to a multiple of xcds) are at most max_wgs and
-// there are at least two slices; narrow launches leave most of the chip idle with one
+// samples over several workgroups.  A launch L of cnt points is sliced when its cnt8 *
+// nslice[L] workgroups (cnt8: cnt rounded up to a multiple of xcds) are at most max_wgs
+// and there are at least two slices; narrow launches leave most of the chip idle with one
 // workgroup per point.  sliced[L] marks launch L; returns the largest cnt8 among them
 // (0: none).
-inline int slice_plan(const std::vector<int> &level_off, int nslice, int max_wgs, int xcds,
+inline int slice_plan(const std::vector<int> &level_off, const std::vector<int> &nslice, int max_wgs, int xcds,
                       std::vector<char> &sliced) {
     const int nl = (int)level_off.size() - 1;
     sliced.assign(nl > 0 ? nl : 0, 0);
     int widest = 0;
-    if (nslice < 2 || max_wgs <= 0) return 0;
+    if (max_wgs <= 0) return 0;
     for (int L = 0; L < nl; L++) {
         const int cnt = level_off[L + 1] - level_off[L];
         const int cnt8 = (cnt + xcds - 1) / xcds * xcds;
-        if (cnt > 0 && (long long)cnt8 * nslice <= max_wgs) {
+        if (cnt > 0 && nslice[L] >= 2 && (long long)cnt8 * nslice[L] <= max_wgs) {
             sliced[L] = 1;
             widest = std::max(widest, cnt8);
         }

@@ -58,10 +58,12 @@ struct SearchArgs {
 };
 
 // Slice launches of the serial chain: the points of one chain launch run one launch per
-// Gauss-Newton evaluation, each point's samples split over nslice workgroups.  Block b
-// works on point slot b % cnt8 (level_pts[slot]) and samples [slice*per, slice*per + per)
-// with slice = b / cnt8; cnt8 is cnt rounded up to a multiple of the XCD count, so the
-// slices of one point share an XCD (and its L2).  No workgroup communicates with another
+// Gauss-Newton evaluation, each point's samples split over nslice workgroups (with
+// per-point geometry, ceil(samples / per) of them; nslice is the launch's largest).
+// Block b works on point slot b % cnt8 (level_pts[slot]) and slice b / cnt8, which takes
+// the point's chunks of 64 samples slice, slice + m, slice + 2 m, ... (m: the point's
+// slice count); cnt8 is cnt rounded up to a multiple of the XCD count, so the slices of
+// one point share an XCD (and its L2).  No workgroup communicates with another
 // inside a launch: launch t writes every slice's sums and, from slice 0, the point's state
 // into buffers t & 1; launch t + 1 reads them (every workgroup of a point adds the
 // slices up, in slice order, and takes the same step) and writes buffers (t + 1) & 1.
@@ -70,7 +72,7 @@ constexpr int kPartStride = 272;     // doubles of one slice's partial sums: 3 s
 struct SliceArgs {
     const int *level_pts;            // points of the chain launch
     int cnt, cnt8;                   // points; cnt rounded up to a multiple of 8
-    int nslice, per;                 // workgroups per point; samples per workgroup
+    int nslice, per;                 // workgroups per point (the launch's largest); samples per workgroup
     int t;                           // launch index within the chain launch (0: set-up and start guess)
     unsigned char *state[2];         // [cnt8][kStateStride] carried point state, by launch parity
     double *part[2];                 // [cnt8][nslice][kPartStride] slice sums, by launch parity

@@ -1926,27 +1926,43 @@ __global__ void __launch_bounds__(B) search_kernel(SearchArgs a, const int *leve
 // The points of a chain launch after its slice launches (rs.t of them): workgroup b
 // continues point level_pts[b] from the state and slice sums that slice launch rs.t - 1
 // left.  Finished points are skipped; the others end their sliced evaluation and run
-// their remaining Gauss-Newton iterations here, one workgroup per point.
+// their remaining Gauss-Newton iterations here, one workgroup per point.  nsl: the point's
+// slices (rs.nslice, or fewer for a point with fewer samples than the launch's largest).
 template <int BPV, int B>
-__global__ void __launch_bounds__(B) resume_kernel(SearchArgs a, const int *level_pts, SliceArgs rs) {
-    DVC_DYN_LDS(float, dyn);
-    float *stage = dyn;
-    DVC_SHARED(PtShared, S);
-    DVC_SHARED(double[(B / 64) * 9], red);
-    DVC_SHARED(double[256], V);
+__device__ __forceinline__ void resume_body(const SearchArgs &a, const int *level_pts, const SliceArgs &rs, int nsl,
+                                            float *stage, PtShared &S, double *red, double *V) {
     const int n = level_pts[blockIdx.x];
     const int p = a.pos[n];
     const int par = (rs.t - 1) & 1;
     const double *sp = rs.part[par] + (size_t)blockIdx.x * rs.nslice * kPartStride;
     double pre[kPrefetch];
-    prefetch_slices<B>(sp, rs.nslice, pre);
+    prefetch_slices<B>(sp, nsl, pre);
     state_load<B>(S, rs.state[par] + (size_t)blockIdx.x * kStateStride);
     __syncthreads();
     if (S.done) return;
-    const int r = finish_sliced<B, false>(a, S, sp, rs.nslice, V, stage, pre);
+    const int r = finish_sliced<B, false>(a, S, sp, nsl, V, stage, pre);
     if (r == 0) gn_iterations<BPV, B>(a, S, a.rvals + (size_t)p * a.nsamp_pad, stage, red, V, S.it, S.obj_old);
     __syncthreads();
     if (threadIdx.x == 0) write_result(a, S, n);
+}
+
+// UNEVEN: every point its own geometry (SearchArgs::pgeo), copied into an LDS SearchArgs
+// first, as search_kernel does.
+template <int BPV, int B, bool UNEVEN>
+__global__ void __launch_bounds__(B) resume_kernel(SearchArgs a, const int *level_pts, SliceArgs rs) {
+    DVC_DYN_LDS(float, dyn);
+    DVC_SHARED(PtShared, S);
+    DVC_SHARED(double[(B / 64) * 9], red);
+    DVC_SHARED(double[256], V);
+    if constexpr (UNEVEN) {
+        DVC_SHARED(SearchArgs, la);
+        const int n = level_pts[blockIdx.x];
+        if (threadIdx.x == 0) apply_geo(la, a, n);
+        __syncthreads();
+        resume_body<BPV, B>(la, level_pts, rs, (la.nsamp + rs.per - 1) / rs.per, dyn, S, red, V);
+    } else {
+        resume_body<BPV, B>(a, level_pts, rs, rs.nslice, dyn, S, red, V);
+    }
 }
 
 // One slice launch (SliceArgs) of the points of a chain launch.  Launch t ends evaluation
@@ -1955,17 +1971,12 @@ __global__ void __launch_bounds__(B) resume_kernel(SearchArgs a, const int *leve
 // set-up of evaluation t and its own slice of the samples, and writes the slice's sums;
 // slice 0 also writes the point's state.  A point whose search ended writes its result
 // (slice 0) and carries its finished state forward.  Every workgroup of a point computes
-// the same set-up, sums and step from the same inputs, so all agree.
+// the same set-up, sums and step from the same inputs, so all agree.  nsl: the point's
+// slices; sa.nslice, the launch's largest, is the stride of the slice records.
 template <int BPV, int BS>
-__global__ void __launch_bounds__(BS) slice_kernel(SearchArgs a, SliceArgs sa) {
-    DVC_DYN_LDS(float, dyn);
-    float *stage = dyn;
-    DVC_SHARED(PtShared, S);
-    DVC_SHARED(double[(BS / 64) * 9], red);
-    DVC_SHARED(double[256], V);
+__device__ __forceinline__ void slice_body(const SearchArgs &a, const SliceArgs &sa, int slot, int slice, int nsl,
+                                           float *stage, PtShared &S, double *red, double *V) {
     const int tid = threadIdx.x, lane = tid & 63;
-    const int slot = (int)blockIdx.x % sa.cnt8, slice = (int)blockIdx.x / sa.cnt8;
-    if (slot >= sa.cnt) return;
     const int n = sa.level_pts[slot];
     const int p = a.pos[n];
     const int cur = sa.t & 1, prev = cur ^ 1;
@@ -1989,14 +2000,14 @@ __global__ void __launch_bounds__(BS) slice_kernel(SearchArgs a, SliceArgs sa) {
         // the slice sums are loaded while the state is: both come from launch t - 1
         const double *sp = sa.part[prev] + (size_t)slot * sa.nslice * kPartStride;
         double pre[kPrefetch];
-        prefetch_slices<BS>(sp, sa.nslice, pre);
+        prefetch_slices<BS>(sp, nsl, pre);
         state_load<BS>(S, sa.state[prev] + (size_t)slot * kStateStride);
         __syncthreads();
         if (S.done) {
             if (slice == 0) state_store<BS>(S, gst);
             return;
         }
-        const int r = finish_sliced<BS, true>(a, S, sp, sa.nslice, V, stage, pre);
+        const int r = finish_sliced<BS, true>(a, S, sp, nsl, V, stage, pre);
         if (r != 0) {
             finished();
             return;
@@ -2018,7 +2029,7 @@ __global__ void __launch_bounds__(BS) slice_kernel(SearchArgs a, SliceArgs sa) {
     }
     // chunks of 64 samples dealt to the slices in turn: every slice gets parts of the
     // whole cloud, so slices cost the same where rows of the cloud are short (its caps)
-    eval_samples<BPV, BS, 1>(a, S, nd, need_j, rsrc, stage, red, V, slice, sa.nslice);
+    eval_samples<BPV, BS, 1>(a, S, nd, need_j, rsrc, stage, red, V, slice, nsl);
     if (tid < 64) {
         double *gp = sa.part[cur] + ((size_t)slot * sa.nslice + slice) * kPartStride;
         const int ne = need_j ? kPartStride : 16;
@@ -2026,6 +2037,29 @@ __global__ void __launch_bounds__(BS) slice_kernel(SearchArgs a, SliceArgs sa) {
             gp[e] = e < 3 ? S.sums[e] : (e < 9 ? S.bbr[e - 3] : (e < 16 ? 0.0 : V[e - 16]));
     }
     if (slice == 0) state_store<BS>(S, gst);
+}
+
+// UNEVEN: every point its own geometry (SearchArgs::pgeo) and its own slice count; the
+// workgroups past a point's slices leave at once.
+template <int BPV, int BS, bool UNEVEN>
+__global__ void __launch_bounds__(BS) slice_kernel(SearchArgs a, SliceArgs sa) {
+    DVC_DYN_LDS(float, dyn);
+    DVC_SHARED(PtShared, S);
+    DVC_SHARED(double[(BS / 64) * 9], red);
+    DVC_SHARED(double[256], V);
+    const int slot = (int)blockIdx.x % sa.cnt8, slice = (int)blockIdx.x / sa.cnt8;
+    if (slot >= sa.cnt) return;
+    if constexpr (UNEVEN) {
+        const int n = sa.level_pts[slot];
+        const int nsl = (a.pgeo[n].nsamp + sa.per - 1) / sa.per;
+        if (slice >= nsl) return;
+        DVC_SHARED(SearchArgs, la);
+        if (threadIdx.x == 0) apply_geo(la, a, n);
+        __syncthreads();
+        slice_body<BPV, BS>(la, sa, slot, slice, nsl, dyn, S, red, V);
+    } else {
+        slice_body<BPV, BS>(a, sa, slot, slice, sa.nslice, dyn, S, red, V);
+    }
 }
 
 #ifndef DVC_EMU
@@ -2102,14 +2136,16 @@ template <int BPV, int BS>
 int SliceInst<BPV, BS>::launch(const SearchArgs &a, const SliceArgs &sa, hipStream_t st) {
     const size_t lds = (size_t)(BS / 64) * 64 * 17 * sizeof(float);
     const unsigned grid = (unsigned)sa.cnt8 * (unsigned)sa.nslice;
-    hipLaunchKernelGGL((slice_kernel<BPV, BS>), dim3(grid), dim3(BS), lds, st, a, sa);
+    if (a.pgeo) hipLaunchKernelGGL((slice_kernel<BPV, BS, true>), dim3(grid), dim3(BS), lds, st, a, sa);
+    else hipLaunchKernelGGL((slice_kernel<BPV, BS, false>), dim3(grid), dim3(BS), lds, st, a, sa);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 template <int BPV, int BS>
 int SliceInst<BPV, BS>::resume(const SearchArgs &a, const SliceArgs &rs, hipStream_t st) {
     const size_t lds = (size_t)(BS / 64) * 64 * 17 * sizeof(float);
-    hipLaunchKernelGGL((resume_kernel<BPV, BS>), dim3(rs.cnt), dim3(BS), lds, st, a, rs.level_pts, rs);
+    if (a.pgeo) hipLaunchKernelGGL((resume_kernel<BPV, BS, true>), dim3(rs.cnt), dim3(BS), lds, st, a, rs.level_pts, rs);
+    else hipLaunchKernelGGL((resume_kernel<BPV, BS, false>), dim3(rs.cnt), dim3(BS), lds, st, a, rs.level_pts, rs);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

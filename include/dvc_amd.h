@@ -263,10 +263,13 @@ int dvc_set_block_threads(dvc_ctx *ctx, int threads);
  * each threads-per-point count whose LDS fits (64, 128, 256, 512, 1024; one point per
  * workgroup) and keeps the fastest for later executes (until the next prepare); then
  * times 128, 256 and 512 threads per point for the launch that computes every point's
- * reference samples (256 by default) and keeps the fastest of those too.  The
- * given starts must be set first in the given-start and judged modes.  Results change
- * only in the summation order of the per-point sums.  best_threads / best_ms (device
- * milliseconds per execute) may be NULL. */
+ * reference samples (256 by default) and keeps the fastest of those too; then, for the
+ * serial chain, slice launches off and eight shapes of them (dvc_set_slice_launches:
+ * widths of 2-8 workgroups per CU, 3-10 slice launches, 512 or 1024 samples per
+ * workgroup), keeping the fastest.  The given starts must be set first in the
+ * given-start and judged modes.  Results change only in the summation order of the
+ * per-point sums.  best_threads / best_ms (device milliseconds per execute) may be
+ * NULL. */
 int dvc_search_autotune(dvc_ctx *ctx, int reps, int32_t *best_threads, float *best_ms);
 /* 1 (default): the level launches of a prepared search are captured into a HIP graph
  * once and replayed by dvc_search_execute; 0: launched one by one. */
@@ -284,14 +287,17 @@ int dvc_set_round_capacity(dvc_ctx *ctx, int32_t points);
 /* Slice launches of the serial chain (no reference counterpart).  A chain launch of few
  * points leaves most of the chip idle with one workgroup per point.  Such a launch can
  * instead run as slice launches: one launch per Gauss-Newton evaluation, each point's
- * samples split over workgroups of samples_per_workgroup samples (0: 256), the last
- * workgroup of a point to finish adding the slices up in slice order and taking the step
- * of Search.cpp:194-246.  After `evaluations` such launches (0: 4) the one-workgroup
- * search resumes the points still searching.  A launch is sliced when its points times
- * their slices are at most max_workgroups (0: 4 x the chip's CUs; -1: never).  Applies to
- * the serial chain with one geometry for all points, no translation grid and the HBM
- * table of reference samples; results differ from the one-workgroup search only in the
- * summation order of the per-point sums.  Takes effect at the next execute. */
+ * samples split over workgroups of samples_per_workgroup samples (0: 512; 256 or a
+ * multiple of 512 up to 8192), every workgroup of a point adding the previous launch's
+ * slice sums up in slice order and taking the step of Search.cpp:194-246.  After
+ * `evaluations` such launches (0: 4) the one-workgroup search resumes the points still
+ * searching.  A launch is sliced when its points times their slices are at most
+ * max_workgroups (0: 4 x the chip's CUs; -1: never); with per-point geometry
+ * (dvc_set_point_geometry) a launch's slices are those of its point with the most
+ * samples, and each point uses as many as its own samples need.  Applies to the serial
+ * chain without a translation grid and with the HBM table of reference samples; results
+ * differ from the one-workgroup search only in the summation order of the per-point sums.
+ * Takes effect at the next execute. */
 int dvc_set_slice_launches(dvc_ctx *ctx, int32_t max_workgroups, int32_t evaluations, int32_t samples_per_workgroup);
 /* Slice launches of the prepared search: chain launches run that way, workgroups per
  * point and slice launches per chain launch (all 0 when none).  Any pointer may be NULL. */

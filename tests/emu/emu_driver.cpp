@@ -23,8 +23,8 @@ static void run_slice(const SearchArgs &a, const SliceArgs &sa, int threads) {
     const unsigned step = threads > 256 ? 2 : 4;
     for (unsigned b0 = 0; b0 < blocks; b0 += step) {
         const unsigned nb = blocks - b0 < step ? blocks - b0 : step;
-        if (threads == 256) emu::run_grid(b0, nb, 256, slds, [&] { slice_kernel<BPV, 256>(a, sa); });
-        else emu::run_grid(b0, nb, 512, slds, [&] { slice_kernel<BPV, 512>(a, sa); });
+        if (threads == 256) emu::run_grid(b0, nb, 256, slds, [&] { slice_kernel<BPV, 256, false>(a, sa); });
+        else emu::run_grid(b0, nb, 512, slds, [&] { slice_kernel<BPV, 512, false>(a, sa); });
     }
 }
 
@@ -61,7 +61,7 @@ static void run_search(const SearchArgs &a, size_t lds, const std::vector<int> &
         for (int b0 = 0; b0 < cnt; b0 += 8) {
             int nb = cnt - b0 < 8 ? cnt - b0 : 8;
             if (rs.cnt > 0)
-                emu::run_grid(b0, nb, 512, (size_t)8 * 64 * 17 * 4, [&] { resume_kernel<BPV, 512>(a, pts, rs); });
+                emu::run_grid(b0, nb, 512, (size_t)8 * 64 * 17 * 4, [&] { resume_kernel<BPV, 512, false>(a, pts, rs); });
             else
                 emu::run_grid(b0, nb, B, lds, [&] { search_kernel<BPV, B, false>(a, pts); });
         }
@@ -130,7 +130,8 @@ extern "C" int emu_search_sliced(const dvc_run_params *rc, const void *ref, cons
         sl.per = slice_per;
         sl.evals = slice_evals;
         sl.nslice = (a.nsamp + slice_per - 1) / slice_per;
-        const int cnt8 = slice_plan(lo, sl.nslice, slice_wgs > 0 ? slice_wgs : 1 << 30, 8, sl.sliced);
+        const std::vector<int> nsl(lo.size() > 0 ? lo.size() - 1 : 0, sl.nslice);
+        const int cnt8 = slice_plan(lo, nsl, slice_wgs > 0 ? slice_wgs : 1 << 30, 8, sl.sliced);
         if (cnt8 > 0) {
             sl.cnt8 = cnt8;
             sl.state.assign((size_t)2 * cnt8 * kStateStride, 0xA5);

@@ -80,3 +80,40 @@ def test_sliced_dofs_objectives_range_fail(ctx, oracle, ndof, obj):
     assert info[0] > 0
     _check(g, o, w)
     assert (o["status"] == -1).any() and (o["status"] == 0).any()
+
+
+@pytest.mark.parametrize("ndof,slices", [(6, (0, 3, 256)), (12, (0, 4, 512))])
+def test_sliced_per_point_geometry(ctx, oracle, ndof, slices):
+    """Per-point geometry (dvc_set_point_geometry) in slice launches: each point is split
+    into as many workgroups as its own sample count needs (1-6 here), a chain launch into
+    as many as its largest point's.  Against the oracle's loop with one RunControl per point
+    (orc_dvc_run_geo) and against the same search without slice launches."""
+    case = synth.translation_u8(size=80, n=4, spacing=8)
+    rng = np.random.default_rng(23)
+    n = len(case.points)
+    geom = rng.integers(0, 2, n)
+    size = rng.integers(13, 22, n)
+    npp = rng.integers(200, 1500, n)
+    p = dict(case.params, ndof=ndof, obj="znssd")
+    order, neigh = D.sort_order_neighbors(case.points)
+    ctx.upload(0, case.ref)
+    ctx.upload(1, case.tar)
+    ctx.set_point_geometry(geom, size, npp)
+    try:
+        g, info = _run(ctx, case, p, order, neigh, slices)
+        w, winfo = _run(ctx, case, p, order, neigh, (-1, 0, 0))
+    finally:
+        ctx.set_point_geometry()
+    run = oracle.make_run(shape=case.ref.shape, bytes_per=1, **p)
+    o, _, _ = oracle.dvc_run_geo(run, case.ref, case.tar, case.points, order, neigh, geom, size, npp, nthreads=8)
+    print(f"per-point geometry, {ndof} DOF, slices {slices}: {info[0]} sliced launches, up to {info[1]} "
+          f"workgroups per point")
+    assert info[0] > 0 and winfo[0] == 0
+    assert np.array_equal(g["status"], o["status"])
+    assert np.array_equal(g["iters"], o["iters"])
+    good = o["status"] == 0
+    conv = good & (o["iters"] < 20)
+    assert np.abs(g["par"][good, :3] - o["par"][good, :3]).max() <= 1e-3
+    assert (np.abs(g["obj"][conv] - o["obj"][conv]) / np.abs(o["obj"][conv])).max() <= 1e-5
+    assert np.array_equal(g["status"], w["status"])
+    assert np.abs(g["par"][good] - w["par"][good]).max() <= 1e-6
