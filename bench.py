@@ -398,6 +398,8 @@ def full_scale(a, D, synth, workload):
     t_knn = time.time() - t1
     log(f"order and 75 neighbours of {len(case.points)} points in {t_knn:.1f} s")
     params = D.make_params(**case.params)
+    if a.threads:
+        ctx.set_block_threads(a.threads)
     t1 = time.time()
     ctx.prepare(params, case.points, order, neigh)
     t_prep = time.time() - t1

@@ -750,7 +750,10 @@ int dvc_search_prepare(dvc_ctx *ctx, const dvc_run_params *rc, const double *poi
     // neighbours are done
     auto chain_launches = [&](std::vector<int> &lp, std::vector<int> &lo) {
         dependency_levels(order, pos, neigh, npts, k, N, lp, lo);
-        const int cap = ctx->round_cap > 0 ? ctx->round_cap : ctx->cus * std::max(1, 512 / bt);
+        int bpc = 0;          // workgroups of the search kernel one CU holds
+        if (search_max_blocks(bpv, bt, search_lds_bytes(nsamp_pad, bt, !table), &bpc) != 0 || bpc < 1)
+            bpc = std::max(1, 512 / bt);
+        const int cap = ctx->round_cap > 0 ? ctx->round_cap : ctx->cus * bpc;
         int widest = 0;
         for (size_t L = 0; L + 1 < lo.size(); L++) widest = std::max(widest, lo[L + 1] - lo[L]);
         if (ctx->round_cap >= 0 && widest > cap) capacity_schedule(order, pos, neigh, k, N, cap, lp, lo);

@@ -1896,10 +1896,16 @@ __device__ __forceinline__ void state_store(const PtShared &S, unsigned char *g)
     for (int w = threadIdx.x; w < kStateBytes / 8; w += B) dst[w] = src[w];
 }
 
+// Wavefronts per SIMD the search kernel is compiled for: three for workgroups of up to
+// 256 threads (at most 168 registers, so three such workgroups share a CU and hide each
+// other's gather latency), else what the workgroup size allows.
+template <int B>
+constexpr int search_waves_per_simd() { return B <= 256 ? 3 : 1; }
+
 // One dependency level of the point loop: workgroup b searches point level_pts[b].
 // All neighbours a point depends on belong to earlier levels (earlier launches).
 template <int BPV, int B, bool UNEVEN>
-__global__ void __launch_bounds__(B) search_kernel(SearchArgs a, const int *level_pts) {
+__global__ void __launch_bounds__(B, search_waves_per_simd<B>()) search_kernel(SearchArgs a, const int *level_pts) {
     DVC_DYN_LDS(float, dyn);
     // [MFMA staging tiles][reference samples, when not read from HBM]
     float *stage = dyn;
