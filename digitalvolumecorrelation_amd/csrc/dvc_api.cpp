@@ -1221,8 +1221,9 @@ int dvc_search_autotune(dvc_ctx *ctx, int reps, int32_t *best_threads, float *be
     // searches and large clouds of per-point geometry), the fastest kept
     if (ctx->plan_mode == DVC_START_CHAIN && ctx->slice_wgs >= 0 && ctx->d_rvals && ctx->rc.basin_radius == 0.0) {
         const int C = ctx->cus;
-        const int cands[][3] = {{-1, 0, 512},     {2 * C, 4, 512},  {4 * C, 3, 512},  {4 * C, 4, 512},  {4 * C, 6, 512},
-                                {8 * C, 4, 512},  {4 * C, 7, 1024}, {8 * C, 7, 1024}, {8 * C, 10, 1024}};
+        const int cands[][3] = {{-1, 0, 512},      {2 * C, 4, 512},  {4 * C, 3, 512},  {4 * C, 4, 512},
+                                {4 * C, 6, 512},   {8 * C, 4, 512},  {4 * C, 7, 1024}, {8 * C, 7, 1024},
+                                {8 * C, 14, 1024}, {8 * C, 14, 2048}};
         int bw = ctx->slice_wgs, be = ctx->slice_evals, bp = ctx->slice_per;
         float bsl = -1.0f;
         for (const auto &c : cands) {

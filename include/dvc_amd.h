@@ -264,8 +264,8 @@ int dvc_set_block_threads(dvc_ctx *ctx, int threads);
  * workgroup) and keeps the fastest for later executes (until the next prepare); then
  * times 128, 256 and 512 threads per point for the launch that computes every point's
  * reference samples (256 by default) and keeps the fastest of those too; then, for the
- * serial chain, slice launches off and eight shapes of them (dvc_set_slice_launches:
- * widths of 2-8 workgroups per CU, 3-10 slice launches, 512 or 1024 samples per
+ * serial chain, slice launches off and nine shapes of them (dvc_set_slice_launches:
+ * widths of 2-8 workgroups per CU, 3-14 slice launches, 512, 1024 or 2048 samples per
  * workgroup), keeping the fastest.  The given starts must be set first in the
  * given-start and judged modes.  Results change only in the summation order of the
  * per-point sums.  best_threads / best_ms (device milliseconds per execute) may be
