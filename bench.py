@@ -11,8 +11,9 @@ One step also fits the strain field of the result (the 26 grid neighbours and th
 point, strain.cpp:402-591) on the device.
 
 Prints one JSON line:  metric/value (points/s), the roofline of the step (algorithmic
-bytes against HBM; algorithmic FLOP of the reference's formulation against FP32
-matrix peak; device time from HIP events on the library's stream), the learned-proposal
+FLOP of the reference's formulation against FP32 matrix peak, the bound of the path;
+algorithmic bytes against HBM beside it; device time from HIP events on the library's
+stream), the learned-proposal
 leg with its agreement counts against the plain chain, and the oracle's CPU throughput
 on the same processing order (same box, stated thread count) with the agreement of the
 GPU run against it point by point.
@@ -564,23 +565,29 @@ def main():
         prof = json.loads(tf.read_text())
         traffic = prof.get("hbm_bytes_per_step")
         valu = prof.get("valu_busy_on_active_cus")
-    roof = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+    own_evals = 1.0 + float(res["iters"][good].mean()) if good.any() else None
+    # the bound of the path in the reference's formulation is FP32 compute (about 400
+    # FLOP per compulsory byte, far above the ~20 FLOP/B ridge of the chip): the line's
+    # roofline is that one, with the HBM figures beside it
+    roof = {"bound": "mfma", "achieved": round(tflops, 3), "peak": FP32_MATRIX_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": tflops / FP32_MATRIX_PEAK_TFLOPS, "traffic": traffic,
             "traffic_per": "step (HBM bytes of one step, FETCH_SIZE + WRITE_SIZE scaled by a 1 GiB copy, from "
                            "profiles/slice_kernel_affine_u16_256_traffic.json)",
             "kernels": "slice_kernel<2,512> (narrow dependency levels), resume_kernel, search_kernel<2,B>",
             "launches_per_step": sliced[0] * (sliced[2] + 1) + (chain_launches - sliced[0]),
-            "algorithmic_bytes_per_point": round(per_pt, 1), "bytes_points_sampled": nsamp,
+            "flop_alg_per_point": round(flop_pt),
+            "definition": "2*4096 per distinct cell of the reference and converged target clouds + "
+                          "2*64 per sample per evaluation, evaluations = 1 + iterations*(1+ndof) + 1 "
+                          "(forward-difference Jacobian), as the reference computes them",
+            "evaluations_per_point_reference": round(evals_pt, 2),
+            "evaluations_per_point_own": round(own_evals, 2) if own_evals else None,
+            "valu_busy_on_active_cus": valu,
             "latency_note": "the serial chain's dependency levels (start guesses from earlier neighbours) bound "
                             "the step: each level waits for the one before; HBM and matrix units are far "
                             "from saturated by construction",
-            "compute": {"flop_alg_per_point": round(flop_pt), "evaluations_per_point_reference": round(evals_pt, 2),
-                        "achieved": round(tflops, 3), "peak": FP32_MATRIX_PEAK_TFLOPS, "unit": "TFLOP/s",
-                        "frac": tflops / FP32_MATRIX_PEAK_TFLOPS,
-                        "definition": "2*4096 per distinct cell of the reference and converged target clouds + "
-                                      "2*64 per sample per evaluation, evaluations = 1 + iterations*(1+ndof) + 1 "
-                                      "(forward-difference Jacobian), as the reference computes them",
-                        "valu_busy_on_active_cus": valu}}
+            "hbm": {"achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": achieved / HBM_PEAK_GBS, "algorithmic_bytes_per_point": round(per_pt, 1),
+                    "bytes_points_sampled": nsamp}}
 
     nn = None
     if not a.no_proposal:
