@@ -708,14 +708,21 @@ __device__ __forceinline__ void set_warp(PtShared &S, int ndof, const double tri
         S.Sm[3] = v[9];  S.Sm[4] = v[7];  S.Sm[5] = v[10];
         S.Sm[6] = v[11]; S.Sm[7] = v[10]; S.Sm[8] = v[8];
     }
-    // d x'/d angle = (I+S) dA rel
-    for (int r = 0; r < 3; r++)
-        for (int i = 0; i < 3; i++)
-            for (int j = 0; j < 3; j++) {
-                double s = dA[r][i * 3 + j];
-                for (int l = 0; l < 3; l++) s += S.Sm[i * 3 + l] * dA[r][l * 3 + j];
-                S.M[r][i * 3 + j] = (float)s;
-            }
+    // d x'/d angle = (I+S) dA rel, for the FP32 Jacobian rows (formed in FP32)
+    if (ndof > 6) {
+        float sm[9];
+        for (int q = 0; q < 9; q++) sm[q] = (float)S.Sm[q];
+        for (int r = 0; r < 3; r++)
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) {
+                    float s = (float)dA[r][i * 3 + j];
+                    for (int l = 0; l < 3; l++) s = __builtin_fmaf(sm[i * 3 + l], (float)dA[r][l * 3 + j], s);
+                    S.M[r][i * 3 + j] = s;
+                }
+    } else {
+        for (int r = 0; r < 3; r++)
+            for (int q = 0; q < 9; q++) S.M[r][q] = (float)dA[r][q];
+    }
     for (int q = 0; q < 9; q++) S.Af[q] = (float)S.A[q];
 }
 
@@ -1165,25 +1172,26 @@ __device__ __forceinline__ void normal_row(const SearchArgs &a, const PtShared &
 // returns x_i = b_i / A_ii.  The pivots A_kk are those of LDL^T (the diagonals of the
 // successive Schur complements), tested as there: returns false, having computed nothing
 // usable, when a pivot is not above 1e-9 of the largest diagonal entry (or is not
-// finite), and the caller solves by column-pivoted QR.  Lanes at or above N compute
-// nothing used.  Compared with an LDL^T computed whole on every lane, a step needs
-// ~N + 1 dependent multiply-adds instead of the whole trailing update, and a lane holds
-// N + 1 FP64 values instead of the whole system.
+// finite), and the caller solves by column-pivoted QR.  The test is taken once, after
+// the last step: a failed pivot only spoils values that are then discarded.  Lanes at
+// or above N compute nothing used.  Compared with an LDL^T computed whole on every lane,
+// a step needs ~N + 1 dependent fused multiply-adds instead of the whole trailing
+// update, and a lane holds N + 1 FP64 values instead of the whole system.
 template <int N>
 __device__ __forceinline__ bool gj_rows(double (&r)[N + 1], int lane, double &x) {
-#pragma clang fp contract(off)
     double dmax = 0.0;
 #pragma unroll
     for (int k = 0; k < N; k++) dmax = fmax(dmax, rdlane(r[k], k));
     const double tol = 1e-9 * dmax;
     double rinv_own = 0.0;
+    bool ok = true;
 #pragma unroll
     for (int k = 0; k < N; k++) {
         double pr[N + 1];
 #pragma unroll
         for (int j = k; j <= N; j++) pr[j] = rdlane(r[j], k);
         const double d = pr[k];
-        if (!(d > tol) || !(d < 1.0e300)) return false;
+        ok = ok && (d > tol) && (d < 1.0e300);
         const double rinv = fast_rcp(d);
         if (lane == k) {
             rinv_own = rinv;
@@ -1191,11 +1199,11 @@ __device__ __forceinline__ bool gj_rows(double (&r)[N + 1], int lane, double &x)
             const double f = r[k] * rinv;
             r[k] = 0.0;
 #pragma unroll
-            for (int j = k + 1; j <= N; j++) r[j] -= f * pr[j];
+            for (int j = k + 1; j <= N; j++) r[j] = __builtin_fma(-f, pr[j], r[j]);
         }
     }
     x = r[N] * rinv_own;
-    return true;
+    return ok;
 }
 
 // Column `lane` (lane < N) of the objective's Gauss-Newton normal matrix J^T J, or for
