@@ -2674,7 +2674,11 @@ __device__ void fullpiv_wave10(const double *M, double (*par)[10]) {
 }
 
 // Eigenvalues of a symmetric 3x3 (cyclic Jacobi), sorted descending
-// (strain.cpp:553-556 takes EigenSolver real parts and sorts them).
+// (strain.cpp:553-556 takes EigenSolver real parts and sorts them).  From the fifth
+// sweep on, an off-diagonal entry that no longer changes either diagonal entry it
+// couples (|a_pp| + 100 |a_pq| == |a_pp|, and the same for a_qq) is set to zero instead
+// of rotated, the classical cyclic-Jacobi stopping rule: rotations cannot drive the
+// entries below the rounding of the diagonal, so without it every call ran all 50 sweeps.
 __device__ void sym3_eigen(const double *S, double *lam) {
 #pragma clang fp contract(off)
     double a[3][3];
@@ -2686,6 +2690,12 @@ __device__ void sym3_eigen(const double *S, double *lam) {
         for (int p = 0; p < 2; p++)
             for (int q = p + 1; q < 3; q++) {
                 if (a[p][q] == 0.0) continue;
+                const double g = 100.0 * fabs(a[p][q]);
+                if (sweep > 3 && fabs(a[p][p]) + g == fabs(a[p][p]) && fabs(a[q][q]) + g == fabs(a[q][q])) {
+                    a[p][q] = 0.0;
+                    a[q][p] = 0.0;
+                    continue;
+                }
                 double theta = (a[q][q] - a[p][p]) / (2.0 * a[p][q]);
                 double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
                 double c = 1.0 / sqrt(t * t + 1.0), s = t * c;

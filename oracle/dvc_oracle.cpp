@@ -731,7 +731,11 @@ static void fullpiv_solve(const double *Arow, const double *b, int n, double *x)
 }
 
 /* Eigenvalues of a symmetric 3x3 by cyclic Jacobi rotations, sorted descending.
- * (strain.cpp:553-556 takes EigenSolver real parts of a symmetric matrix and sorts.) */
+ * (strain.cpp:553-556 takes EigenSolver real parts of a symmetric matrix and sorts.)
+ * From the fifth sweep on, an off-diagonal entry too small to change either diagonal
+ * entry it couples (|a_pp| + 100 |a_pq| == |a_pp|, same for a_qq) is set to zero: the
+ * classical cyclic-Jacobi stopping rule, since rotations alone stall at the rounding
+ * level of the diagonal. */
 static void sym3_eigen(const double *S, double *lam) {
     double a[3][3];
     for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) a[i][j] = 0.5 * (S[i * 3 + j] + S[j * 3 + i]);
@@ -741,6 +745,12 @@ static void sym3_eigen(const double *S, double *lam) {
         for (int p = 0; p < 2; p++)
             for (int q = p + 1; q < 3; q++) {
                 if (a[p][q] == 0.0) continue;
+                const double g = 100.0 * fabs(a[p][q]);
+                if (sweep > 3 && fabs(a[p][p]) + g == fabs(a[p][p]) && fabs(a[q][q]) + g == fabs(a[q][q])) {
+                    a[p][q] = 0.0;
+                    a[q][p] = 0.0;
+                    continue;
+                }
                 double theta = (a[q][q] - a[p][p]) / (2.0 * a[p][q]);
                 double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + std::sqrt(theta * theta + 1.0));
                 double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;

@@ -177,6 +177,26 @@ def test_dense_solvers(oracle):
     assert np.allclose(lam, np.sort(np.linalg.eigvalsh(S[:3, :3]))[::-1], rtol=1e-13, atol=1e-13)
 
 
+def test_sym3_eigen(oracle):
+    """The strain fit's symmetric 3x3 eigenvalues (strain.cpp:553-556): against LAPACK's
+    on random symmetric matrices over eight decades of scale, indefinite ones included,
+    to 1e-14 of the largest |eigenvalue|; exact on diagonal matrices; repeated
+    eigenvalues of a rotated diagonal; the zero matrix."""
+    rng = np.random.default_rng(11)
+    for _ in range(300):
+        M = rng.normal(size=(3, 3)) * 10.0 ** rng.uniform(-6, 2)
+        S = M + M.T
+        want = np.sort(np.linalg.eigvalsh(S))[::-1]
+        got = oracle.sym3_eigen(S)
+        assert np.abs(got - want).max() <= 1e-14 * np.abs(want).max(), (S, got, want)
+    D = np.diag([3.0, -1.0, 2.0])
+    assert np.array_equal(oracle.sym3_eigen(D), [3.0, 2.0, -1.0])
+    Q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+    S = Q @ np.diag([2.0, 2.0, -0.5]) @ Q.T
+    assert np.allclose(oracle.sym3_eigen(S), [2.0, 2.0, -0.5], rtol=0, atol=1e-14)
+    assert np.array_equal(oracle.sym3_eigen(np.zeros((3, 3))), np.zeros(3))
+
+
 # ------------------------------------------------------------------ search
 
 @pytest.fixture(scope="module")
