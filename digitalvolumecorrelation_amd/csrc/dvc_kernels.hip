@@ -2678,7 +2678,7 @@ __device__ void fullpiv_wave10(const double *M, double (*par)[10]) {
 // sweep on, an off-diagonal entry that no longer changes either diagonal entry it
 // couples (|a_pp| + 100 |a_pq| == |a_pp|, and the same for a_qq) is set to zero instead
 // of rotated, the classical cyclic-Jacobi stopping rule: rotations cannot drive the
-// entries below the rounding of the diagonal, so without it every call ran all 50 sweeps.
+// entries below the rounding of the diagonal, so without it every call would run all 50 sweeps.
 __device__ void sym3_eigen(const double *S, double *lam) {
 #pragma clang fp contract(off)
     double a[3][3];
