@@ -888,6 +888,17 @@ static int plan_slices(dvc_ctx *ctx) {
                 ns = std::max(ns, ctx->pt_nsamp[ctx->level_pts_h[q]]);
         }
         ctx->slice_nsl[L] = (ns + ctx->slice_per - 1) / ctx->slice_per;
+        // a point's slices share one XCD: with more of them than the XCD has CUs, some
+        // CUs would run two slices one after the other, so wide launches take fewer,
+        // larger slices (the kernels deal a point's 64-sample chunks to however many
+        // slices it has; per-point geometry keeps one slice size, the kernels derive
+        // each point's count from it)
+        if (ctx->pt_nsamp.empty()) {
+            const int cnt = ctx->level_off[L + 1] - ctx->level_off[L];
+            const int per_xcd = (cnt + kXcds - 1) / kXcds;
+            const int cap = std::max(1, ctx->cus / kXcds) / std::max(1, per_xcd);
+            if (cap >= 2 && ctx->slice_nsl[L] > cap) ctx->slice_nsl[L] = cap;
+        }
     }
     const int cnt8 = slice_plan(ctx->level_off, ctx->slice_nsl, max_wgs, kXcds, ctx->sliced);
     if (cnt8 == 0) return DVC_OK;

@@ -292,15 +292,18 @@ int dvc_set_round_capacity(dvc_ctx *ctx, int32_t points);
  * slice sums up in slice order and taking the step of Search.cpp:194-246.  After
  * `evaluations` such launches (0: 4) the one-workgroup search resumes the points still
  * searching.  A launch is sliced when its points times their slices are at most
- * max_workgroups (0: 4 x the chip's CUs; -1: never); with per-point geometry
- * (dvc_set_point_geometry) a launch's slices are those of its point with the most
- * samples, and each point uses as many as its own samples need.  Applies to the serial
+ * max_workgroups (0: 4 x the chip's CUs; -1: never).  A point's slices run on one XCD:
+ * where a launch's points would need more of them than an XCD has CUs, each point takes
+ * fewer, larger slices (at least 2).  With per-point geometry (dvc_set_point_geometry)
+ * a launch's slices are those of its point with the most samples, and each point uses
+ * as many as its own samples need.  Applies to the serial
  * chain without a translation grid and with the HBM table of reference samples; results
  * differ from the one-workgroup search only in the summation order of the per-point sums.
  * Takes effect at the next execute. */
 int dvc_set_slice_launches(dvc_ctx *ctx, int32_t max_workgroups, int32_t evaluations, int32_t samples_per_workgroup);
 /* Slice launches of the prepared search: chain launches run that way, workgroups per
- * point and slice launches per chain launch (all 0 when none).  Any pointer may be NULL. */
+ * point (the most any launch uses) and slice launches per chain launch (all 0 when
+ * none).  Any pointer may be NULL. */
 int dvc_search_slice_info(dvc_ctx *ctx, int32_t *sliced_launches, int32_t *slices_per_point, int32_t *evaluations);
 /* Shape of the prepared search: chain launches (dependency levels, or launches of the
  * round capacity), threads per point, points searched.  Any pointer may be NULL. */
