@@ -117,3 +117,27 @@ def test_sliced_per_point_geometry(ctx, oracle, ndof, slices):
     assert (np.abs(g["obj"][conv] - o["obj"][conv]) / np.abs(o["obj"][conv])).max() <= 1e-5
     assert np.array_equal(g["status"], w["status"])
     assert np.abs(g["par"][good] - w["par"][good]).max() <= 1e-6
+
+
+def test_wide_launches_take_fewer_slices(ctx):
+    """The 15^3 affine workload (256^3 u16, sphere 41 / 5000 samples, 12-DOF): its widest
+    chain launches hold 25-30 points, 4 per XCD, so at 10 slices per point an XCD would
+    need 40 workgroups for its 32 CUs; those launches run 8 slices per point instead
+    (the kernels deal each point's 64-sample chunks over however many slices it has).
+    Sliced results against the one-workgroup search on all 3375 points: statuses equal,
+    |du| <= 1e-3 voxel everywhere, and |dpar| <= 1e-6 wherever the iteration counts agree
+    (they may differ where a stopping test sits within the FP32 summation-order
+    difference; at most 2 points)."""
+    case = synth.affine_u16()
+    order, neigh = D.sort_order_neighbors(case.points, case.start)
+    ctx.upload(0, case.ref)
+    ctx.upload(1, case.tar)
+    g, info = _run(ctx, case, case.params, order, neigh, (0, 4, 512))
+    w, winfo = _run(ctx, case, case.params, order, neigh, (-1, 0, 0))
+    print(f"wide launches: {info[0]} sliced launches, up to {info[1]} workgroups per point")
+    assert info[0] == 256 and info[1] == 10 and winfo[0] == 0
+    assert np.array_equal(g["status"], w["status"]) and (g["status"] == 0).all()
+    assert np.abs(g["par"][:, :3] - w["par"][:, :3]).max() <= 1e-3
+    same = g["iters"] == w["iters"]
+    assert (~same).sum() <= 2
+    assert np.abs(g["par"][same] - w["par"][same]).max() <= 1e-6
