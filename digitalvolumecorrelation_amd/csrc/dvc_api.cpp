@@ -111,6 +111,7 @@ struct dvc_ctx {
     int slice_wgs = 0, slice_evals = 0, slice_per = 512;
     std::vector<char> sliced;                  // chain launches run as slice launches
     std::vector<int> slice_nsl;                // workgroups per point of each chain launch (when sliced)
+    std::vector<int> slice_cap;                // per-point geometry: cap on a point's workgroups (0: none)
     int slice_cnt8 = 0, slice_n = 0;           // buffer shape of the current plan
     std::vector<int> level_pts_h;              // points of the chain launches (host copy of d_levels)
     std::vector<int> pt_nsamp;                 // samples of each point (per-point geometry only)
@@ -177,6 +178,7 @@ void free_plan(dvc_ctx *c) {
     c->slice_cnt8 = c->slice_n = 0;
     c->sliced.clear();
     c->slice_nsl.clear();
+    c->slice_cap.clear();
     hipFree(c->d_start); hipFree(c->d_res_prop); hipFree(c->d_res_judge); hipFree(c->d_kept);
     hipFree(c->d_repair); hipFree(c->d_rep_levels);
     hipFree(c->d_sdisp); hipFree(c->d_sobj); hipFree(c->d_sout); hipFree(c->d_sstat); hipFree(c->d_snpts);
@@ -876,6 +878,7 @@ static int plan_slices(dvc_ctx *ctx) {
     const int nl = ctx->level_off.empty() ? 0 : (int)ctx->level_off.size() - 1;
     ctx->sliced.assign(nl, 0);
     ctx->slice_nsl.assign(nl, 0);
+    ctx->slice_cap.assign(nl, 0);
     const bool eligible = ctx->plan_mode == DVC_START_CHAIN && ctx->d_rvals && ctx->rc.basin_radius == 0.0 &&
                           ctx->slice_wgs >= 0;
     if (!eligible) return DVC_OK;
@@ -888,16 +891,38 @@ static int plan_slices(dvc_ctx *ctx) {
                 ns = std::max(ns, ctx->pt_nsamp[ctx->level_pts_h[q]]);
         }
         ctx->slice_nsl[L] = (ns + ctx->slice_per - 1) / ctx->slice_per;
-        // a point's slices share one XCD: with more of them than the XCD has CUs, some
-        // CUs would run two slices one after the other, so wide launches take fewer,
-        // larger slices (the kernels deal a point's 64-sample chunks to however many
-        // slices it has; per-point geometry keeps one slice size, the kernels derive
-        // each point's count from it)
+        // a point's slices share one XCD (the point's slot within the launch, mod 8):
+        // with more of them than the XCD has CUs, some CUs would run two slices one after
+        // the other, so wide launches take fewer, larger slices (the kernels deal a
+        // point's 64-sample chunks to however many slices it has)
+        const int cnt = ctx->level_off[L + 1] - ctx->level_off[L];
+        const int cus_xcd = std::max(1, ctx->cus / kXcds);
         if (ctx->pt_nsamp.empty()) {
-            const int cnt = ctx->level_off[L + 1] - ctx->level_off[L];
             const int per_xcd = (cnt + kXcds - 1) / kXcds;
-            const int cap = std::max(1, ctx->cus / kXcds) / std::max(1, per_xcd);
+            const int cap = cus_xcd / std::max(1, per_xcd);
             if (cap >= 2 && ctx->slice_nsl[L] > cap) ctx->slice_nsl[L] = cap;
+        } else {
+            // per-point geometry: the largest cap c >= 2 on every point's slices that
+            // leaves no XCD more slices than CUs; none when even 2 does not fit
+            auto fits = [&](int c) {
+                int load[kXcds] = {0};
+                for (int q = 0; q < cnt; q++) {
+                    const int sl = (ctx->pt_nsamp[ctx->level_pts_h[ctx->level_off[L] + q]] + ctx->slice_per - 1) /
+                                   ctx->slice_per;
+                    load[q % kXcds] += std::min(sl, c);
+                }
+                for (int x = 0; x < kXcds; x++)
+                    if (load[x] > cus_xcd) return false;
+                return true;
+            };
+            if (!fits(ctx->slice_nsl[L])) {
+                int c = ctx->slice_nsl[L] - 1;
+                while (c >= 2 && !fits(c)) c--;
+                if (c >= 2) {
+                    ctx->slice_cap[L] = c;
+                    ctx->slice_nsl[L] = c;
+                }
+            }
         }
     }
     const int cnt8 = slice_plan(ctx->level_off, ctx->slice_nsl, max_wgs, kXcds, ctx->sliced);
@@ -986,6 +1011,7 @@ static int enqueue_search_only(dvc_ctx *ctx, hipStream_t st) {
                 sa.level_pts = lp;
                 sa.cnt = cnt;
                 sa.nslice = ctx->slice_nsl[L];
+                sa.maxsl = ctx->slice_cap[L];
                 sa.cnt8 = (cnt + kXcds - 1) / kXcds * kXcds;
                 for (int t = 0; t < slice_evals_of(ctx); t++) {
                     sa.t = t;

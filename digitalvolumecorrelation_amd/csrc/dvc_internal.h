@@ -73,6 +73,7 @@ struct SliceArgs {
     const int *level_pts;            // points of the chain launch
     int cnt, cnt8;                   // points; cnt rounded up to a multiple of 8
     int nslice, per;                 // workgroups per point (the launch's largest); samples per workgroup
+    int maxsl;                       // per-point geometry: at most this many workgroups per point (0: no cap)
     int t;                           // launch index within the chain launch (0: set-up and start guess)
     unsigned char *state[2];         // [cnt8][kStateStride] carried point state, by launch parity
     double *part[2];                 // [cnt8][nslice][kPartStride] slice sums, by launch parity

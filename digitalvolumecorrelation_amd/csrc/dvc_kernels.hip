@@ -1937,6 +1937,13 @@ __global__ void __launch_bounds__(B, search_waves_per_simd<B>()) search_kernel(S
     if (tid == 0) write_result(a, S, n);
 }
 
+// Workgroups of a point with per-point geometry in a slice launch: one per sa.per of its
+// samples, at most sa.maxsl when that is set.
+__device__ __forceinline__ int point_slices(int nsamp, const SliceArgs &sa) {
+    const int n = (nsamp + sa.per - 1) / sa.per;
+    return sa.maxsl > 0 && n > sa.maxsl ? sa.maxsl : n;
+}
+
 // The points of a chain launch after its slice launches (rs.t of them): workgroup b
 // continues point level_pts[b] from the state and slice sums that slice launch rs.t - 1
 // left.  Finished points are skipped; the others end their sliced evaluation and run
@@ -1973,7 +1980,7 @@ __global__ void __launch_bounds__(B) resume_kernel(SearchArgs a, const int *leve
         const int n = level_pts[blockIdx.x];
         if (threadIdx.x == 0) apply_geo(la, a, n);
         __syncthreads();
-        resume_body<BPV, B>(la, level_pts, rs, (la.nsamp + rs.per - 1) / rs.per, dyn, S, red, V);
+        resume_body<BPV, B>(la, level_pts, rs, point_slices(la.nsamp, rs), dyn, S, red, V);
     } else {
         resume_body<BPV, B>(a, level_pts, rs, rs.nslice, dyn, S, red, V);
     }
@@ -2065,7 +2072,7 @@ __global__ void __launch_bounds__(BS) slice_kernel(SearchArgs a, SliceArgs sa) {
     if (slot >= sa.cnt) return;
     if constexpr (UNEVEN) {
         const int n = sa.level_pts[slot];
-        const int nsl = (a.pgeo[n].nsamp + sa.per - 1) / sa.per;
+        const int nsl = point_slices(a.pgeo[n].nsamp, sa);
         if (slice >= nsl) return;
         DVC_SHARED(SearchArgs, la);
         if (threadIdx.x == 0) apply_geo(la, a, n);
