@@ -295,8 +295,8 @@ int dvc_set_round_capacity(dvc_ctx *ctx, int32_t points);
  * max_workgroups (0: 4 x the chip's CUs; -1: never).  A point's slices run on one XCD:
  * where a launch's points would need more of them than an XCD has CUs, each point takes
  * fewer, larger slices (at least 2).  With per-point geometry (dvc_set_point_geometry)
- * a launch's slices are those of its point with the most samples, and each point uses
- * as many as its own samples need.  Applies to the serial
+ * each point uses as many slices as its own samples need, up to the launch's cap, and a
+ * launch's slice count is that of its point with the most.  Applies to the serial
  * chain without a translation grid and with the HBM table of reference samples; results
  * differ from the one-workgroup search only in the summation order of the per-point sums.
  * Takes effect at the next execute. */
