@@ -104,7 +104,8 @@ struct dvc_ctx {
     int *d_kept = nullptr;                     // 1 proposal kept, 0 judge kept, 2 repaired
     int *d_repair = nullptr;                   // 1 where a judged point is searched again
     int *d_rep_levels = nullptr;               // repair launches (the serial chain's launches)
-    std::vector<int> rep_off;
+    std::vector<int> rep_off, rep_pts_h;       // their offsets, and their points on the host
+    int *h_repair = nullptr;                   // pinned host copy of d_repair (one int per point)
     // slice launches of the chain (SliceArgs): largest workgroup count of a sliced launch
     // (0 automatic, -1 off), slice launches per chain launch before the one-workgroup
     // search resumes the points still searching (0 automatic), samples per workgroup
@@ -181,12 +182,15 @@ void free_plan(dvc_ctx *c) {
     c->slice_cap.clear();
     hipFree(c->d_start); hipFree(c->d_res_prop); hipFree(c->d_res_judge); hipFree(c->d_kept);
     hipFree(c->d_repair); hipFree(c->d_rep_levels);
+    if (c->h_repair) hipHostFree(c->h_repair);
     hipFree(c->d_sdisp); hipFree(c->d_sobj); hipFree(c->d_sout); hipFree(c->d_sstat); hipFree(c->d_snpts);
     c->d_sdisp = c->d_sobj = c->d_sout = nullptr;
     c->d_sstat = c->d_snpts = nullptr;
     c->d_start = nullptr; c->d_res_prop = c->d_res_judge = nullptr; c->d_kept = nullptr;
     c->d_repair = nullptr; c->d_rep_levels = nullptr;
+    c->h_repair = nullptr;
     c->rep_off.clear();
+    c->rep_pts_h.clear();
     c->starts_set = false;
     c->d_rvals = nullptr;
     c->d_rstat = nullptr;
@@ -800,6 +804,7 @@ int dvc_search_prepare(dvc_ctx *ctx, const dvc_run_params *rc, const double *poi
         HIPCHK(hipMemsetAsync(ctx->d_repair, 0, sizeof(int) * (size_t)npts, ctx->stream));
         HIPCHK(hipMemcpyAsync(ctx->d_rep_levels, rep_pts.data(), sizeof(int) * rep_pts.size(), hipMemcpyHostToDevice,
                               ctx->stream));
+        HIPCHK(hipHostMalloc((void **)&ctx->h_repair, sizeof(int) * (size_t)npts, hipHostMallocDefault));
     }
     std::vector<int> ref_order;
     if (table) {
@@ -843,6 +848,7 @@ int dvc_search_prepare(dvc_ctx *ctx, const dvc_run_params *rc, const double *poi
     ctx->level_off = level_off;
     ctx->level_pts_h = level_pts;
     ctx->rep_off = rep_off;
+    ctx->rep_pts_h = rep_pts;
     ctx->pt_nsamp.clear();
     if (uneven) {
         ctx->pt_nsamp.resize(npts);
@@ -961,9 +967,12 @@ static int enqueue_strain(dvc_ctx *ctx, hipStream_t st) {
     return DVC_OK;
 }
 
+// Everything a prepared search enqueues up to its results; in the judged mode up to the
+// selection (the repair launches and the strain fit follow in dvc_search_execute, once
+// the points to repair are known on the host).
 static int enqueue_search(dvc_ctx *ctx, hipStream_t st) {
     if (int e = enqueue_search_only(ctx, st)) return e;
-    if (ctx->strain_ndp > 0) return enqueue_strain(ctx, st);
+    if (ctx->strain_ndp > 0 && ctx->plan_mode != DVC_START_JUDGED) return enqueue_strain(ctx, st);
     return DVC_OK;
 }
 
@@ -971,12 +980,11 @@ static int enqueue_search(dvc_ctx *ctx, hipStream_t st) {
 // level (a level's points only read results of earlier levels).  A level run as slice
 // launches takes slice_evals_of(ctx) launches of the slice kernel, then the
 // one-workgroup search resumes the points that are still searching.
-static int enqueue_search_only(dvc_ctx *ctx, hipStream_t st) {
-    const Geometry &G = ctx->geo;
-    const dvc_run_params &rc = ctx->rc;
-    SearchArgs a;
-    fill_search_args(a, rc, G, ctx->vol[0].bpv, (int)ctx->vol[0].W, (int)ctx->vol[0].H, (int)ctx->vol[0].T,
-                     ctx->npts, ctx->nproc, ctx->k, ctx->off_min, ctx->off_max);
+// The search kernels' arguments of a prepared search (serial chain: results and start
+// guesses in d_res).
+static void prepared_args(dvc_ctx *ctx, SearchArgs &a) {
+    fill_search_args(a, ctx->rc, ctx->geo, ctx->vol[0].bpv, (int)ctx->vol[0].W, (int)ctx->vol[0].H,
+                     (int)ctx->vol[0].T, ctx->npts, ctx->nproc, ctx->k, ctx->off_min, ctx->off_max);
     a.vol[0] = ctx->vol[0].d; a.vol[1] = ctx->vol[1].d;
     if (((uintptr_t)a.vol[0] | (uintptr_t)a.vol[1]) & 3) a.gather32 = 0;   // attached, unaligned
     a.sph = ctx->d_sph; a.pts = ctx->d_pts; a.order = ctx->d_order; a.pos = ctx->d_pos; a.neigh = ctx->d_neigh;
@@ -986,6 +994,33 @@ static int enqueue_search_only(dvc_ctx *ctx, hipStream_t st) {
     a.rvals = ctx->d_rvals;
     a.rstat = ctx->d_rstat;
     a.pgeo = ctx->d_pgeo;
+}
+
+// The judged mode's repair: the points where the passes do not both converge and agree
+// (h_repair, copied from the selection), searched again in the serial chain's launches
+// from the reference's start guess over the final results.  Only the launches that hold
+// such a point are enqueued; in them every other point leaves its workgroup at once.
+static int enqueue_repairs(dvc_ctx *ctx, hipStream_t st) {
+    SearchArgs a;
+    prepared_args(ctx, a);
+    a.repair = ctx->d_repair;
+    const int bt = ctx->chosen_threads;
+    const size_t lds = search_lds_bytes(a.nsamp_pad, bt, a.rvals == nullptr);
+    const int depth = (int)ctx->rep_off.size() - 1;
+    for (int L = 0; L < depth; L++) {
+        const int q0 = ctx->rep_off[L], q1 = ctx->rep_off[L + 1];
+        bool any = false;
+        for (int q = q0; q < q1 && !any; q++) any = ctx->h_repair[ctx->rep_pts_h[q]] != 0;
+        if (!any) continue;
+        if (launch_search(a, ctx->d_rep_levels + q0, bt, q1 - q0, lds, st) != 0)
+            return fail(ctx, DVC_ERR_HIP, "repair launch failed");
+    }
+    return DVC_OK;
+}
+
+static int enqueue_search_only(dvc_ctx *ctx, hipStream_t st) {
+    SearchArgs a;
+    prepared_args(ctx, a);
     int bt = ctx->chosen_threads;
     size_t lds = search_lds_bytes(a.nsamp_pad, bt, a.rvals == nullptr);
     const long long res_words = (long long)ctx->npts * (long long)(sizeof(dvc_point_result) / 4);
@@ -1045,18 +1080,6 @@ static int enqueue_search_only(dvc_ctx *ctx, hipStream_t st) {
         if (launch_judge(ctx->d_order, ctx->nproc, ctx->d_res_prop, ctx->d_res_judge, ctx->d_res, ctx->d_kept,
                          ctx->d_repair, st) != 0)
             return fail(ctx, DVC_ERR_HIP, "judge launch failed");
-        // repair: the points where the passes do not both converge and agree, searched
-        // again in the serial chain's launches from the reference's start guess over the
-        // final results (every other point leaves its launch at once)
-        a.repair = ctx->d_repair;
-        a.res = ctx->d_res;
-        a.nbres = ctx->d_res;
-        const int depth = (int)ctx->rep_off.size() - 1;
-        for (int L = 0; L < depth; L++) {
-            const int cnt = ctx->rep_off[L + 1] - ctx->rep_off[L];
-            if (launch_search(a, ctx->d_rep_levels + ctx->rep_off[L], bt, cnt, lds, st) != 0)
-                return fail(ctx, DVC_ERR_HIP, "repair launch failed");
-        }
     }
     return DVC_OK;
 }
@@ -1195,6 +1218,15 @@ int dvc_search_execute(dvc_ctx *ctx) {
         HIPCHK(hipGraphLaunch(ctx->gexec, ctx->stream));
     } else if (int e = enqueue_search(ctx, ctx->stream)) {
         return e;
+    }
+    if (ctx->plan_mode == DVC_START_JUDGED) {
+        // the points to repair, then the repair launches that hold one, and the strain fit
+        HIPCHK(hipMemcpyAsync(ctx->h_repair, ctx->d_repair, sizeof(int) * (size_t)ctx->npts, hipMemcpyDeviceToHost,
+                              ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        if (int e = enqueue_repairs(ctx, ctx->stream)) return e;
+        if (ctx->strain_ndp > 0)
+            if (int e = enqueue_strain(ctx, ctx->stream)) return e;
     }
     HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
     ctx->timing_pending = true;

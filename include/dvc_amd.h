@@ -205,8 +205,9 @@ int dvc_search_points(dvc_ctx *ctx, const dvc_run_params *rc, const double *poin
 /* The same loop split for callers that keep inputs resident: prepare uploads the
  * point cloud and its order/neighbour tables and derives the chain's launches (the
  * dependency levels, or launches of the round capacity, dvc_set_round_capacity);
- * execute enqueues them on the context stream without synchronising; fetch waits and
- * copies results back. */
+ * execute enqueues them on the context stream without synchronising (in the judged mode,
+ * DVC_START_JUDGED, it waits once, for the selection, and then enqueues only the repair
+ * launches that hold a point to repair); fetch waits and copies results back. */
 int dvc_search_prepare(dvc_ctx *ctx, const dvc_run_params *rc, const double *points,
                        int32_t npts, const int32_t *order, int32_t nproc,
                        const int32_t *neigh, int32_t k);
@@ -233,7 +234,8 @@ int dvc_set_point_geometry(dvc_ctx *ctx, const int32_t *geom, const int32_t *siz
  *                      passes converged and agree (translations within 1e-3 voxel,
  *                      objectives within 1e-5 relative) the judge's result is the point's.
  *                      Every other point is searched again (repair) in the serial chain's
- *                      launches, from the reference's start guess over the final results.
+ *                      launches, from the reference's start guess over the final results;
+ *                      of those launches only the ones that hold such a point run.
  *                      Every result is thus the plain search from the reference's start
  *                      guess; the given starts only make the neighbours' results available
  *                      at once.
