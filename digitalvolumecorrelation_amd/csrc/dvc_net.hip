@@ -347,29 +347,41 @@ __global__ void __launch_bounds__(256) gemm_nt_kernel(GemmArgs g) {
 // channels-last input [B][D][D][D][C] (stride 2, padding 1), so the patch matrix is
 // never stored.  C a multiple of 8: one 16-byte load per lane and k-step (8 channels
 // of one tap); C = 2: four 4-byte loads (2 channels of 4 taps).  Taps past 27 and
-// voxels outside the input read as zero.  Epilogue: bias, ReLU, BF16.
-template <int NB, int C>
+// voxels outside the input read as zero.  Each wave computes RT consecutive 16-row
+// tiles, so a k-step's weight fragments serve RT products and RT gathers are in flight
+// together; every output still sums its k-steps in the same order.  Epilogue: bias,
+// ReLU, BF16.
+template <int NB, int C, int RT = 1>
 __global__ void __launch_bounds__(256) conv_fwd_kernel(const uint16_t *in, int B, int D, const uint16_t *Wb, int Kp,
                                                        int N, const float *bias, uint16_t *out) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int Do = D >> 1;
     const int M = B * Do * Do * Do;
-    const int row0 = blockIdx.x * 64 + wave * 16;
+    const int row0 = blockIdx.x * 64 * RT + wave * 16 * RT;
     const int col0 = blockIdx.y * (16 * NB);
     const int kq = (lane >> 4) * 8;
-    const int arow = row0 + (lane & 15);
-    const bool aok = arow < M;
-    int b = 0, oz = 0, oy = 0, ox = 0;
-    if (aok) {
-        const int d3 = Do * Do * Do;
-        b = arow / d3;
-        const int r = arow - b * d3;
-        oz = r / (Do * Do); oy = (r / Do) % Do; ox = r % Do;
-    }
-    const uint16_t *base = in + (long long)b * D * D * D * C;
-    floatx4 acc[NB];
+    bool aok[RT];
+    int oz[RT], oy[RT], ox[RT];
+    const uint16_t *base[RT];
 #pragma unroll
-    for (int t = 0; t < NB; t++) acc[t] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int rt = 0; rt < RT; rt++) {
+        const int arow = row0 + rt * 16 + (lane & 15);
+        aok[rt] = arow < M;
+        int b = 0;
+        oz[rt] = oy[rt] = ox[rt] = 0;
+        if (aok[rt]) {
+            const int d3 = Do * Do * Do;
+            b = arow / d3;
+            const int r = arow - b * d3;
+            oz[rt] = r / (Do * Do); oy[rt] = (r / Do) % Do; ox[rt] = r % Do;
+        }
+        base[rt] = in + (long long)b * D * D * D * C;
+    }
+    floatx4 acc[RT][NB];
+#pragma unroll
+    for (int rt = 0; rt < RT; rt++)
+#pragma unroll
+        for (int t = 0; t < NB; t++) acc[rt][t] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
     const uint4 zero = {0u, 0u, 0u, 0u};
     const uint16_t *bp[NB];
     bool bok[NB];
@@ -379,47 +391,54 @@ __global__ void __launch_bounds__(256) conv_fwd_kernel(const uint16_t *in, int B
         bok[t] = ncol < N;
         bp[t] = Wb + (long long)(bok[t] ? ncol : 0) * Kp + kq;
     }
-    auto voxel_ptr = [&](int tap, bool &ok) -> const uint16_t * {
+    auto voxel_ptr = [&](int rt, int tap, bool &ok) -> const uint16_t * {
         const int kz = tap / 9, ky = (tap / 3) % 3, kx = tap % 3;
-        const int z = 2 * oz + kz - 1, y = 2 * oy + ky - 1, x = 2 * ox + kx - 1;
-        ok = aok && tap < 27 && z >= 0 && z < D && y >= 0 && y < D && x >= 0 && x < D;
-        return base + (((long long)z * D + y) * D + x) * C;
+        const int z = 2 * oz[rt] + kz - 1, y = 2 * oy[rt] + ky - 1, x = 2 * ox[rt] + kx - 1;
+        ok = aok[rt] && tap < 27 && z >= 0 && z < D && y >= 0 && y < D && x >= 0 && x < D;
+        return base[rt] + (((long long)z * D + y) * D + x) * C;
     };
 #pragma unroll 4
     for (int k = 0; k < Kp; k += 32) {
-        uint4 av;
-        if (C % 8 == 0) {
-            const int kk = k + kq;
-            bool ok;
-            const uint16_t *ptr = voxel_ptr(kk / C, ok);
-            av = ok ? *(const uint4 *)(ptr + (kk % C)) : zero;
-        } else {
-            uint32_t w[4];
+        uint4 bv[NB];
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
+        for (int t = 0; t < NB; t++) bv[t] = bok[t] ? *(const uint4 *)(bp[t] + k) : zero;
+#pragma unroll
+        for (int rt = 0; rt < RT; rt++) {
+            uint4 av;
+            if (C % 8 == 0) {
+                const int kk = k + kq;
                 bool ok;
-                const uint16_t *ptr = voxel_ptr((k + kq) / 2 + j, ok);
-                w[j] = ok ? *(const uint32_t *)ptr : 0u;
-            }
-            av = uint4{w[0], w[1], w[2], w[3]};
-        }
-        const bf16x8 a8 = __builtin_bit_cast(bf16x8, av);
+                const uint16_t *ptr = voxel_ptr(rt, kk / C, ok);
+                av = ok ? *(const uint4 *)(ptr + (kk % C)) : zero;
+            } else {
+                uint32_t w[4];
 #pragma unroll
-        for (int t = 0; t < NB; t++) {
-            const uint4 bv = bok[t] ? *(const uint4 *)(bp[t] + k) : zero;
-            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a8, __builtin_bit_cast(bf16x8, bv), acc[t], 0, 0, 0);
+                for (int j = 0; j < 4; j++) {
+                    bool ok;
+                    const uint16_t *ptr = voxel_ptr(rt, (k + kq) / 2 + j, ok);
+                    w[j] = ok ? *(const uint32_t *)ptr : 0u;
+                }
+                av = uint4{w[0], w[1], w[2], w[3]};
+            }
+            const bf16x8 a8 = __builtin_bit_cast(bf16x8, av);
+#pragma unroll
+            for (int t = 0; t < NB; t++)
+                acc[rt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a8, __builtin_bit_cast(bf16x8, bv[t]), acc[rt][t],
+                                                                     0, 0, 0);
         }
     }
 #pragma unroll
-    for (int t = 0; t < NB; t++)
+    for (int rt = 0; rt < RT; rt++)
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int m = row0 + 4 * (lane >> 4) + r;
-            const int n = col0 + t * 16 + (lane & 15);
-            if (m >= M || n >= N) continue;
-            float v = acc[t][r] + bias[n];
-            out[(long long)m * N + n] = f2bf(v > 0.0f ? v : 0.0f);
-        }
+        for (int t = 0; t < NB; t++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int m = row0 + rt * 16 + 4 * (lane >> 4) + r;
+                const int n = col0 + t * 16 + (lane & 15);
+                if (m >= M || n >= N) continue;
+                float v = acc[rt][t][r] + bias[n];
+                out[(long long)m * N + n] = f2bf(v > 0.0f ? v : 0.0f);
+            }
 }
 
 // dst[o + m*ldd + n] = sum_z part[z][m][n] for n < ncols (drops padding columns)
@@ -789,14 +808,15 @@ int conv_fwd(dvc_net *n, hipStream_t st, int l, int B) {
     const long long M = layer_rows(l, B);
     if (M > 2147483647LL) return net_fail(n, DVC_ERR_ARG, "batch too large");
     const int nb = N >= 64 ? 4 : (N >= 32 ? 2 : 1);
-    dim3 grid((unsigned)((M + 63) / 64), (unsigned)((N + 16 * nb - 1) / (16 * nb)));
+    const int rt = l == 1 ? 4 : (l == 2 ? 2 : 1);   // row tiles per wave (8^3 and 4^3 outputs per crop)
+    dim3 grid((unsigned)((M + 64 * rt - 1) / (64 * rt)), (unsigned)((N + 16 * nb - 1) / (16 * nb)));
     const uint16_t *in = n->act[l];
     const float *bias = n->p + n->L.b[l];
     uint16_t *out = n->act[l + 1];
     switch (l) {
         case 0: hipLaunchKernelGGL((conv_fwd_kernel<1, 2>), grid, dim3(256), 0, st, in, B, D, n->wb[l], Kp, N, bias, out); break;
-        case 1: hipLaunchKernelGGL((conv_fwd_kernel<2, 16>), grid, dim3(256), 0, st, in, B, D, n->wb[l], Kp, N, bias, out); break;
-        case 2: hipLaunchKernelGGL((conv_fwd_kernel<4, 32>), grid, dim3(256), 0, st, in, B, D, n->wb[l], Kp, N, bias, out); break;
+        case 1: hipLaunchKernelGGL((conv_fwd_kernel<2, 16, 4>), grid, dim3(256), 0, st, in, B, D, n->wb[l], Kp, N, bias, out); break;
+        case 2: hipLaunchKernelGGL((conv_fwd_kernel<4, 32, 2>), grid, dim3(256), 0, st, in, B, D, n->wb[l], Kp, N, bias, out); break;
         default: hipLaunchKernelGGL((conv_fwd_kernel<4, 64>), grid, dim3(256), 0, st, in, B, D, n->wb[l], Kp, N, bias, out); break;
     }
     return launch_ok(n, "conv forward");
