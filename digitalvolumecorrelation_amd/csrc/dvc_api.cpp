@@ -1000,6 +1000,9 @@ static void prepared_args(dvc_ctx *ctx, SearchArgs &a) {
 // (h_repair, copied from the selection), searched again in the serial chain's launches
 // from the reference's start guess over the final results.  Only the launches that hold
 // such a point are enqueued; in them every other point leaves its workgroup at once.
+// Where the serial chain would slice a launch (same rules as plan_slices), the repair
+// runs it as slice launches too: a repaired point's search then spreads over several
+// CUs instead of one workgroup.
 static int enqueue_repairs(dvc_ctx *ctx, hipStream_t st) {
     SearchArgs a;
     prepared_args(ctx, a);
@@ -1007,13 +1010,47 @@ static int enqueue_repairs(dvc_ctx *ctx, hipStream_t st) {
     const int bt = ctx->chosen_threads;
     const size_t lds = search_lds_bytes(a.nsamp_pad, bt, a.rvals == nullptr);
     const int depth = (int)ctx->rep_off.size() - 1;
+    const bool sliceable = ctx->pt_nsamp.empty() && ctx->d_rvals && ctx->rc.basin_radius == 0.0 && ctx->slice_wgs >= 0;
+    const int nsl0 = (ctx->geo.subv_num + ctx->slice_per - 1) / ctx->slice_per;
+    const int max_wgs = ctx->slice_wgs > 0 ? ctx->slice_wgs : 4 * ctx->cus;
+    const int cus_xcd = std::max(1, ctx->cus / kXcds);
     for (int L = 0; L < depth; L++) {
         const int q0 = ctx->rep_off[L], q1 = ctx->rep_off[L + 1];
         bool any = false;
         for (int q = q0; q < q1 && !any; q++) any = ctx->h_repair[ctx->rep_pts_h[q]] != 0;
         if (!any) continue;
-        if (launch_search(a, ctx->d_rep_levels + q0, bt, q1 - q0, lds, st) != 0)
+        const int cnt = q1 - q0, cnt8 = (cnt + kXcds - 1) / kXcds * kXcds;
+        int nsl = nsl0;
+        const int cap = cus_xcd / std::max(1, cnt8 / kXcds);
+        if (cap >= 2 && nsl > cap) nsl = cap;
+        if (sliceable && nsl >= 2 && (long long)cnt8 * nsl <= max_wgs) {
+            if (cnt8 > ctx->slice_cnt8 || nsl0 > ctx->slice_n || !ctx->d_state) {
+                const int c8 = std::max(cnt8, ctx->slice_cnt8), ns = std::max(nsl0, ctx->slice_n);
+                if (int e = dev_alloc(ctx, &ctx->d_state, (size_t)2 * c8 * kStateStride)) return e;
+                if (int e = dev_alloc(ctx, &ctx->d_part, (size_t)2 * c8 * ns * kPartStride)) return e;
+                ctx->slice_cnt8 = c8;
+                ctx->slice_n = ns;
+            }
+            SliceArgs sa{};
+            sa.per = ctx->slice_per;
+            for (int q = 0; q < 2; q++) {
+                sa.state[q] = ctx->d_state + (size_t)q * ctx->slice_cnt8 * kStateStride;
+                sa.part[q] = ctx->d_part + (size_t)q * ctx->slice_cnt8 * ctx->slice_n * kPartStride;
+            }
+            sa.level_pts = ctx->d_rep_levels + q0;
+            sa.cnt = cnt;
+            sa.cnt8 = cnt8;
+            sa.nslice = nsl;
+            for (int t = 0; t < slice_evals_of(ctx); t++) {
+                sa.t = t;
+                if (launch_slice(a, sa, slice_threads_of(ctx), st) != 0)
+                    return fail(ctx, DVC_ERR_HIP, "repair slice launch failed");
+            }
+            sa.t = slice_evals_of(ctx);
+            if (launch_resume(a, sa, st) != 0) return fail(ctx, DVC_ERR_HIP, "repair resume launch failed");
+        } else if (launch_search(a, ctx->d_rep_levels + q0, bt, cnt, lds, st) != 0) {
             return fail(ctx, DVC_ERR_HIP, "repair launch failed");
+        }
     }
     return DVC_OK;
 }

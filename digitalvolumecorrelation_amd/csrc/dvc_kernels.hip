@@ -2011,6 +2011,16 @@ __device__ __forceinline__ void slice_body(const SearchArgs &a, const SliceArgs 
         state_store<BS>(S, gst);
     };
     if (sa.t == 0) {
+        if (a.repair && !a.repair[n]) {
+            // a judged search's repair launch: this point's result stands; its state
+            // only carries "finished" to the later launches
+            if (slice == 0) {
+                if (tid == 0) S.done = 1;
+                __syncthreads();
+                state_store<BS>(S, gst);
+            }
+            return;
+        }
         const float *rs = nullptr;
         if (!point_start<BPV, BS, false>(a, p, n, S, nullptr, stage, red, V, rs)) {
             finished();
