@@ -235,7 +235,8 @@ int dvc_set_point_geometry(dvc_ctx *ctx, const int32_t *geom, const int32_t *siz
  *                      objectives within 1e-5 relative) the judge's result is the point's.
  *                      Every other point is searched again (repair) in the serial chain's
  *                      launches, from the reference's start guess over the final results;
- *                      of those launches only the ones that hold such a point run.
+ *                      of those launches only the ones that hold such a point run (as
+ *                      slice launches where the chain would slice them).
  *                      Every result is thus the plain search from the reference's start
  *                      guess; the given starts only make the neighbours' results available
  *                      at once.
