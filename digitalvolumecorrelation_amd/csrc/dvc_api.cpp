@@ -875,6 +875,41 @@ static int slices_of(const dvc_ctx *ctx) {
 // Threads per slice workgroup: one sample per thread up to 512, more above.
 static int slice_threads_of(const dvc_ctx *ctx) { return ctx->slice_per >= kSliceThreads[1] ? kSliceThreads[1] : kSliceThreads[0]; }
 
+// Workgroups per point of a chain launch run as slice launches (points pts[0..cnt), in
+// workgroup-slot order): one per slice_per samples of its largest point.  A point's
+// slices share one XCD (its slot mod 8): with more of them than the XCD has CUs, some
+// CUs would run two slices one after the other, so wide launches take fewer, larger
+// slices (the kernels deal a point's 64-sample chunks to however many slices it has).
+// With per-point geometry *cap is the largest cap c >= 2 on every point's slices that
+// leaves no XCD more slices than CUs (0: none needed, or even 2 does not fit).
+static int launch_slices(const dvc_ctx *ctx, const int *pts, int cnt, int *cap) {
+    const bool uneven = !ctx->pt_nsamp.empty();
+    auto slices = [&](int pt) {
+        return ((uneven ? ctx->pt_nsamp[pt] : ctx->geo.subv_num) + ctx->slice_per - 1) / ctx->slice_per;
+    };
+    int nsl = 0;
+    for (int q = 0; q < cnt; q++) nsl = std::max(nsl, slices(pts[q]));
+    *cap = 0;
+    const int cus_xcd = std::max(1, ctx->cus / kXcds);
+    if (!uneven) {
+        const int c = cus_xcd / std::max(1, (cnt + kXcds - 1) / kXcds);
+        return c >= 2 && nsl > c ? c : nsl;
+    }
+    auto fits = [&](int c) {
+        int load[kXcds] = {0};
+        for (int q = 0; q < cnt; q++) load[q % kXcds] += std::min(slices(pts[q]), c);
+        for (int x = 0; x < kXcds; x++)
+            if (load[x] > cus_xcd) return false;
+        return true;
+    };
+    if (fits(nsl)) return nsl;
+    int c = nsl - 1;
+    while (c >= 2 && !fits(c)) c--;
+    if (c < 2) return nsl;
+    *cap = c;
+    return c;
+}
+
 // Which chain launches of the prepared search run as slice launches, their workgroups per
 // point, and their buffers.  Slice launches serve the serial chain with no translation
 // grid and the HBM table of reference samples.  With per-point geometry a chain launch
@@ -889,48 +924,9 @@ static int plan_slices(dvc_ctx *ctx) {
                           ctx->slice_wgs >= 0;
     if (!eligible) return DVC_OK;
     const int max_wgs = ctx->slice_wgs > 0 ? ctx->slice_wgs : 4 * ctx->cus;
-    for (int L = 0; L < nl; L++) {
-        int ns = ctx->geo.subv_num;
-        if (!ctx->pt_nsamp.empty()) {
-            ns = 0;
-            for (int q = ctx->level_off[L]; q < ctx->level_off[L + 1]; q++)
-                ns = std::max(ns, ctx->pt_nsamp[ctx->level_pts_h[q]]);
-        }
-        ctx->slice_nsl[L] = (ns + ctx->slice_per - 1) / ctx->slice_per;
-        // a point's slices share one XCD (the point's slot within the launch, mod 8):
-        // with more of them than the XCD has CUs, some CUs would run two slices one after
-        // the other, so wide launches take fewer, larger slices (the kernels deal a
-        // point's 64-sample chunks to however many slices it has)
-        const int cnt = ctx->level_off[L + 1] - ctx->level_off[L];
-        const int cus_xcd = std::max(1, ctx->cus / kXcds);
-        if (ctx->pt_nsamp.empty()) {
-            const int per_xcd = (cnt + kXcds - 1) / kXcds;
-            const int cap = cus_xcd / std::max(1, per_xcd);
-            if (cap >= 2 && ctx->slice_nsl[L] > cap) ctx->slice_nsl[L] = cap;
-        } else {
-            // per-point geometry: the largest cap c >= 2 on every point's slices that
-            // leaves no XCD more slices than CUs; none when even 2 does not fit
-            auto fits = [&](int c) {
-                int load[kXcds] = {0};
-                for (int q = 0; q < cnt; q++) {
-                    const int sl = (ctx->pt_nsamp[ctx->level_pts_h[ctx->level_off[L] + q]] + ctx->slice_per - 1) /
-                                   ctx->slice_per;
-                    load[q % kXcds] += std::min(sl, c);
-                }
-                for (int x = 0; x < kXcds; x++)
-                    if (load[x] > cus_xcd) return false;
-                return true;
-            };
-            if (!fits(ctx->slice_nsl[L])) {
-                int c = ctx->slice_nsl[L] - 1;
-                while (c >= 2 && !fits(c)) c--;
-                if (c >= 2) {
-                    ctx->slice_cap[L] = c;
-                    ctx->slice_nsl[L] = c;
-                }
-            }
-        }
-    }
+    for (int L = 0; L < nl; L++)
+        ctx->slice_nsl[L] = launch_slices(ctx, ctx->level_pts_h.data() + ctx->level_off[L],
+                                          ctx->level_off[L + 1] - ctx->level_off[L], &ctx->slice_cap[L]);
     const int cnt8 = slice_plan(ctx->level_off, ctx->slice_nsl, max_wgs, kXcds, ctx->sliced);
     if (cnt8 == 0) return DVC_OK;
     const int nslice = slices_of(ctx);
@@ -1010,19 +1006,17 @@ static int enqueue_repairs(dvc_ctx *ctx, hipStream_t st) {
     const int bt = ctx->chosen_threads;
     const size_t lds = search_lds_bytes(a.nsamp_pad, bt, a.rvals == nullptr);
     const int depth = (int)ctx->rep_off.size() - 1;
-    const bool sliceable = ctx->pt_nsamp.empty() && ctx->d_rvals && ctx->rc.basin_radius == 0.0 && ctx->slice_wgs >= 0;
-    const int nsl0 = (ctx->geo.subv_num + ctx->slice_per - 1) / ctx->slice_per;
+    const bool sliceable = ctx->d_rvals && ctx->rc.basin_radius == 0.0 && ctx->slice_wgs >= 0;
+    const int nsl0 = slices_of(ctx);
     const int max_wgs = ctx->slice_wgs > 0 ? ctx->slice_wgs : 4 * ctx->cus;
-    const int cus_xcd = std::max(1, ctx->cus / kXcds);
     for (int L = 0; L < depth; L++) {
         const int q0 = ctx->rep_off[L], q1 = ctx->rep_off[L + 1];
         bool any = false;
         for (int q = q0; q < q1 && !any; q++) any = ctx->h_repair[ctx->rep_pts_h[q]] != 0;
         if (!any) continue;
         const int cnt = q1 - q0, cnt8 = (cnt + kXcds - 1) / kXcds * kXcds;
-        int nsl = nsl0;
-        const int cap = cus_xcd / std::max(1, cnt8 / kXcds);
-        if (cap >= 2 && nsl > cap) nsl = cap;
+        int cap = 0;
+        const int nsl = launch_slices(ctx, ctx->rep_pts_h.data() + q0, cnt, &cap);
         if (sliceable && nsl >= 2 && (long long)cnt8 * nsl <= max_wgs) {
             if (cnt8 > ctx->slice_cnt8 || nsl0 > ctx->slice_n || !ctx->d_state) {
                 const int c8 = std::max(cnt8, ctx->slice_cnt8), ns = std::max(nsl0, ctx->slice_n);
@@ -1041,6 +1035,7 @@ static int enqueue_repairs(dvc_ctx *ctx, hipStream_t st) {
             sa.cnt = cnt;
             sa.cnt8 = cnt8;
             sa.nslice = nsl;
+            sa.maxsl = cap;
             for (int t = 0; t < slice_evals_of(ctx); t++) {
                 sa.t = t;
                 if (launch_slice(a, sa, slice_threads_of(ctx), st) != 0)
