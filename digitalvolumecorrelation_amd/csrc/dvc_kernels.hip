@@ -528,6 +528,7 @@ struct PtShared {
     // start guess inputs, not part of the carried state
     double nb_sum[3];                // translations of the good earlier neighbours, summed
     int nb_good;                     // their number
+    double dA[3][9];                 // d A / d angle of the current warp (set_warp to set_warp_rows)
 };
 constexpr int kStateBytes = (int)offsetof(PtShared, nb_sum);
 static_assert(kStateBytes % 8 == 0, "carried point state is copied in 8-byte words");
@@ -687,19 +688,19 @@ __device__ void rotation(const double trig[6], double A[9], double dA[3][9]) {
 }
 
 // Warp state for parameter vector evalpar (thread 0), given the cosines and sines of
-// its three angles (trig).  SearchParams.cpp:53-107.
+// its three angles (trig): translation, rotation A, strain Sm and, in S.dA, the rotation's
+// derivatives for set_warp_rows.  SearchParams.cpp:53-107.
 __device__ __forceinline__ void set_warp(PtShared &S, int ndof, const double trig[6]) {
 #pragma clang fp contract(off)
     for (int q = 0; q < 3; q++) {
         S.u[q] = S.evalpar[q];
         S.mref[q] = S.sref[q] + S.u[q];
     }
-    double dA[3][9];
     if (ndof > 3) {
-        rotation(trig, S.A, dA);
+        rotation(trig, S.A, S.dA);
     } else {
         for (int q = 0; q < 9; q++) S.A[q] = (q % 4 == 0) ? 1.0 : 0.0;
-        for (int r = 0; r < 3; r++) for (int q = 0; q < 9; q++) dA[r][q] = 0.0;
+        for (int r = 0; r < 3; r++) for (int q = 0; q < 9; q++) S.dA[r][q] = 0.0;
     }
     for (int q = 0; q < 9; q++) S.Sm[q] = 0.0;
     if (ndof > 6) {
@@ -708,22 +709,19 @@ __device__ __forceinline__ void set_warp(PtShared &S, int ndof, const double tri
         S.Sm[3] = v[9];  S.Sm[4] = v[7];  S.Sm[5] = v[10];
         S.Sm[6] = v[11]; S.Sm[7] = v[10]; S.Sm[8] = v[8];
     }
-    // d x'/d angle = (I+S) dA rel, for the FP32 Jacobian rows (formed in FP32)
-    if (ndof > 6) {
-        float sm[9];
-        for (int q = 0; q < 9; q++) sm[q] = (float)S.Sm[q];
-        for (int r = 0; r < 3; r++)
-            for (int i = 0; i < 3; i++)
-                for (int j = 0; j < 3; j++) {
-                    float s = (float)dA[r][i * 3 + j];
-                    for (int l = 0; l < 3; l++) s = __builtin_fmaf(sm[i * 3 + l], (float)dA[r][l * 3 + j], s);
-                    S.M[r][i * 3 + j] = s;
-                }
-    } else {
-        for (int r = 0; r < 3; r++)
-            for (int q = 0; q < 9; q++) S.M[r][q] = (float)dA[r][q];
+}
+
+// The FP32 factors of the Jacobian rows after set_warp, one entry per lane (lanes 0-26
+// of wavefront 0): d x'/d angle = (I+S) dA rel, formed in FP32, and A in FP32.
+__device__ __forceinline__ void set_warp_rows(PtShared &S, int ndof, int lane) {
+    if (lane < 27) {
+        const int r = lane / 9, q = lane % 9, i = q / 3, j = q % 3;
+        float s = (float)S.dA[r][q];
+        if (ndof > 6)
+            for (int l = 0; l < 3; l++) s = __builtin_fmaf((float)S.Sm[i * 3 + l], (float)S.dA[r][l * 3 + j], s);
+        S.M[r][q] = s;
     }
-    for (int q = 0; q < 9; q++) S.Af[q] = (float)S.A[q];
+    if (lane < 9) S.Af[lane] = (float)S.A[lane];
 }
 
 // ------------------------------------------------------------- evaluation
@@ -769,6 +767,10 @@ __device__ __forceinline__ bool eval_begin(const SearchArgs &a, PtShared &S, int
             S.pfail = pfail;
             S.fail = fail;
         }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        set_warp_rows(S, ndof, lane);
     }
     __syncthreads();
     return S.fail == 0;
